@@ -1,0 +1,76 @@
+"""Headline benchmark: steps/s of the periodic-RPY (Smith-Ewald) Brownian
+dynamics step at N = 1000 beads (a = 10 A, phi = 0.05, L = 437.6 A,
+alpha = sqrt(pi), |m|_1, |n|_1 <= 2, Ermak-McCammon, dt = 1 ps, flux plane
+x = 0), FP64, on one MI355X.
+
+A step is one Box.propagate: Ewald mobility assembly, Cholesky of the
+3000 x 3000 diffusion matrix, B.xi propagation with overlap check, flux and
+wrap; positions and matrices are resident in HBM, the host supplies the
+seeded N(0,1) stream.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    from pybrown_amd.workloads import workload
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    xyz, a, inp, dt, _ = workload(args.workload, seed=0)
+    box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp, xi_pool_attempts=64)
+    for _ in range(args.warmup):
+        box.propagate(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        box.propagate(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    out = {
+        "metric": "steps/s at N=1000 RPY-Ewald",
+        "value": 1e3 / ms,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "rpy_smith N=1000 a=10 phi=0.05 L=437.6 alpha=sqrt(pi) m=n=2 ermak dt=1ps",
+                   "N": len(a), "box_length": inp["box_length"]},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

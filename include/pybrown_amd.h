@@ -1,0 +1,144 @@
+/*
+ * pybrown_amd.h -- C ABI of libpybrown_amd.so, the MI355X (gfx950) engine for
+ * pyBrown's hydrodynamic step.
+ *
+ * Reference citations are relative to the pyBrown source tree (src/bdsim/).
+ * Matrices crossing this boundary are dense, row-major, 3N x 3N doubles (the
+ * layout pyBrown obtains with numpy.reshape(..., (3N, 3N)),
+ * pyBrown/diffusion.py:54).  All pointers are host pointers unless a name
+ * ends in _dev.
+ */
+#ifndef PYBROWN_AMD_H
+#define PYBROWN_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * 1. Drop-in replacements for libpyBrown.so.  Same symbol names, argument
+ *    order, meaning and (absence of) return codes as the reference; pyBrown
+ *    loads them with ctypes.cdll.LoadLibrary (pyBrown/diffusion.py:24-26,
+ *    pyBrown/bead.py:24-26).  On a HIP failure they print a message to
+ *    stderr and leave it in pbx_last_error(); the Python layer raises.
+ * ------------------------------------------------------------------------ */
+
+/* clib/diff_tensor.c:73  -- free-space RPY mobility (overlapping and contained
+ * beads included).  `pointers` is the packed pair list -rij[p][q], p<q, of
+ * pyBrown/diffusion.py:40.  Writes all 9N^2 entries of M_matrix. */
+void RPY_M_matrix(double* as, double* pointers, int number_of_beads, double* M_matrix);
+
+/* clib/diff_tensor.c:170 -- periodic RPY by Smith-Ewald summation; real-space
+ * images with |m|_1 <= m_max, reciprocal vectors with 0 < |n|_1 <= n_max
+ * (n_max <= 8); alpha in box units. */
+void RPY_Smith_M_matrix(double* as, double* pointers, double box_length, double alpha, int m_max, int n_max,
+                        int number_of_beads, double* M_matrix);
+
+/* clib/diff_tensor.c:270 -- Jeffrey-Onishi two-sphere resistance, 18 packed
+ * values (blocks 00, 11, 10 as xx,yy,zz,xy,xz,yz). */
+void JO_2B_R_matrix(double ai, double aj, double rx, double ry, double rz, double* R_matrix);
+
+/* clib/diff_tensor.c:346 -- lubrication correction; ACCUMULATES (+=) into
+ * correction_matrix like the reference. */
+void JO_R_lubrication_correction_matrix(double* as, double* pointers, int number_of_beads, double cutoff_distance,
+                                        int cichocki_correction, double* correction_matrix);
+
+/* clib/bead.c:27 -- minimum-image pointer matrix, N x N x 3; diagonal entries
+ * keep the caller's values. */
+void pointer_pbc_matrix(double* positions, int number_of_beads, double box_length, double* pointers);
+
+/* clib/bead.c:169 -- 1 if any non-bonded pair overlaps, else 0 (-1 on a HIP
+ * error).  connection_matrix is N x N int32 (may be NULL = no bonds). */
+int check_overlaps(double* positions, double* as, int number_of_beads, double box_length, double overlap_treshold,
+                   int* connection_matrix);
+
+/* ------------------------------------------------------------------------
+ * 2. Dense kernels at the reference's numpy.linalg call sites.
+ * ------------------------------------------------------------------------ */
+
+/* pyBrown/box.py:554 np.linalg.cholesky -- lower factor, upper triangle set
+ * to zero.  Returns 0, or k > 0 when the pivot at column k-1 is not
+ * positive (numpy raises LinAlgError there), or a negative error. */
+int pbx_cholesky(const double* A, int n, double* L);
+
+/* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
+int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
+
+/* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
+ * FP64 MFMA tile product used by the factorisations, for testing. */
+int pbx_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K);
+
+/* ------------------------------------------------------------------------
+ * 3. Device-resident simulation: Box(beads, input_data) and
+ *    Box.propagate(dt, build_Dff, build_Dnf, cholesky)
+ *    (pyBrown/box.py:42, :150).  One HIP stream per handle; positions,
+ *    matrices and factors stay in HBM between steps.
+ * ------------------------------------------------------------------------ */
+
+enum { PBX_NOHI = 0, PBX_RPY = 1, PBX_RPY_SMITH = 2, PBX_RPY_LUB = 3, PBX_RPY_SMITH_LUB = 4 };
+enum { PBX_ERMAK = 0, PBX_MIDPOINT = 1 };
+
+/* The JSON keywords of BD.py:37-64 that reach the step. */
+typedef struct pbx_config {
+    int hydrodynamics;          /* PBX_NOHI .. PBX_RPY_SMITH_LUB            */
+    int scheme;                 /* propagation_scheme: PBX_ERMAK/MIDPOINT    */
+    int ewald_real;             /* m_max                                     */
+    int ewald_imag;             /* n_max                                     */
+    int cichocki_correction;
+    int check_overlaps;
+    int measure_flux;
+    int n_labels;               /* distinct bead labels (flux counters)     */
+    long long max_move_attempts;
+    int xi_pool_attempts;       /* noise draws fetched per refill (0 = 64)  */
+    int reserved;
+    double box_length, T, viscosity;
+    double ewald_alpha;
+    double lubrication_cutoff;
+    double m_midpoint;
+    double overlap_treshold;    /* spelling of the reference keyword        */
+    double flux_normal[3];
+    double flux_plane_point[3];
+    double external_force[3];
+} pbx_config;
+
+typedef struct PbxSim pbx_sim;
+
+/* Fills out[0..count) with successive N(0,1) draws; returns 0 on success.
+ * The engine calls it whenever its noise pool is empty. */
+typedef int (*pbx_xi_fill_fn)(void* user, double* out, long long count);
+
+/* Box.__init__ (box.py:42-109): uploads radii, positions (N x 3) and label
+ * indices; fails with PBX_ERR_OVERLAP (-5) for an overlapping start. */
+int pbx_sim_create(const pbx_config* cfg, int N, const double* radii, const double* xyz, const int* label_idx,
+                   pbx_sim** out);
+int pbx_sim_destroy(pbx_sim* sim);
+int pbx_sim_set_noise_source(pbx_sim* sim, pbx_xi_fill_fn fn, void* user);
+
+/* Box.propagate (box.py:150-198).  Returns 0; k > 0 for a non-SPD matrix
+ * (pivot k-1); -4 when max_move_attempts is exhausted; other negatives for
+ * argument/HIP errors. */
+int pbx_sim_propagate(pbx_sim* sim, double dt, int build_Dff, int build_Dnf, int cholesky);
+
+int pbx_sim_get_positions(pbx_sim* sim, double* xyz);
+int pbx_sim_set_positions(pbx_sim* sim, const double* xyz);
+int pbx_sim_get_flux(pbx_sim* sim, long long* per_label);
+/* which: 0 = D, 1 = B (lower, upper zeroed), 2 = far-field resistance Rff */
+int pbx_sim_get_matrix(pbx_sim* sim, int which, double* out);
+/* hydrodynamics nohi: which 0 = diagonal D, 1 = sqrt(D) (3N values) */
+int pbx_sim_get_diag(pbx_sim* sim, int which, double* out);
+long long pbx_sim_draw_count(pbx_sim* sim);
+int pbx_sim_last_attempts(pbx_sim* sim);
+int pbx_sim_sync(pbx_sim* sim);
+
+/* ------------------------------------------------------------------------
+ * 4. Errors
+ * ------------------------------------------------------------------------ */
+const char* pbx_last_error(void);
+int pbx_last_error_code(void);
+void pbx_clear_error(void);
+const char* pbx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

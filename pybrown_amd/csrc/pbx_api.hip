@@ -1,0 +1,775 @@
+// C ABI of libpybrown_amd.so (declared in include/pybrown_amd.h).
+//
+// Two layers:
+//  1. Stateless entry points with the reference's exact names, signatures and
+//     semantics (host pointers in and out; src/bdsim/clib/diff_tensor.c and
+//     bead.c), so pyBrown's ctypes bindings can load this library unchanged.
+//  2. A device-resident simulation handle (pbx_sim_*) that keeps positions,
+//     matrices and factors in HBM and performs Box.propagate
+//     (src/bdsim/pyBrown/box.py:150-349) with HIP kernels; the host supplies
+//     N(0,1) draws through a callback so the reference's RandomState stream
+//     is consumed attempt by attempt exactly as box.py consumes it.
+#include "pbx_common.h"
+#include "../../include/pybrown_amd.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+// launchers from pbx_step.hip
+int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, double sq2dt, double mult,
+                     double* pos, double* disp, int N, const double* plane_n, const double* plane_p, int flux_on,
+                     const int* label, long long* flux, hipStream_t s);
+int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, double k, double mult, double* pos,
+                   int N, const double* plane_n, const double* plane_p, int flux_on, const int* label,
+                   long long* flux, hipStream_t s);
+int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
+                         const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s);
+int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s);
+int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const double* x, const double* z, double alpha,
+                     double* y, hipStream_t s);
+int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
+                         hipStream_t s);
+int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, hipStream_t s);
+int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
+                             int cich, double mul, double div, double* C, int ld, hipStream_t s);
+int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
+                    double scale);
+
+// ---------------------------------------------------------------------------
+// error state
+
+static thread_local int g_err_code = 0;
+static thread_local char g_err_msg[512] = "";
+
+extern "C" void pbx_set_error(int code, const char* fmt, ...) {
+    g_err_code = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err_msg, sizeof(g_err_msg), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char* pbx_last_error(void) { return g_err_msg; }
+extern "C" int pbx_last_error_code(void) { return g_err_code; }
+extern "C" void pbx_clear_error(void) {
+    g_err_code = 0;
+    g_err_msg[0] = 0;
+}
+extern "C" const char* pbx_version(void) { return "pybrown_amd 0.1 gfx950"; }
+
+// ---------------------------------------------------------------------------
+// small RAII helpers for the stateless layer
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t bytes) {
+        if (bytes == 0) bytes = 8;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            pbx_set_error(PBX_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+            p = nullptr;
+            return PBX_ERR_OOM;
+        }
+        return PBX_OK;
+    }
+    template <class T>
+    T* as() {
+        return (T*)p;
+    }
+};
+
+#define TRY(x)                   \
+    do {                         \
+        int rc_ = (x);           \
+        if (rc_) return rc_;     \
+    } while (0)
+
+int up(DevBuf& b, const void* h, size_t bytes) {
+    TRY(b.alloc(bytes));
+    if (bytes) PBX_HIP(hipMemcpy(b.p, h, bytes, hipMemcpyHostToDevice));
+    return PBX_OK;
+}
+
+void report(int rc, const char* where) {
+    if (rc) fprintf(stderr, "pybrown_amd: %s failed (%d): %s\n", where, rc, g_err_msg);
+}
+
+int rpy_matrix_impl(const double* as, const double* P, int N, double* M, int smith, double L, double alpha, int m,
+                    int nn) {
+    if (N <= 0) return PBX_OK;
+    size_t n = 3 * (size_t)N;
+    size_t npairs = (size_t)N * (N - 1) / 2;
+    DevBuf da, dP, dM;
+    TRY(up(da, as, sizeof(double) * N));
+    TRY(up(dP, P, sizeof(double) * 3 * npairs));
+    TRY(dM.alloc(sizeof(double) * n * n));
+    int rc = smith ? pbx_launch_smith_packed(da.as<double>(), dP.as<double>(), N, L, alpha, m, nn, dM.as<double>(),
+                                             (int)n, 0)
+                   : pbx_launch_rpy_packed(da.as<double>(), dP.as<double>(), N, dM.as<double>(), (int)n, 0);
+    TRY(rc);
+    PBX_HIP(hipMemcpy(M, dM.p, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    return PBX_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// 1. reference-compatible stateless entry points
+
+// diff_tensor.c:73 RPY_M_matrix
+extern "C" void RPY_M_matrix(double* as, double* pointers, int number_of_beads, double* M_matrix) {
+    pbx_clear_error();
+    report(rpy_matrix_impl(as, pointers, number_of_beads, M_matrix, 0, 0, 0, 0, 0), "RPY_M_matrix");
+}
+
+// diff_tensor.c:170 RPY_Smith_M_matrix
+extern "C" void RPY_Smith_M_matrix(double* as, double* pointers, double box_length, double alpha, int m_max,
+                                   int n_max, int number_of_beads, double* M_matrix) {
+    pbx_clear_error();
+    report(rpy_matrix_impl(as, pointers, number_of_beads, M_matrix, 1, box_length, alpha, m_max, n_max),
+           "RPY_Smith_M_matrix");
+}
+
+// diff_tensor.c:270 JO_2B_R_matrix (writes 18 values)
+extern "C" void JO_2B_R_matrix(double ai, double aj, double rx, double ry, double rz, double* R_matrix) {
+    pbx_clear_error();
+    DevBuf d;
+    int rc = d.alloc(sizeof(double) * 18);
+    if (!rc) rc = pbx_launch_jo_2b(ai, aj, rx, ry, rz, d.as<double>(), 0);
+    if (!rc && hipMemcpy(R_matrix, d.p, sizeof(double) * 18, hipMemcpyDeviceToHost) != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "JO_2B_R_matrix: copy-out failed");
+        rc = PBX_ERR_HIP;
+    }
+    report(rc, "JO_2B_R_matrix");
+}
+
+// diff_tensor.c:346 JO_R_lubrication_correction_matrix (accumulates into C)
+extern "C" void JO_R_lubrication_correction_matrix(double* as, double* pointers, int number_of_beads,
+                                                   double cutoff_distance, int cichocki_correction,
+                                                   double* correction_matrix) {
+    pbx_clear_error();
+    auto run = [&]() -> int {
+        int N = number_of_beads;
+        if (N <= 1) return PBX_OK;
+        size_t n = 3 * (size_t)N, npairs = (size_t)N * (N - 1) / 2;
+        DevBuf da, dP, dC;
+        TRY(up(da, as, sizeof(double) * N));
+        TRY(up(dP, pointers, sizeof(double) * 3 * npairs));
+        TRY(up(dC, correction_matrix, sizeof(double) * n * n));
+        TRY(pbx_launch_jo_correction(nullptr, dP.as<double>(), da.as<double>(), N, 0.0, cutoff_distance,
+                                     cichocki_correction, 1.0, 1.0, dC.as<double>(), (int)n, 0));
+        PBX_HIP(hipMemcpy(correction_matrix, dC.p, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+        return PBX_OK;
+    };
+    report(run(), "JO_R_lubrication_correction_matrix");
+}
+
+// bead.c:27 pointer_pbc_matrix (diagonal entries keep the caller's values)
+extern "C" void pointer_pbc_matrix(double* positions, int number_of_beads, double box_length, double* pointers) {
+    pbx_clear_error();
+    auto run = [&]() -> int {
+        int N = number_of_beads;
+        if (N <= 1) return PBX_OK;
+        DevBuf dx, dP;
+        TRY(up(dx, positions, sizeof(double) * 3 * N));
+        TRY(up(dP, pointers, sizeof(double) * 3 * (size_t)N * N));
+        TRY(pbx_launch_pointers(dx.as<double>(), N, box_length, dP.as<double>(), 0));
+        PBX_HIP(hipMemcpy(pointers, dP.p, sizeof(double) * 3 * (size_t)N * N, hipMemcpyDeviceToHost));
+        return PBX_OK;
+    };
+    report(run(), "pointer_pbc_matrix");
+}
+
+// bead.c:169 check_overlaps; returns 1 on overlap, 0 otherwise (-1 on error)
+extern "C" int check_overlaps(double* positions, double* as, int number_of_beads, double box_length,
+                              double overlap_treshold, int* connection_matrix) {
+    pbx_clear_error();
+    int result = 0;
+    auto run = [&]() -> int {
+        int N = number_of_beads;
+        if (N <= 1) return PBX_OK;
+        DevBuf dx, da, dm, df;
+        TRY(up(dx, positions, sizeof(double) * 3 * N));
+        TRY(up(da, as, sizeof(double) * N));
+        if (connection_matrix) TRY(up(dm, connection_matrix, sizeof(int) * (size_t)N * N));
+        TRY(df.alloc(sizeof(int)));
+        PBX_HIP(hipMemset(df.p, 0, sizeof(int)));
+        TRY(pbx_launch_overlaps(dx.as<double>(), da.as<double>(), N, box_length, overlap_treshold,
+                                connection_matrix ? dm.as<int>() : nullptr, df.as<int>(), 0));
+        PBX_HIP(hipMemcpy(&result, df.p, sizeof(int), hipMemcpyDeviceToHost));
+        return PBX_OK;
+    };
+    int rc = run();
+    report(rc, "check_overlaps");
+    return rc ? -1 : (result ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// dense kernels on host arrays (the numpy.linalg call sites of box.py)
+
+// Lower Cholesky of the n x n row-major SPD matrix A (box.py:554).  L gets
+// the factor with a zero upper triangle.  Returns 0, k>0 (non-positive pivot
+// at column k-1) or a negative error.
+extern "C" int pbx_cholesky(const double* A, int n, double* L) {
+    pbx_clear_error();
+    if (n <= 0) return PBX_ERR_ARG;
+    int np = (int)pbx_round_up(n, 64), ld = np;
+    DevBuf dA, dW, dI;
+    TRY(dA.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dW.alloc(sizeof(double) * pbx_dense_work_doubles(np)));
+    TRY(dI.alloc(sizeof(int)));
+    PBX_HIP(hipMemset(dI.p, 0, sizeof(int)));
+    PBX_HIP(hipMemcpy2D(dA.p, sizeof(double) * ld, A, sizeof(double) * n, sizeof(double) * n, n,
+                        hipMemcpyHostToDevice));
+    TRY(pbx_fill_pad(dA.as<double>(), n, np, ld, 0));
+    TRY(pbx_potrf(dA.as<double>(), np, ld, dW.as<double>(), dI.as<int>(), 0));
+    int info = 0;
+    PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
+    PBX_HIP(hipMemcpy2D(L, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
+                        hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r)
+        for (int c = r + 1; c < n; ++c) L[(size_t)r * n + c] = 0.0;
+    if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
+    return info;
+}
+
+// X = scale * inv(A) for SPD A (box.py:514,524,537 np.linalg.inv call sites)
+extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double* X) {
+    pbx_clear_error();
+    if (n <= 0) return PBX_ERR_ARG;
+    int np = (int)pbx_round_up(n, 64), ld = np;
+    DevBuf dA, dX, dW, dI;
+    TRY(dA.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dX.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dW.alloc(sizeof(double) * pbx_dense_work_doubles(np)));
+    TRY(dI.alloc(sizeof(int)));
+    PBX_HIP(hipMemset(dI.p, 0, sizeof(int)));
+    PBX_HIP(hipMemcpy2D(dA.p, sizeof(double) * ld, A, sizeof(double) * n, sizeof(double) * n, n,
+                        hipMemcpyHostToDevice));
+    TRY(pbx_fill_pad(dA.as<double>(), n, np, ld, 0));
+    TRY(pbx_spd_inverse(dA.as<double>(), np, ld, dW.as<double>(), dX.as<double>(), dI.as<int>(), 0, scale));
+    int info = 0;
+    PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
+    PBX_HIP(hipMemcpy2D(X, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
+                        hipMemcpyDeviceToHost));
+    if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
+    return info;
+}
+
+// C = A B^T, all dimensions multiples of 64 (checks the MFMA tile layout)
+extern "C" int pbx_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K) {
+    pbx_clear_error();
+    DevBuf dA, dB, dC;
+    TRY(up(dA, A, sizeof(double) * (size_t)M * K));
+    TRY(up(dB, B, sizeof(double) * (size_t)N * K));
+    TRY(dC.alloc(sizeof(double) * (size_t)M * N));
+    TRY(pbx_gemm_nt_test(dA.as<double>(), dB.as<double>(), dC.as<double>(), M, N, K, 0));
+    PBX_HIP(hipMemcpy(C, dC.p, sizeof(double) * (size_t)M * N, hipMemcpyDeviceToHost));
+    return PBX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// 2. device-resident simulation
+
+struct PbxSim {
+    pbx_config cfg;
+    int N = 0, n = 0, np = 0, ld = 0;
+    double kBT = 0.0;
+    int F_nonzero = 0;
+    hipStream_t stream = nullptr;
+    // device state
+    double *pos = nullptr, *a = nullptr, *F = nullptr, *disp = nullptr, *drift = nullptr;
+    double *w = nullptr, *xipad = nullptr;
+    int* label = nullptr;
+    long long* flux = nullptr;
+    int *flag = nullptr, *info = nullptr;
+    double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
+    double *Wchol = nullptr, *Winv = nullptr;
+    double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
+    // noise pool
+    double* xi_dev = nullptr;
+    double* xi_host = nullptr;  // pinned
+    long long xi_cap = 0;       // attempts per refill
+    long long xi_next = 0, xi_have = 0;
+    pbx_xi_fill_fn xi_fill = nullptr;
+    void* xi_user = nullptr;
+    long long draw_count = 0;
+    int* flag_host = nullptr;  // pinned
+    int last_attempts = 0;
+    int have_D = 0, have_B = 0, have_Rff = 0;
+    std::vector<void*> owned;
+};
+
+namespace {
+
+int sim_alloc(PbxSim* s, void** p, size_t bytes) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+        pbx_set_error(PBX_ERR_OOM, "hipMalloc(%zu) failed", bytes);
+        *p = nullptr;
+        return PBX_ERR_OOM;
+    }
+    s->owned.push_back(*p);
+    return PBX_OK;
+}
+
+int sim_matrix(PbxSim* s, double** p) {
+    TRY(sim_alloc(s, (void**)p, sizeof(double) * (size_t)s->np * s->ld));
+    PBX_HIP(hipMemsetAsync(*p, 0, sizeof(double) * (size_t)s->np * s->ld, s->stream));
+    return pbx_fill_pad(*p, s->n, s->np, s->ld, s->stream);
+}
+
+bool is_lub(const PbxSim* s) { return s->cfg.hydrodynamics == PBX_RPY_LUB || s->cfg.hydrodynamics == PBX_RPY_SMITH_LUB; }
+bool is_smith(const PbxSim* s) {
+    return s->cfg.hydrodynamics == PBX_RPY_SMITH || s->cfg.hydrodynamics == PBX_RPY_SMITH_LUB;
+}
+
+// Mobility (optionally Ewald) times `scale`, written into M.
+int assemble_M(PbxSim* s, double* M, double scale) {
+    const pbx_config& c = s->cfg;
+    if (is_smith(s))
+        return pbx_launch_smith(s->pos, s->a, s->N, c.box_length, c.ewald_alpha, c.ewald_real, c.ewald_imag, scale,
+                                M, s->ld, 0, s->stream);
+    return pbx_launch_rpy(s->pos, s->a, s->N, c.box_length, scale, M, s->ld, 0, s->stream);
+}
+
+int check_info(PbxSim* s) {
+    int info = 0;
+    PBX_HIP(hipMemcpyAsync(s->flag_host + 1, s->info, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    info = s->flag_host[1];
+    if (info) {
+        pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
+        PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
+    }
+    return info;
+}
+
+// box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
+int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
+    const pbx_config& c = s->cfg;
+    if (!is_lub(s)) {
+        // D = M * (kBT 1e19 / eta)
+        TRY(assemble_M(s, Dout, s->kBT * 1e19 / c.viscosity));
+        return PBX_OK;
+    }
+    // Mff = M 1e19 / eta ; Rff = inv(Mff)
+    TRY(assemble_M(s, RffOut, 1e19 / c.viscosity));
+    TRY(pbx_spd_inverse(RffOut, s->np, s->ld, s->Winv, s->X, s->info, s->stream, 1.0));
+    return PBX_OK;
+}
+
+// box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R)
+int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
+    const pbx_config& c = s->cfg;
+    PBX_HIP(hipMemcpyAsync(Dout, RffIn, sizeof(double) * (size_t)s->np * s->ld, hipMemcpyDeviceToDevice, s->stream));
+    TRY(pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff, c.cichocki_correction,
+                                 c.viscosity, 1e19, Dout, s->ld, s->stream));
+    TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT));
+    TRY(pbx_fill_pad(Dout, s->n, s->np, s->ld, s->stream));
+    return PBX_OK;
+}
+
+// box.py:546-556 _decompose_D_matrix
+int factor(PbxSim* s) {
+    TRY(pbx_copy_lower(s->D, s->B, s->np, s->ld, s->stream));
+    TRY(pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream));
+    return PBX_OK;
+}
+
+// next 3N draws of the host N(0,1) stream (one move attempt, box.py:393-399)
+int next_xi(PbxSim* s, const double** out) {
+    if (s->xi_next >= s->xi_have) {
+        if (!s->xi_fill) {
+            pbx_set_error(PBX_ERR_STATE, "no noise source registered");
+            return PBX_ERR_STATE;
+        }
+        PBX_HIP(hipStreamSynchronize(s->stream));  // previous pool no longer in use
+        long long cnt = s->xi_cap * (long long)s->n;
+        if (s->xi_fill(s->xi_user, s->xi_host, cnt) != 0) {
+            pbx_set_error(PBX_ERR_STATE, "noise source callback failed");
+            return PBX_ERR_STATE;
+        }
+        PBX_HIP(hipMemcpyAsync(s->xi_dev, s->xi_host, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream));
+        s->xi_next = 0;
+        s->xi_have = s->xi_cap;
+    }
+    *out = s->xi_dev + s->xi_next * (long long)s->n;
+    s->xi_next += 1;
+    s->draw_count += s->n;
+    return PBX_OK;
+}
+
+int overlap_now(PbxSim* s, int* hit) {
+    PBX_HIP(hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
+    TRY(pbx_launch_overlaps(s->pos, s->a, s->N, s->cfg.box_length, s->cfg.overlap_treshold, nullptr, s->flag,
+                            s->stream));
+    PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    *hit = s->flag_host[0];
+    return PBX_OK;
+}
+
+struct StepCtx {
+    double sq2dt;
+    const double* pn;
+    const double* pp;
+    int flux_on;
+};
+
+int stoch(PbxSim* s, const double* Bm, const double* xi, double mult, StepCtx& c) {
+    int diag = s->cfg.hydrodynamics == PBX_NOHI;
+    return pbx_launch_stoch(diag ? s->Bd : Bm, s->ld, diag, xi, c.sq2dt, mult, s->pos, s->disp, s->N, c.pn, c.pp,
+                            c.flux_on, s->label, s->flux, s->stream);
+}
+
+int det(PbxSim* s, const double* Dm, double dt, double mult, StepCtx& c) {
+    if (!s->F_nonzero) return PBX_OK;  // translating by D.0 leaves positions unchanged
+    int diag = s->cfg.hydrodynamics == PBX_NOHI;
+    return pbx_launch_det(diag ? s->Dd : Dm, s->ld, diag, s->F, dt / s->kBT, mult, s->pos, s->N, c.pn, c.pp,
+                          c.flux_on, s->label, s->flux, s->stream);
+}
+
+int undo(PbxSim* s, StepCtx& c) {
+    return pbx_launch_translate(s->disp, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
+}
+
+// box.py:202-250
+int ermak(PbxSim* s, double dt, StepCtx& c) {
+    TRY(det(s, s->D, dt, 1.0, c));
+    int attempts = 0;
+    while (true) {
+        ++attempts;
+        const double* xi;
+        TRY(next_xi(s, &xi));
+        TRY(stoch(s, s->B, xi, 1.0, c));
+        if (!s->cfg.check_overlaps) break;
+        int hit = 0;
+        TRY(overlap_now(s, &hit));
+        if (!hit) break;
+        TRY(undo(s, c));
+        if (attempts >= s->cfg.max_move_attempts) {
+            s->last_attempts = attempts;
+            pbx_set_error(PBX_ERR_MOVES, "maximal number of move attepts exceeded");
+            return PBX_ERR_MOVES;
+        }
+    }
+    s->last_attempts = attempts;
+    return PBX_OK;
+}
+
+// box.py:254-349
+int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
+    const pbx_config& cf = s->cfg;
+    if (cf.hydrodynamics == PBX_NOHI) {
+        pbx_set_error(PBX_ERR_ARG, "midpoint scheme needs a matrix diffusion tensor (hydrodynamics != nohi)");
+        return PBX_ERR_ARG;
+    }
+    const double m = cf.m_midpoint;
+    TRY(det(s, s->D, dt, 1.0 / m, c));
+    int attempts = 0;
+    while (true) {
+        ++attempts;
+        const double* xi;
+        TRY(next_xi(s, &xi));
+        TRY(stoch(s, s->B, xi, 1.0 / m, c));
+        if (cf.check_overlaps) {
+            int hit = 0;
+            TRY(overlap_now(s, &hit));
+            if (hit) {
+                TRY(undo(s, c));
+                continue;
+            }
+        }
+        // D at the midpoint (the reference's midpoint Cholesky is discarded by
+        // the B0 restore, box.py:303-316, and is not computed here)
+        const double* Dm = s->D;
+        if (bDff || (is_lub(s) && bDnf)) {
+            if (bDff) TRY(build_Dff(s, s->Dmid, s->RffMid));
+            if (is_lub(s) && bDnf) TRY(build_Dtot(s, bDff ? s->RffMid : s->Rff, s->Dmid));
+            if (is_lub(s) || bDff) Dm = s->Dmid;
+            if (is_lub(s) && bDff && !bDnf) Dm = s->D;  // self.D untouched by Dff in lub modes
+            if (is_lub(s)) TRY(check_info(s));
+        }
+        // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi
+        PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
+        TRY(pbx_launch_trsv_lt(s->B, s->np, s->ld, s->Wchol, s->xipad, s->w, s->stream));
+        TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
+        if (s->F_nonzero)
+            TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));
+        TRY(det(s, s->D, dt, 1.0 - 1.0 / m, c));
+        TRY(stoch(s, s->B, xi, 1.0 - 1.0 / m, c));
+        TRY(pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream));
+        if (!cf.check_overlaps) break;
+        int hit = 0;
+        TRY(overlap_now(s, &hit));
+        if (!hit) break;
+        TRY(det(s, s->D, dt, -(1.0 - 1.0 / m), c));
+        TRY(stoch(s, s->B, xi, -1.0, c));
+        TRY(pbx_launch_translate(s->drift, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream));
+        if (attempts >= cf.max_move_attempts) {
+            s->last_attempts = attempts;
+            pbx_set_error(PBX_ERR_MOVES, "maximal number of move attepts exceeded");
+            return PBX_ERR_MOVES;
+        }
+    }
+    s->last_attempts = attempts;
+    return PBX_OK;
+}
+
+}  // namespace
+
+extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii, const double* xyz,
+                              const int* label_idx, pbx_sim** out) {
+    pbx_clear_error();
+    *out = nullptr;
+    if (!cfg || N <= 0 || !radii || !xyz) {
+        pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: bad arguments");
+        return PBX_ERR_ARG;
+    }
+    if (cfg->hydrodynamics < PBX_NOHI || cfg->hydrodynamics > PBX_RPY_SMITH_LUB || cfg->scheme < 0 ||
+        cfg->scheme > 1) {
+        pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: unknown hydrodynamics/scheme");
+        return PBX_ERR_ARG;
+    }
+    PbxSim* s = new PbxSim();
+    s->cfg = *cfg;
+    s->N = N;
+    s->n = 3 * N;
+    s->np = (int)pbx_round_up(s->n, 64);
+    s->ld = s->np;
+    s->kBT = 1.380649e-23 * cfg->T;  // scipy.constants.Boltzmann * T (box.py:636)
+    s->xi_cap = cfg->xi_pool_attempts > 0 ? cfg->xi_pool_attempts : 64;
+    int rc = PBX_OK;
+    auto fail = [&](int code) {
+        pbx_sim_destroy(s);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "hipStreamCreate failed");
+        delete s;
+        return PBX_ERR_HIP;
+    }
+    const size_t n3 = s->n;
+    if ((rc = sim_alloc(s, (void**)&s->pos, sizeof(double) * n3))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->a, sizeof(double) * N))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->F, sizeof(double) * s->np))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->disp, sizeof(double) * s->np))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->drift, sizeof(double) * s->np))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->w, sizeof(double) * s->np))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->xipad, sizeof(double) * s->np))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->label, sizeof(int) * N))) return fail(rc);
+    int nl = cfg->n_labels > 0 ? cfg->n_labels : 1;
+    if ((rc = sim_alloc(s, (void**)&s->flux, sizeof(long long) * nl))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->flag, sizeof(int) * 2))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->info, sizeof(int)))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * s->xi_cap * n3))) return fail(rc);
+    if (hipHostMalloc((void**)&s->xi_host, sizeof(double) * s->xi_cap * n3, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->flag_host, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
+        pbx_set_error(PBX_ERR_OOM, "hipHostMalloc failed");
+        return fail(PBX_ERR_OOM);
+    }
+    auto hip_ok = [&](hipError_t e, const char* what) {
+        if (e != hipSuccess) {
+            pbx_set_error(PBX_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+            return false;
+        }
+        return true;
+    };
+    if (!hip_ok(hipMemcpyAsync(s->pos, xyz, sizeof(double) * n3, hipMemcpyHostToDevice, s->stream), "pos") ||
+        !hip_ok(hipMemcpyAsync(s->a, radii, sizeof(double) * N, hipMemcpyHostToDevice, s->stream), "radii") ||
+        !hip_ok(hipMemsetAsync(s->xipad, 0, sizeof(double) * s->np, s->stream), "xipad") ||
+        !hip_ok(hipMemsetAsync(s->info, 0, sizeof(int), s->stream), "info") ||
+        !hip_ok(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream), "flux"))
+        return fail(PBX_ERR_HIP);
+    std::vector<int> lab(N, 0);
+    if (label_idx)
+        for (int i = 0; i < N; ++i) lab[i] = label_idx[i];
+    std::vector<double> F(s->np, 0.0);
+    for (int i = 0; i < N; ++i)
+        for (int c = 0; c < 3; ++c) {
+            F[3 * i + c] = cfg->external_force[c];
+            if (cfg->external_force[c] != 0.0) s->F_nonzero = 1;
+        }
+    if (!hip_ok(hipMemcpy(s->label, lab.data(), sizeof(int) * N, hipMemcpyHostToDevice), "labels") ||
+        !hip_ok(hipMemcpy(s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice), "F"))
+        return fail(PBX_ERR_HIP);
+
+    if (cfg->hydrodynamics == PBX_NOHI) {
+        // box.py:488-492 and :550-552: D = kBT 1e19/6/pi/a/eta, B = sqrt(D)
+        std::vector<double> Dd(n3), Bd(n3);
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < 3; ++c) {
+                double d = s->kBT * 1e19 / 6 / 3.14159265358979323846 / radii[i] / cfg->viscosity;
+                Dd[3 * i + c] = d;
+                Bd[3 * i + c] = sqrt(d);
+            }
+        if ((rc = sim_alloc(s, (void**)&s->Dd, sizeof(double) * n3))) return fail(rc);
+        if ((rc = sim_alloc(s, (void**)&s->Bd, sizeof(double) * n3))) return fail(rc);
+        if (!hip_ok(hipMemcpy(s->Dd, Dd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Dd") ||
+            !hip_ok(hipMemcpy(s->Bd, Bd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Bd"))
+            return fail(PBX_ERR_HIP);
+        s->have_D = s->have_B = 1;
+    } else {
+        if ((rc = sim_matrix(s, &s->D))) return fail(rc);
+        if ((rc = sim_matrix(s, &s->B))) return fail(rc);
+        if ((rc = sim_alloc(s, (void**)&s->Wchol, sizeof(double) * pbx_dense_work_doubles(s->np)))) return fail(rc);
+        if (cfg->scheme == PBX_MIDPOINT)
+            if ((rc = sim_matrix(s, &s->Dmid))) return fail(rc);
+        if (is_lub(s)) {
+            if ((rc = sim_matrix(s, &s->Rff))) return fail(rc);
+            if ((rc = sim_matrix(s, &s->X))) return fail(rc);
+            if ((rc = sim_alloc(s, (void**)&s->Winv, sizeof(double) * pbx_dense_work_doubles(s->np))))
+                return fail(rc);
+            if (cfg->scheme == PBX_MIDPOINT)
+                if ((rc = sim_matrix(s, &s->RffMid))) return fail(rc);
+        }
+    }
+    if (!hip_ok(hipStreamSynchronize(s->stream), "create sync")) return fail(PBX_ERR_HIP);
+    if (cfg->check_overlaps) {
+        int hit = 0;
+        if ((rc = overlap_now(s, &hit))) return fail(rc);
+        if (hit) {
+            pbx_set_error(PBX_ERR_OVERLAP, "starting configuration contains overlaps");
+            return fail(PBX_ERR_OVERLAP);
+        }
+    }
+    *out = s;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_destroy(pbx_sim* s) {
+    if (!s) return PBX_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->owned) (void)hipFree(p);
+    if (s->xi_host) (void)hipHostFree(s->xi_host);
+    if (s->flag_host) (void)hipHostFree(s->flag_host);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_noise_source(pbx_sim* s, pbx_xi_fill_fn fn, void* user) {
+    s->xi_fill = fn;
+    s->xi_user = user;
+    s->xi_next = s->xi_have = 0;
+    return PBX_OK;
+}
+
+// box.py:150-198 propagate
+extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int build_Dnf_flag, int cholesky_flag) {
+    pbx_clear_error();
+    const pbx_config& cf = s->cfg;
+    StepCtx c;
+    c.sq2dt = sqrt(2 * dt);
+    c.pn = cf.flux_normal;
+    c.pp = cf.flux_plane_point;
+    c.flux_on = cf.measure_flux;
+    if (cf.measure_flux) {
+        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
+        PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
+    }
+    if (cf.hydrodynamics != PBX_NOHI) {
+        if (build_Dff_flag) {
+            TRY(build_Dff(s, s->D, s->Rff));
+            if (is_lub(s)) s->have_Rff = 1; else s->have_D = 1;
+        }
+        if (is_lub(s) && build_Dnf_flag) {
+            if (!s->have_Rff) {
+                pbx_set_error(PBX_ERR_STATE, "lubrication step before the far-field matrix was built");
+                return PBX_ERR_STATE;
+            }
+            TRY(build_Dtot(s, s->Rff, s->D));
+            s->have_D = 1;
+        }
+        if (cholesky_flag) {
+            if (!s->have_D) {
+                pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
+                return PBX_ERR_STATE;
+            }
+            TRY(factor(s));
+            s->have_B = 1;
+        }
+        if (!s->have_B) {
+            pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
+            return PBX_ERR_STATE;
+        }
+        int info = check_info(s);
+        if (info) return info;
+    }
+    if (cf.scheme == PBX_ERMAK)
+        TRY(ermak(s, dt, c));
+    else
+        TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
+    TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_get_positions(pbx_sim* s, double* xyz) {
+    PBX_HIP(hipMemcpyAsync(xyz, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_positions(pbx_sim* s, const double* xyz) {
+    PBX_HIP(hipMemcpyAsync(s->pos, xyz, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_get_flux(pbx_sim* s, long long* out) {
+    int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    PBX_HIP(hipMemcpyAsync(out, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+// which: 0 = D, 1 = B (upper triangle zeroed), 2 = Rff
+extern "C" int pbx_sim_get_matrix(pbx_sim* s, int which, double* out) {
+    const double* src = nullptr;
+    if (s->cfg.hydrodynamics == PBX_NOHI) {
+        pbx_set_error(PBX_ERR_ARG, "nohi keeps D and B as vectors (pbx_sim_get_diag)");
+        return PBX_ERR_ARG;
+    }
+    if (which == 0) src = s->D;
+    if (which == 1) src = s->B;
+    if (which == 2) src = s->Rff;
+    if (!src) {
+        pbx_set_error(PBX_ERR_ARG, "matrix %d not held by this simulation", which);
+        return PBX_ERR_ARG;
+    }
+    PBX_HIP(hipMemcpy2DAsync(out, sizeof(double) * s->n, src, sizeof(double) * s->ld, sizeof(double) * s->n, s->n,
+                             hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    if (which == 1)
+        for (int r = 0; r < s->n; ++r)
+            for (int c = r + 1; c < s->n; ++c) out[(size_t)r * s->n + c] = 0.0;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_get_diag(pbx_sim* s, int which, double* out) {
+    const double* src = which == 0 ? s->Dd : s->Bd;
+    if (!src) {
+        pbx_set_error(PBX_ERR_ARG, "diagonal D/B exist only for hydrodynamics nohi");
+        return PBX_ERR_ARG;
+    }
+    PBX_HIP(hipMemcpy(out, src, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    return PBX_OK;
+}
+
+extern "C" long long pbx_sim_draw_count(pbx_sim* s) { return s->draw_count; }
+extern "C" int pbx_sim_last_attempts(pbx_sim* s) { return s->last_attempts; }
+
+extern "C" int pbx_sim_sync(pbx_sim* s) {
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}

@@ -1,0 +1,93 @@
+// Shared definitions for the MI355X (gfx950) hydrodynamic-step kernels.
+//
+// Storage conventions used by every kernel in this library:
+//  * bead data: xyz as N x 3 doubles (bead-major), radii as N doubles;
+//  * dense 3N x 3N matrices are row-major with leading dimension `ld`, padded
+//    to `np = round_up(3N, 64)` rows/columns; the pad holds the identity
+//    (zero off-diagonal, one on the diagonal) so blocked factorisations run
+//    on whole 64 x 64 tiles with no bounds checks and leave the pad intact.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define PBX_TILE 64
+
+// status codes returned by the C ABI (0 = success)
+#define PBX_OK 0
+#define PBX_ERR_ARG -1
+#define PBX_ERR_HIP -2
+#define PBX_ERR_OOM -3
+#define PBX_ERR_MOVES -4     // max_move_attempts exceeded
+#define PBX_ERR_OVERLAP -5   // starting configuration overlaps
+#define PBX_ERR_STATE -6
+// k > 0: the Cholesky factorisation met a non-positive pivot at column k-1
+
+static inline size_t pbx_round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+extern "C" void pbx_set_error(int code, const char* fmt, ...);
+
+#define PBX_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            pbx_set_error(PBX_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,          \
+                          hipGetErrorString(e_));                                          \
+            return PBX_ERR_HIP;                                                            \
+        }                                                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+// Minimum-image reduction with the reference's loop semantics
+// (src/bdsim/clib/bead.c:44-50); the result lies in (-L/2, L/2].  Callers
+// guarantee L > 0; non-finite d is returned unchanged and |d| > 64 L is first
+// reduced with fmod, so every loop ends after at most 65 iterations.
+__device__ __forceinline__ double pbx_min_image(double d, double L) {
+    if (!isfinite(d)) return d;
+    if (fabs(d) > 64.0 * L) d = fmod(d, L);
+    const double h = L / 2.0;
+    while (d >= h) d -= L;
+    while (d <= -h) d += L;
+    return d;
+}
+
+// Host-side check shared by every launcher that reduces by a box length.
+static inline bool pbx_box_ok(double L) { return L > 0.0 && L < 1e300; }
+#define PBX_CHECK_BOX(L)                                                                   \
+    do {                                                                                   \
+        if (!pbx_box_ok(L)) {                                                              \
+            pbx_set_error(PBX_ERR_ARG, "box_length must be positive and finite (%g)", (L)); \
+            return PBX_ERR_ARG;                                                            \
+        }                                                                                  \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// launchers shared across translation units
+
+// pair assembly (pbx_pairs.hip)
+int pbx_launch_rpy(const double* xyz, const double* a, int N, double L, double scale, double* M, int ld,
+                   int write_pad, hipStream_t s);
+int pbx_launch_smith(const double* xyz, const double* a, int N, double L, double alpha, int m_max, int n_max,
+                     double scale, double* M, int ld, int write_pad, hipStream_t s);
+int pbx_launch_rpy_packed(const double* a, const double* P, int N, double* M, int ld, hipStream_t s);
+int pbx_launch_smith_packed(const double* a, const double* P, int N, double L, double alpha, int m_max, int n_max,
+                            double* M, int ld, hipStream_t s);
+int pbx_launch_pointers(const double* xyz, int N, double L, double* P3, hipStream_t s);
+int pbx_launch_overlaps(const double* xyz, const double* a, int N, double L, double thr, const int* mask,
+                        int* flag, hipStream_t s);
+
+// lubrication (pbx_lub.hip): C = ((dR * mul) / div) + C
+int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
+                             int cichocki, double mul, double div, double* C, int ld, hipStream_t s);
+int pbx_launch_jo_2b(double ai, double aj, double rx, double ry, double rz, double* out18, hipStream_t s);
+
+// dense linear algebra (pbx_dense.hip); np must be a multiple of 64
+int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s);
+int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
+                    double scale);
+int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
+int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
+size_t pbx_dense_work_doubles(int np);
+int pbx_gemm_nt_test(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t s);

@@ -1,0 +1,302 @@
+// Propagation kernels on gfx950: Ermak-McCammon / midpoint sub-steps with the
+// signed plane-crossing count and the periodic wrap (reference:
+// src/bdsim/pyBrown/box.py:353-389, :614-617; bead.py:92-132, :170-183).
+//
+// Matrix-vector products read the row-major matrices one wave per row:
+// lanes stride the row (64 x 8 B = 512 B per load), a butterfly reduces the
+// partial sums.  They are HBM-bound; the lower-triangular product with B
+// reads only c <= r.
+#include "pbx_common.h"
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// dot of a row segment [0, len) with x
+__device__ __forceinline__ double row_dot(const double* __restrict__ row, const double* __restrict__ x, int len) {
+    const int lane = threadIdx.x & 63;
+    double s0 = 0.0, s1 = 0.0;
+    int c = lane;
+    for (; c + 64 < len; c += 128) {
+        s0 += row[c] * x[c];
+        s1 += row[c + 64] * x[c + 64];
+    }
+    if (c < len) s0 += row[c] * x[c];
+    return wave_sum(s0 + s1);
+}
+
+struct FluxPlane {
+    double n[3], p[3];
+    int on;
+};
+
+// sign of n.(r - p) > 0 with plain left-to-right arithmetic
+__device__ __forceinline__ int plane_side(const FluxPlane& f, double x, double y, double z) {
+#pragma clang fp contract(off)
+    double d = f.n[0] * (x - f.p[0]) + f.n[1] * (y - f.p[1]) + f.n[2] * (z - f.p[2]);
+    return d > 0.0 ? 1 : 0;
+}
+
+// r_b += v_b (bead b), with the per-label signed crossing count
+__device__ __forceinline__ void move_bead(double* pos, const double* v, int b, const FluxPlane& f,
+                                          const int* label, long long* flux) {
+    double x0 = pos[3 * b], y0 = pos[3 * b + 1], z0 = pos[3 * b + 2];
+    double x1 = x0 + v[0], y1 = y0 + v[1], z1 = z0 + v[2];
+    pos[3 * b] = x1;
+    pos[3 * b + 1] = y1;
+    pos[3 * b + 2] = z1;
+    if (f.on) {
+        int s0 = plane_side(f, x0, y0, z0), s1 = plane_side(f, x1, y1, z1);
+        if (s0 != s1) atomicAdd((unsigned long long*)&flux[label[b]], (unsigned long long)(long long)(s1 - s0));
+    }
+}
+
+// Stochastic sub-step (box.py:366-375): BX = ((B xi) * sqrt(2 dt)) * mult,
+// B lower-triangular.  One 192-thread workgroup per bead (a wave per row).
+__global__ __launch_bounds__(192) void stoch_kernel(const double* __restrict__ B, int ld, const double* __restrict__ xi,
+                                                    double sq2dt, double mult, double* pos, double* disp,
+                                                    FluxPlane f, const int* label, long long* flux) {
+    __shared__ double v[3];
+    const int b = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int r = 3 * b + w;
+    double s = row_dot(B + (size_t)r * ld, xi, r + 1);
+    if ((threadIdx.x & 63) == 0) {
+        double d = (s * sq2dt) * mult;
+        v[w] = d;
+        disp[r] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) move_bead(pos, v, b, f, label, flux);
+}
+
+// nohi variant: B is the element-wise sqrt of the diagonal D (box.py:369)
+__global__ void stoch_diag_kernel(const double* __restrict__ Bd, const double* __restrict__ xi, double sq2dt,
+                                  double mult, double* pos, double* disp, int N, FluxPlane f, const int* label,
+                                  long long* flux) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    double v[3];
+    for (int c = 0; c < 3; ++c) {
+        v[c] = ((Bd[3 * b + c] * xi[3 * b + c]) * sq2dt) * mult;
+        disp[3 * b + c] = v[c];
+    }
+    move_bead(pos, v, b, f, label, flux);
+}
+
+// Deterministic sub-step (box.py:353-362): FX = (((dt/kBT) D) F) * mult with
+// D full symmetric; one wave per row.
+__global__ __launch_bounds__(192) void det_kernel(const double* __restrict__ D, int ld, const double* __restrict__ F,
+                                                  double k, double mult, int n, double* pos, FluxPlane f,
+                                                  const int* label, long long* flux) {
+    __shared__ double v[3];
+    const int b = blockIdx.x;
+    const int w = threadIdx.x >> 6;
+    const int r = 3 * b + w;
+    const int lane = threadIdx.x & 63;
+    const double* row = D + (size_t)r * ld;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += (k * row[c]) * F[c];
+    s = wave_sum(s);
+    if (lane == 0) v[w] = s * mult;
+    __syncthreads();
+    if (threadIdx.x == 0) move_bead(pos, v, b, f, label, flux);
+}
+
+__global__ void det_diag_kernel(const double* __restrict__ Dd, const double* __restrict__ F, double k, double mult,
+                                int N, double* pos, FluxPlane f, const int* label, long long* flux) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    double v[3];
+    for (int c = 0; c < 3; ++c) v[c] = ((k * Dd[3 * b + c]) * F[3 * b + c]) * mult;
+    move_bead(pos, v, b, f, label, flux);
+}
+
+// r += sign * v for all beads (undo of a rejected sub-step, midpoint drift)
+__global__ void translate_kernel(const double* __restrict__ vin, double sign, int N, double* pos, FluxPlane f,
+                                 const int* label, long long* flux) {
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    double v[3] = {sign * vin[3 * b], sign * vin[3 * b + 1], sign * vin[3 * b + 2]};
+    move_bead(pos, v, b, f, label, flux);
+}
+
+// bead.py:170-183 keep_in_box: wrap into [-L/2, L/2)
+__global__ void wrap_kernel(double* pos, int n3, double L) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n3) return;
+    double x = pos[t];
+    if (!isfinite(x)) return;
+    if (fabs(x) > 64.0 * L) x = fmod(x, L);
+    while (x < -L / 2) x += L;
+    while (x >= L / 2) x -= L;
+    pos[t] = x;
+}
+
+// y = (L x) * scale for lower-triangular L, one wave per row
+__global__ __launch_bounds__(256) void lower_gemv_kernel(const double* __restrict__ Lm, int n, int ld,
+                                                         const double* __restrict__ x, double* y, double scale) {
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    double s = row_dot(Lm + (size_t)r * ld, x, r + 1);
+    if ((threadIdx.x & 63) == 0) y[r] = s * scale;
+}
+
+// y = alpha * (D x - Lo z): full D, lower-triangular Lo (midpoint stochastic
+// drift, box.py:308: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi with x = B0^{-T} xi)
+__global__ __launch_bounds__(256) void drift_kernel(const double* __restrict__ D, const double* __restrict__ Lo,
+                                                    int n, int ld, const double* __restrict__ x,
+                                                    const double* __restrict__ z, double alpha, double* y) {
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    double s1 = row_dot(D + (size_t)r * ld, x, n);
+    double s2 = row_dot(Lo + (size_t)r * ld, z, r + 1);
+    if ((threadIdx.x & 63) == 0) y[r] = alpha * (s1 - s2);
+}
+
+// y += beta * (D1 - D0) F over full rows (midpoint deterministic drift)
+__global__ __launch_bounds__(256) void det_drift_kernel(const double* __restrict__ D1, const double* __restrict__ D0,
+                                                        int n, int ld, const double* __restrict__ F, double beta,
+                                                        double* y) {
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += (beta * (D1[(size_t)r * ld + c] - D0[(size_t)r * ld + c])) * F[c];
+    s = wave_sum(s);
+    if (lane == 0) y[r] += s;
+}
+
+// Solve L^T y = x for lower-triangular L (order np, multiple of 64) given the
+// inverses W_k of its diagonal tiles: back substitution over tile rows in one
+// 1024-thread workgroup.  y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m).
+__global__ __launch_bounds__(1024) void trsv_lt_kernel(const double* __restrict__ Lm, int np, int ld,
+                                                       const double* __restrict__ W, const double* __restrict__ x,
+                                                       double* y) {
+    __shared__ double part[16][64];
+    __shared__ double rvec[64];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int T = np / 64;
+    for (int k = T - 1; k >= 0; --k) {
+        double s = 0.0;
+        for (int rho = 64 * (k + 1) + g; rho < np; rho += 16) s += Lm[(size_t)rho * ld + 64 * k + c] * y[rho];
+        part[g][c] = s;
+        __syncthreads();
+        if (g == 0) {
+            double t = 0.0;
+            for (int q = 0; q < 16; ++q) t += part[q][c];
+            rvec[c] = x[64 * k + c] - t;
+        }
+        __syncthreads();
+        // y_k[c] = sum_{c'} W_k[c'][c] rvec[c']  (W_k lower: c' >= c)
+        const double* Wk = W + (size_t)k * 64 * 64;
+        double u = 0.0;
+        for (int cp = c + g; cp < 64; cp += 16)
+            if (cp >= c) u += Wk[cp * 64 + c] * rvec[cp];
+        part[g][c] = u;
+        __syncthreads();
+        if (g == 0) {
+            double t = 0.0;
+            for (int q = 0; q < 16; ++q) t += part[q][c];
+            y[64 * k + c] = t;
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers used by the simulation driver (pbx_api.hip)
+
+struct PbxFlux {
+    FluxPlane f;
+};
+
+int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, double sq2dt, double mult,
+                     double* pos, double* disp, int N, const double* plane_n, const double* plane_p, int flux_on,
+                     const int* label, long long* flux, hipStream_t s) {
+    FluxPlane f;
+    for (int c = 0; c < 3; ++c) {
+        f.n[c] = plane_n ? plane_n[c] : 0.0;
+        f.p[c] = plane_p ? plane_p[c] : 0.0;
+    }
+    f.on = flux_on;
+    if (diag_only)
+        hipLaunchKernelGGL(stoch_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, B, xi, sq2dt, mult, pos, disp,
+                           N, f, label, flux);
+    else
+        hipLaunchKernelGGL(stoch_kernel, dim3(N), dim3(192), 0, s, B, ld, xi, sq2dt, mult, pos, disp, f, label,
+                           flux);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, double k, double mult, double* pos,
+                   int N, const double* plane_n, const double* plane_p, int flux_on, const int* label,
+                   long long* flux, hipStream_t s) {
+    FluxPlane f;
+    for (int c = 0; c < 3; ++c) {
+        f.n[c] = plane_n ? plane_n[c] : 0.0;
+        f.p[c] = plane_p ? plane_p[c] : 0.0;
+    }
+    f.on = flux_on;
+    if (diag_only)
+        hipLaunchKernelGGL(det_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, F, k, mult, N, pos, f, label,
+                           flux);
+    else
+        hipLaunchKernelGGL(det_kernel, dim3(N), dim3(192), 0, s, D, ld, F, k, mult, 3 * N, pos, f, label, flux);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
+                         const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s) {
+    FluxPlane f;
+    for (int c = 0; c < 3; ++c) {
+        f.n[c] = plane_n ? plane_n[c] : 0.0;
+        f.p[c] = plane_p ? plane_p[c] : 0.0;
+    }
+    f.on = flux_on;
+    hipLaunchKernelGGL(translate_kernel, dim3((N + 255) / 256), dim3(256), 0, s, v, sign, N, pos, f, label, flux);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s) {
+    PBX_CHECK_BOX(L);
+    int n3 = 3 * N;
+    hipLaunchKernelGGL(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_lower_gemv(const double* Lm, int n, int ld, const double* x, double* y, double scale, hipStream_t s) {
+    hipLaunchKernelGGL(lower_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, Lm, n, ld, x, y, scale);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const double* x, const double* z, double alpha,
+                     double* y, hipStream_t s) {
+    hipLaunchKernelGGL(drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, Lo, n, ld, x, z, alpha, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(det_drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D1, D0, n, ld, F, beta, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, hipStream_t s) {
+    hipLaunchKernelGGL(trsv_lt_kernel, dim3(1), dim3(1024), 0, s, Lm, np, ld, W, x, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}

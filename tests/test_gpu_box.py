@@ -1,0 +1,143 @@
+"""Trajectories of the device-resident Box against the oracle step
+(oracle/box_oracle.py) with the same seeded RandomState stream.
+
+Bar: positions within 1e-10 relative (max|dr| / max|r|) after every step,
+flux counts, move-attempt counts and draw counts identical."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.box_oracle import OracleBox
+
+pytestmark = pytest.mark.gpu
+
+FLUX = {"normal": [1.0, 0.0, 0.0], "plane_point": [0.0, 0.0, 0.0]}
+
+
+def _gpu_box(xyz, a, inp, labels=None):
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    labels = labels or ["XXX"] * len(a)
+    beads = [Bead(p, r, label=lab) for p, r, lab in zip(xyz, a, labels)]
+    return Box(beads, inp)
+
+
+def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None):
+    labels = labels or ["XXX"] * len(a)
+    g = _gpu_box(xyz, a, inp, labels)
+    o = OracleBox(xyz, a, labels, inp)
+    worst = 0.0
+    for j in range(steps):
+        f = flags(j) if flags else (True, True, True)
+        g.propagate(dt, *f)
+        o.propagate(dt, *f)
+        pos_g = np.array([b.r for b in g.beads])
+        err = np.abs(pos_g - o.r).max() / np.abs(o.r).max()
+        worst = max(worst, err)
+        assert err <= tol, "step %d: position error %.3e" % (j, err)
+        if "measure_flux" in inp:
+            assert g.net_flux == o.net_flux, "step %d flux %s vs %s" % (j, g.net_flux, o.net_flux)
+        assert g.last_move_attempts == o.attempt_log[-1]
+    assert g.draw_count == o.draw_count
+    return g, o, worst
+
+
+def _grid(coords, a):
+    xyz = np.array([[x, y, z] for x in coords for y in coords for z in coords], float)
+    return xyz, np.full(len(xyz), a)
+
+
+BASE = {"box_length": 35.0, "T": 298.0, "viscosity": 0.01, "seed": 0}
+
+
+def test_ermak_nohi_bit_exact():
+    xyz, a = _grid([-6.0, -3.0, 0.0, 3.0, 6.0], 1.0)
+    inp = dict(BASE, hydrodynamics="nohi", measure_flux=FLUX)
+    g, o, worst = _run_pair(xyz, a, inp, 100, 1e-10, tol=0.0)
+    assert worst == 0.0
+
+
+def test_ermak_rpy_grid():
+    xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
+    _run_pair(xyz, a, dict(BASE, hydrodynamics="rpy"), 100, 1e-10)
+
+
+@pytest.mark.parametrize("mn", [0, 1, 2])
+def test_ermak_rpy_smith_grid(mn):
+    xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy_smith", ewald_alpha=math.sqrt(math.pi), ewald_real=mn, ewald_imag=mn)
+    _run_pair(xyz, a, inp, 100, 1e-10)
+
+
+def test_c1_rpy_flux_first_200_steps():
+    from pybrown_amd.workloads import workload
+    xyz, a, inp, dt, _ = workload("c1")
+    _run_pair(xyz, a, inp, 200, dt)
+
+
+def test_rpy_smith_c2_density_with_flux():
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(120, 10.0)
+    Lb = box_for(a, 0.05)
+    xyz = rsa_positions(a, Lb, 2)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 0, "hydrodynamics": "rpy_smith",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "measure_flux": FLUX}
+    _run_pair(xyz, a, inp, 60, 1.0)
+
+
+def test_rejections_match():
+    # dense, large steps: many overlap rejections and redraws
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(40, 1.0)
+    Lb = box_for(a, 0.3)
+    xyz = rsa_positions(a, Lb, 8, min_gap=0.01)
+    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 3, "hydrodynamics": "rpy",
+           "measure_flux": FLUX}
+    g, o, _ = _run_pair(xyz, a, inp, 40, 2.0)
+    assert sum(o.attempt_log) > len(o.attempt_log), "workload produced no rejections"
+
+
+@pytest.mark.parametrize("mn,m_mid", [(1, 2.0), (2, 100.0)])
+def test_midpoint_rpy_smith_lub_grid(mn, m_mid):
+    # test_box.py:409-511 design (cutoff 10, grid spacing 2.2)
+    xyz, a = _grid([-2.2, 0.0, 2.2], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy_smith_lub", propagation_scheme="midpoint", lubrication_cutoff=10,
+               ewald_alpha=math.sqrt(math.pi), ewald_real=mn, ewald_imag=mn, m_midpoint=m_mid)
+    _run_pair(xyz, a, inp, 30, 1e-10)
+
+
+def test_midpoint_c3_like_bidisperse():
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.array([20.0] * 30 + [10.0] * 30)
+    Lb = box_for(a, 0.25)
+    xyz = rsa_positions(a, Lb, 4, min_gap=0.1)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 1, "hydrodynamics": "rpy_smith_lub",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "lubrication_cutoff": 0.5,
+           "propagation_scheme": "midpoint", "m_midpoint": 100, "overlap_treshold": 0.1, "measure_flux": FLUX}
+    _run_pair(xyz, a, inp, 20, 0.1)
+
+
+def test_ermak_rpy_lub_with_cadence_flags():
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(45, 1.0)
+    Lb = box_for(a, 0.2)
+    xyz = rsa_positions(a, Lb, 6, min_gap=0.05)
+    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 5, "hydrodynamics": "rpy_lub",
+           "lubrication_cutoff": 1.0, "measure_flux": FLUX}
+    # diff_freq 3, lub_freq 2, chol_freq 2 (BD.py:177)
+    _run_pair(xyz, a, inp, 24, 0.05, flags=lambda j: (j % 3 == 0, j % 2 == 0, j % 2 == 0))
+
+
+def test_external_force_drift():
+    xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy", external_force=[1e-11, 0.0, -2e-11], measure_flux=FLUX)
+    _run_pair(xyz, a, inp, 30, 1e-3)
+
+
+def test_labels_split_flux():
+    xyz, a = _grid([-6.0, -3.0, 0.0, 3.0, 6.0], 1.0)
+    labels = ["A" if i % 2 else "B" for i in range(len(a))]
+    inp = dict(BASE, hydrodynamics="rpy", measure_flux=FLUX)
+    g, o, _ = _run_pair(xyz, a, inp, 50, 0.5, labels=labels)
+    assert set(g.net_flux) == {"A", "B"}
