@@ -21,6 +21,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+def cpu_baseline(name, steps=3):
+    """Reference CPU path on the host cores: the reference's clib kernels
+    (oracle/_ref, built from its sources; the restatement oracle/liborc.so if
+    that build is absent) driven by the box.py step restatement with
+    numpy.linalg.cholesky, one BLAS thread as the reference's inert OpenMP
+    build runs.  Bounded sample: 1 warm-up step + `steps` timed steps."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    from oracle.box_oracle import OracleBox
+    from oracle.orc import ref_available
+    from pybrown_amd.workloads import workload
+
+    kind = "reference" if ref_available() else "port"
+    xyz, a, inp, dt, _ = workload(name, seed=0)
+    with threadpool_limits(limits=1):
+        box = OracleBox(xyz, a, ["XXX"] * len(a), inp, lib="ref" if kind == "reference" else "orc")
+        box.propagate(dt)
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            box.propagate(dt)
+            times.append(time.perf_counter() - t0)
+    per_step = float(np.median(times))
+    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": 1, "kind": kind,
+            "sample": "%d timed steps (after 1 warm-up) of the same N=%d workload; median %.3f s/step; "
+                      "clib kernels built -O2 from the reference sources + numpy.linalg.cholesky, 1 thread"
+                      % (steps, len(a), per_step)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

@@ -129,6 +129,11 @@ int pbx_sim_get_diag(pbx_sim* sim, int which, double* out);
 long long pbx_sim_draw_count(pbx_sim* sim);
 int pbx_sim_last_attempts(pbx_sim* sim);
 int pbx_sim_sync(pbx_sim* sim);
+/* Stage timing with HIP events on the simulation's stream (no extra sync
+ * when off).  get_timing fills stage_ms[3] = cumulative milliseconds in
+ * assembly (+lubrication), Cholesky, propagation; returns timed steps. */
+int pbx_sim_set_timing(pbx_sim* sim, int on);
+long long pbx_sim_get_timing(pbx_sim* sim, double* stage_ms);
 
 /* ------------------------------------------------------------------------
  * 4. Errors

@@ -78,6 +78,8 @@ _sig("pbx_sim_get_diag", _i, [ctypes.c_void_p, _i, _dp])
 _sig("pbx_sim_draw_count", ctypes.c_longlong, [ctypes.c_void_p])
 _sig("pbx_sim_last_attempts", _i, [ctypes.c_void_p])
 _sig("pbx_sim_sync", _i, [ctypes.c_void_p])
+_sig("pbx_sim_set_timing", _i, [ctypes.c_void_p, _i])
+_sig("pbx_sim_get_timing", ctypes.c_longlong, [ctypes.c_void_p, _dp])
 # errors
 _sig("pbx_last_error", ctypes.c_char_p, [])
 _sig("pbx_last_error_code", _i, [])
@@ -90,6 +92,7 @@ EXPORTED = [
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_propagate",
     "pbx_sim_get_positions", "pbx_sim_set_positions", "pbx_sim_get_flux", "pbx_sim_get_matrix",
     "pbx_sim_get_diag", "pbx_sim_draw_count", "pbx_sim_last_attempts", "pbx_sim_sync",
+    "pbx_sim_set_timing", "pbx_sim_get_timing",
     "pbx_last_error", "pbx_last_error_code", "pbx_clear_error", "pbx_version",
 ]
 

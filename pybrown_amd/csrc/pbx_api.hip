@@ -307,6 +307,11 @@ struct PbxSim {
     int last_attempts = 0;
     int have_D = 0, have_B = 0, have_Rff = 0;
     std::vector<void*> owned;
+    // optional stage timing with HIP events on the simulation stream
+    int timing = 0;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double stage_ms[3] = {0, 0, 0};  // assembly (+lubrication), factorisation, propagation
+    long long timed_steps = 0;
 };
 
 namespace {
@@ -649,6 +654,8 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
 extern "C" int pbx_sim_destroy(pbx_sim* s) {
     if (!s) return PBX_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int k = 0; k < 4; ++k)
+        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
     for (void* p : s->owned) (void)hipFree(p);
     if (s->xi_host) (void)hipHostFree(s->xi_host);
     if (s->flag_host) (void)hipHostFree(s->flag_host);
@@ -677,6 +684,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         int nl = cf.n_labels > 0 ? cf.n_labels : 1;
         PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
     }
+    if (s->timing) PBX_HIP(hipEventRecord(s->ev[0], s->stream));
     if (cf.hydrodynamics != PBX_NOHI) {
         if (build_Dff_flag) {
             TRY(build_Dff(s, s->D, s->Rff));
@@ -690,6 +698,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
             TRY(build_Dtot(s, s->Rff, s->D));
             s->have_D = 1;
         }
+        if (s->timing) PBX_HIP(hipEventRecord(s->ev[1], s->stream));
         if (cholesky_flag) {
             if (!s->have_D) {
                 pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
@@ -698,6 +707,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
             TRY(factor(s));
             s->have_B = 1;
         }
+        if (s->timing) PBX_HIP(hipEventRecord(s->ev[2], s->stream));
         if (!s->have_B) {
             pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
             return PBX_ERR_STATE;
@@ -710,8 +720,37 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     else
         TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
     TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
+    if (s->timing && cf.hydrodynamics != PBX_NOHI) {
+        PBX_HIP(hipEventRecord(s->ev[3], s->stream));
+        PBX_HIP(hipEventSynchronize(s->ev[3]));
+        float t01 = 0, t12 = 0, t23 = 0;
+        PBX_HIP(hipEventElapsedTime(&t01, s->ev[0], s->ev[1]));
+        PBX_HIP(hipEventElapsedTime(&t12, s->ev[1], s->ev[2]));
+        PBX_HIP(hipEventElapsedTime(&t23, s->ev[2], s->ev[3]));
+        s->stage_ms[0] += t01;
+        s->stage_ms[1] += t12;
+        s->stage_ms[2] += t23;
+        s->timed_steps += 1;
+    }
     PBX_HIP(hipStreamSynchronize(s->stream));
     return PBX_OK;
+}
+
+// Stage timing with HIP events recorded on the simulation's stream:
+// [0] mobility assembly (+ lubrication and inverses), [1] Cholesky,
+// [2] propagation (noise products, overlap checks, flux, wrap).
+extern "C" int pbx_sim_set_timing(pbx_sim* s, int on) {
+    if (on && !s->ev[0])
+        for (int k = 0; k < 4; ++k) PBX_HIP(hipEventCreate(&s->ev[k]));
+    s->timing = on;
+    s->stage_ms[0] = s->stage_ms[1] = s->stage_ms[2] = 0.0;
+    s->timed_steps = 0;
+    return PBX_OK;
+}
+
+extern "C" long long pbx_sim_get_timing(pbx_sim* s, double* stage_ms) {
+    for (int k = 0; k < 3; ++k) stage_ms[k] = s->stage_ms[k];
+    return s->timed_steps;
 }
 
 extern "C" int pbx_sim_get_positions(pbx_sim* s, double* xyz) {
