@@ -64,6 +64,10 @@ int pbx_cholesky(const double* A, int n, double* L);
 /* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
 
+/* Selects the diagonal-tile factorisation used by every Cholesky in this
+ * process: 1 = column by column, 2 = 8-column panels (default). */
+int pbx_set_potrf_variant(int variant);
+
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
  * FP64 MFMA tile product used by the factorisations, for testing. */
 int pbx_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K);

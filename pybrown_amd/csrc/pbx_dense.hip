@@ -126,6 +126,127 @@ __device__ void tile_potrf_inv(double* sL, double* sW, double* sd, int col0, int
     __syncthreads();
 }
 
+// Broadcast a double from a uniform lane (v_readlane on both halves).
+__device__ __forceinline__ double bcast(double x, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// 1/sqrt(d): hardware estimate refined by two Newton steps (FMA), then
+// sqrt(d) = d * rsqrt(d).  Within an ulp or two of the correctly rounded
+// values; non-positive or NaN d propagates NaN.
+__device__ __forceinline__ double rsqrt_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    double h = 0.5 * d;
+    double r = fma(-h * y, y, 0.5);
+    y = fma(y, r, y);
+    r = fma(-h * y, y, 0.5);
+    y = fma(y, r, y);
+    return y;
+}
+
+// Same contract as tile_potrf_inv with far fewer workgroup barriers:
+// 8-column panels are factored by wave 0 alone (lane = row, panel in
+// registers, pivots broadcast with readlane); the trailing update
+// A22 -= P P^T (K = 8) runs on the FP64 matrix cores across all four
+// waves; rows of inv(L) for the panel are formed by waves 1-3 once the
+// panel rows are final.
+__device__ void tile_potrf_inv_v2(double* sL, double* sW, double* sd, int col0, int* info) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int e = tid; e < T64 * SP; e += blockDim.x) sW[e] = 0.0;
+    for (int kb = 0; kb < 8; ++kb) {
+        const int c0 = kb * 8;
+        if (w == 0) {
+            const bool act = lane >= c0;
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = act ? sL[lane * SP + c0 + q] : 0.0;
+            int first_bad = -1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = c0 + q;
+                const double d = bcast(v[q], j);
+                if (!(d > 0.0) && first_bad < 0) first_bad = q;
+                const double y = rsqrt_nr(d);
+                if (lane == j) {
+                    v[q] = d * y;
+                    sd[j] = d * y;
+                } else if (lane > j) {
+                    v[q] = v[q] * y;
+                }
+#pragma unroll
+                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] -= v[q] * bcast(v[q], c0 + q2);
+            }
+            if (act) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
+            }
+            if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
+        }
+        __syncthreads();
+        // inv(L) rows c0..c0+7 (waves 1-3): W[r][c] for c <= r, sequential in r
+        // within the panel; uses L rows r (final) and W rows < r.
+        if (w > 0) {
+            const int t = tid - 64;  // 192 threads: column c = t % 64, split = t / 64
+            const int c = t & 63, part = t >> 6;
+            for (int r = c0; r < c0 + 8; ++r) {
+                double s = 0.0;
+                if (c <= r) {
+                    for (int k = c + part; k < r; k += 3) s -= sL[r * SP + k] * sW[k * SP + c];
+                }
+                // reduce the three partial sums through LDS scratch in sW's pad column
+                __shared__ double red[3][64];
+                red[part][c] = s;
+                __builtin_amdgcn_s_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+                if (part == 0 && c <= r) {
+                    double tot = red[0][c] + red[1][c] + red[2][c] + (c == r ? 1.0 : 0.0);
+                    sW[r * SP + c] = tot / sd[r];
+                }
+                __builtin_amdgcn_s_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            }
+        } else {
+            // wave 0 meets the waves-1..3 barriers of the inverse rows
+            for (int r = 0; r < 16; ++r) {
+                __builtin_amdgcn_s_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+            }
+        }
+        // trailing update of rows/cols >= c0+8 on the matrix cores
+        if (kb < 7) {
+            const int r0 = (w >> 1) * 32, cq = (w & 1) * 32;
+            const int li = lane & 15, lk = lane >> 4;
+            const int lo = c0 + 8;
+            const bool live = (r0 + 31 >= lo) && !(cq > r0 + 31);
+            v4d acc[2][2];
+            acc_zero(acc);
+            if (live) {
+#pragma unroll
+                for (int ks = 0; ks < 8; ks += 4) {
+                    const int k = c0 + ks + lk;
+                    double a0 = sL[(r0 + li) * SP + k], a1 = sL[(r0 + 16 + li) * SP + k];
+                    double b0 = sL[(cq + li) * SP + k], b1 = sL[(cq + 16 + li) * SP + k];
+                    acc[0][0] = mfma(a0, b0, acc[0][0]);
+                    acc[0][1] = mfma(a0, b1, acc[0][1]);
+                    acc[1][0] = mfma(a1, b0, acc[1][0]);
+                    acc[1][1] = mfma(a1, b1, acc[1][1]);
+                }
+            }
+            __syncthreads();
+            if (live) {
+                acc_for_each(acc, [&](int r, int c, double v) {
+                    if (r >= lo && c >= lo && c <= r) sL[r * SP + c] -= v;
+                });
+            }
+            __syncthreads();
+        }
+    }
+    if (tid < 64) sL[tid * SP + tid] = sd[tid];
+    __syncthreads();
+}
+
 // Write the factored diagonal tile (lower incl. diagonal) and its inverse.
 __device__ void store_diag(const double* sL, const double* sW, double* A, int ld, int k, double* work) {
     double* Akk = A + (size_t)k * T64 * ld + (size_t)k * T64;
@@ -137,6 +258,15 @@ __device__ void store_diag(const double* sL, const double* sW, double* A, int ld
     }
 }
 
+template <int V>
+__device__ __forceinline__ void factor_tile(double* sL, double* sW, double* sd, int col0, int* info) {
+    if (V == 2)
+        tile_potrf_inv_v2(sL, sW, sd, col0, info);
+    else
+        tile_potrf_inv(sL, sW, sd, col0, info);
+}
+
+template <int V>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info) {
     __shared__ double sL[T64 * SP];
     __shared__ double sW[T64 * SP];
@@ -147,7 +277,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int 
         sL[r * SP + c] = Akk[(size_t)r * ld + c];
     }
     __syncthreads();
-    tile_potrf_inv(sL, sW, sd, k * T64, info);
+    factor_tile<V>(sL, sW, sd, k * T64, info);
     store_diag(sL, sW, A, ld, k, work);
 }
 
@@ -176,6 +306,7 @@ __device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
 
 // A_ij -= L_ik L_jk^T over the trailing lower tiles; the workgroup owning
 // (k+1, k+1) (blockIdx 0) then factors that tile.
+template <int V>
 __global__ __launch_bounds__(256) void syrk_kernel(double* A, int ld, int k, double* work, int* info) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
@@ -208,7 +339,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(double* A, int ld, int k, dou
     double* sW = sB;
     acc_for_each(acc, [&](int r, int c, double v) { sL[r * SP + c] = Aij[(size_t)r * ld + c] - v; });
     __syncthreads();
-    tile_potrf_inv(sL, sW, sd, (k + 1) * T64, info);
+    factor_tile<V>(sL, sW, sd, (k + 1) * T64, info);
     store_diag(sL, sW, A, ld, k + 1, work);
 }
 
@@ -327,6 +458,15 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const dou
 
 size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64; }
 
+// Diagonal-tile factorisation variant: 1 = column-by-column (two barriers per
+// column), 2 = 8-column panels with MFMA trailing updates.
+static int g_potrf_variant = 2;
+extern "C" int pbx_set_potrf_variant(int v) {
+    if (v != 1 && v != 2) return PBX_ERR_ARG;
+    g_potrf_variant = v;
+    return PBX_OK;
+}
+
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s) {
     if (np == n) return PBX_OK;
     size_t total = (size_t)np * np;
@@ -351,11 +491,18 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         return PBX_ERR_ARG;
     }
     const int T = np / T64;
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);
+    const bool v2 = g_potrf_variant == 2;
+    if (v2)
+        hipLaunchKernelGGL(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);
+    else
+        hipLaunchKernelGGL(potrf_diag_kernel<1>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);
     for (int k = 0; k + 1 < T; ++k) {
         hipLaunchKernelGGL(trsm_kernel, dim3(T - 1 - k), dim3(256), 0, s, A, ld, k, work);
         int cnt = (T - 1 - k) * (T - k) / 2;
-        hipLaunchKernelGGL(syrk_kernel, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
+        if (v2)
+            hipLaunchKernelGGL(syrk_kernel<2>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
+        else
+            hipLaunchKernelGGL(syrk_kernel<1>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
     }
     PBX_HIP(hipGetLastError());
     return PBX_OK;

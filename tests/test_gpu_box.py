@@ -87,14 +87,11 @@ def test_rpy_smith_c2_density_with_flux():
 
 
 def test_rejections_match():
-    # dense, large steps: many overlap rejections and redraws
-    from pybrown_amd.workloads import rsa_positions, box_for
-    a = np.full(40, 1.0)
-    Lb = box_for(a, 0.3)
-    xyz = rsa_positions(a, Lb, 8, min_gap=0.01)
-    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 3, "hydrodynamics": "rpy",
-           "measure_flux": FLUX}
-    g, o, _ = _run_pair(xyz, a, inp, 40, 2.0)
+    # beads 0.3 A apart: overlap rejections and redraws on most steps
+    xyz, a = _grid([-2.3, 0.0, 2.3], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy", seed=3, measure_flux=FLUX)
+    g, o, _ = _run_pair(xyz, a, inp, 30, 0.02)
+    print("move attempts per step:", o.attempt_log)
     assert sum(o.attempt_log) > len(o.attempt_log), "workload produced no rejections"
 
 
@@ -118,26 +115,27 @@ def test_midpoint_c3_like_bidisperse():
     _run_pair(xyz, a, inp, 20, 0.1)
 
 
-def test_ermak_rpy_lub_with_cadence_flags():
+@pytest.mark.parametrize("hyd", ["rpy_lub", "rpy_smith_lub"])
+def test_ermak_lub_with_cadence_flags(hyd):
     from pybrown_amd.workloads import rsa_positions, box_for
     a = np.full(45, 1.0)
-    Lb = box_for(a, 0.2)
+    Lb = box_for(a, 0.1)
     xyz = rsa_positions(a, Lb, 6, min_gap=0.05)
-    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 5, "hydrodynamics": "rpy_lub",
-           "lubrication_cutoff": 1.0, "measure_flux": FLUX}
+    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 5, "hydrodynamics": hyd,
+           "ewald_real": 1, "ewald_imag": 1, "lubrication_cutoff": 1.0, "measure_flux": FLUX}
     # diff_freq 3, lub_freq 2, chol_freq 2 (BD.py:177)
     _run_pair(xyz, a, inp, 24, 0.05, flags=lambda j: (j % 3 == 0, j % 2 == 0, j % 2 == 0))
 
 
 def test_external_force_drift():
     xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
-    inp = dict(BASE, hydrodynamics="rpy", external_force=[1e-11, 0.0, -2e-11], measure_flux=FLUX)
-    _run_pair(xyz, a, inp, 30, 1e-3)
+    inp = dict(BASE, hydrodynamics="rpy", external_force=[2e-21, 0.0, -3e-21], measure_flux=FLUX)
+    _run_pair(xyz, a, inp, 30, 0.01)
 
 
 def test_labels_split_flux():
     xyz, a = _grid([-6.0, -3.0, 0.0, 3.0, 6.0], 1.0)
     labels = ["A" if i % 2 else "B" for i in range(len(a))]
-    inp = dict(BASE, hydrodynamics="rpy", measure_flux=FLUX)
-    g, o, _ = _run_pair(xyz, a, inp, 50, 0.5, labels=labels)
+    inp = dict(BASE, hydrodynamics="rpy", seed=3, measure_flux=FLUX)
+    g, o, _ = _run_pair(xyz, a, inp, 20, 0.05, labels=labels)
     assert set(g.net_flux) == {"A", "B"}

@@ -137,8 +137,16 @@ def test_lubrication_accumulates(orc):
     np.testing.assert_allclose(C.reshape(9, 9), Co, rtol=1e-12, atol=1e-12 * np.abs(Co).max())
 
 
+@pytest.fixture(params=[1, 2], ids=["potrf_v1", "potrf_v2"])
+def potrf_variant(request):
+    from pybrown_amd._lib import lib
+    assert lib.pbx_set_potrf_variant(request.param) == 0
+    yield request.param
+    lib.pbx_set_potrf_variant(2)
+
+
 @pytest.mark.parametrize("n", [64, 100, 300, 1000, 3000])
-def test_cholesky_vs_numpy(orc, n):
+def test_cholesky_vs_numpy(orc, n, potrf_variant):
     from pybrown_amd.diffusion import cholesky
     from pybrown_amd.workloads import rsa_positions, box_for
     from oracle.orc import Oracle
@@ -160,7 +168,7 @@ def test_cholesky_vs_numpy(orc, n):
     assert np.all(np.triu(Lg, 1) == 0)
 
 
-def test_cholesky_reports_non_spd():
+def test_cholesky_reports_non_spd(potrf_variant):
     from pybrown_amd.diffusion import cholesky
     A = np.eye(130)
     A[70, 70] = -1.0
@@ -169,7 +177,7 @@ def test_cholesky_reports_non_spd():
 
 
 @pytest.mark.parametrize("n", [64, 200, 700])
-def test_spd_inverse_vs_numpy(n):
+def test_spd_inverse_vs_numpy(n, potrf_variant):
     from pybrown_amd.diffusion import spd_inverse
     rng = np.random.RandomState(n)
     X = rng.normal(size=(n, n))
