@@ -58,6 +58,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1, 2, 3; 0 = library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -72,6 +73,9 @@ def main():
     from pybrown_amd.workloads import workload
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from pybrown_amd._lib import lib
+    if args.potrf:
+        assert lib.pbx_set_potrf_variant(args.potrf) == 0
     xyz, a, inp, dt, _ = workload(args.workload, seed=0)
     box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp, xi_pool_attempts=64)
     for _ in range(args.warmup):
@@ -83,6 +87,17 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
+    # stage breakdown with HIP events on the simulation's stream (separate steps)
+    import ctypes
+    import numpy as np
+    lib.pbx_sim_set_timing(box._sim, 1)
+    for _ in range(min(10, args.steps)):
+        box.propagate(dt)
+    st = np.zeros(3)
+    nt = lib.pbx_sim_get_timing(box._sim, st.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    lib.pbx_sim_set_timing(box._sim, 0)
+    print("stage ms/step: assembly %.3f  cholesky %.3f  propagation %.3f  (over %d steps; wall %.3f)"
+          % (st[0] / nt, st[1] / nt, st[2] / nt, nt, ms), file=sys.stderr)
     out = {
         "metric": "steps/s at N=1000 RPY-Ewald",
         "value": 1e3 / ms,

@@ -64,8 +64,10 @@ int pbx_cholesky(const double* A, int n, double* L);
 /* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
 
-/* Selects the diagonal-tile factorisation used by every Cholesky in this
- * process: 1 = column by column, 2 = 8-column panels (default). */
+/* Selects the Cholesky used by every factorisation in this process:
+ * 1 = one launch per panel, column-by-column diagonal tiles;
+ * 2 = one launch per panel, 8-column-panel diagonal tiles (default);
+ * 3 = one persistent launch over the tile dependency graph. */
 int pbx_set_potrf_variant(int variant);
 
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the

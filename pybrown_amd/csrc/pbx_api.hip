@@ -238,6 +238,10 @@ extern "C" int pbx_cholesky(const double* A, int n, double* L) {
                         hipMemcpyDeviceToHost));
     for (int r = 0; r < n; ++r)
         for (int c = r + 1; c < n; ++c) L[(size_t)r * n + c] = 0.0;
+    if (info == PBX_INFO_DAG_WAIT) {
+        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        return PBX_ERR_STATE;
+    }
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
     return info;
 }
@@ -261,6 +265,10 @@ extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double
     PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
     PBX_HIP(hipMemcpy2D(X, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
                         hipMemcpyDeviceToHost));
+    if (info == PBX_INFO_DAG_WAIT) {
+        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        return PBX_ERR_STATE;
+    }
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
     return info;
 }
@@ -352,8 +360,12 @@ int check_info(PbxSim* s) {
     PBX_HIP(hipStreamSynchronize(s->stream));
     info = s->flag_host[1];
     if (info) {
-        pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
         PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
+        if (info == PBX_INFO_DAG_WAIT) {
+            pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+            return PBX_ERR_STATE;
+        }
+        pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
     }
     return info;
 }

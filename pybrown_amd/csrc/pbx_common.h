@@ -22,6 +22,9 @@
 #define PBX_ERR_OVERLAP -5   // starting configuration overlaps
 #define PBX_ERR_STATE -6
 // k > 0: the Cholesky factorisation met a non-positive pivot at column k-1
+// (except PBX_INFO_DAG_WAIT: a dependency wait of the single-launch Cholesky
+// ran out of budget; the kernel drained and the factor is not valid)
+#define PBX_INFO_DAG_WAIT 0x40000000
 
 static inline size_t pbx_round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 

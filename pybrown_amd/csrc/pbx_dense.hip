@@ -19,6 +19,12 @@
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
 #include "pbx_common.h"
 
+#define TRY_RC(x)            \
+    do {                     \
+        int rc_ = (x);       \
+        if (rc_) return rc_; \
+    } while (0)
+
 namespace {
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -45,7 +51,8 @@ __device__ __forceinline__ void stage(double* s, const double* X, int ld, bool r
 }
 
 // acc += op(A) op(B) for one 64-deep slab, operands staged k-major.
-// acc[x][y][v] <-> C[r0 + 16x + 4*(lane/16) + v][c0 + 16y + lane%16]
+// acc[x][y][v] <-> C[r0 + 16x + lane/16 + 4v][c0 + 16y + lane%16] (the f64 16x16x4
+// accumulator interleaves rows: register v of lane group g holds row g + 4v)
 __device__ __forceinline__ void mma64(const double* sA, const double* sB, v4d acc[2][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
@@ -76,7 +83,7 @@ __device__ __forceinline__ void acc_for_each(v4d acc[2][2], F f) {
     const int li = lane & 15, lk = lane >> 4;
     for (int x = 0; x < 2; ++x)
         for (int y = 0; y < 2; ++y)
-            for (int v = 0; v < 4; ++v) f(r0 + 16 * x + 4 * lk + v, c0 + 16 * y + li, acc[x][y][v]);
+            for (int v = 0; v < 4; ++v) f(r0 + 16 * x + lk + 4 * v, c0 + 16 * y + li, acc[x][y][v]);
 }
 
 // Factor the SPD 64 x 64 tile held in LDS (lower triangle, stride SP) and
@@ -456,13 +463,18 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const dou
 
 }  // namespace
 
-size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64; }
+size_t pbx_dag_state_ints(int np);
+int pbx_potrf_dag(double* A, int np, int ld, double* work, int* info_dev, int* state, int grid, hipStream_t s);
 
-// Diagonal-tile factorisation variant: 1 = column-by-column (two barriers per
-// column), 2 = 8-column panels with MFMA trailing updates.
+// work = [T inverse tiles of the diagonal factor][single-launch DAG state]
+size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64 + (pbx_dag_state_ints(np) + 1) / 2; }
+
+// Cholesky variant: 1 = launch per panel, column-by-column diagonal tiles;
+// 2 = launch per panel, 8-column panel diagonal tiles; 3 = single persistent
+// launch over the tile dependency graph (pbx_chol_dag.hip).
 static int g_potrf_variant = 2;
 extern "C" int pbx_set_potrf_variant(int v) {
-    if (v != 1 && v != 2) return PBX_ERR_ARG;
+    if (v < 1 || v > 3) return PBX_ERR_ARG;
     g_potrf_variant = v;
     return PBX_OK;
 }
@@ -491,6 +503,11 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         return PBX_ERR_ARG;
     }
     const int T = np / T64;
+    if (g_potrf_variant == 3) {
+        int* state = (int*)(work + (size_t)np * T64);
+        TRY_RC(pbx_potrf_dag(A, np, ld, work, info_dev, state, 0, s));
+        return PBX_OK;
+    }
     const bool v2 = g_potrf_variant == 2;
     if (v2)
         hipLaunchKernelGGL(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);

@@ -139,3 +139,19 @@ def test_labels_split_flux():
     inp = dict(BASE, hydrodynamics="rpy", seed=3, measure_flux=FLUX)
     g, o, _ = _run_pair(xyz, a, inp, 20, 0.05, labels=labels)
     assert set(g.net_flux) == {"A", "B"}
+
+
+def test_single_launch_cholesky_trajectory():
+    # the persistent tile-graph Cholesky inside the step (variant 3)
+    from pybrown_amd._lib import lib
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(200, 10.0)
+    Lb = box_for(a, 0.05)
+    xyz = rsa_positions(a, Lb, 12)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 2, "hydrodynamics": "rpy_smith",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "measure_flux": FLUX}
+    assert lib.pbx_set_potrf_variant(3) == 0
+    try:
+        _run_pair(xyz, a, inp, 10, 1.0)
+    finally:
+        lib.pbx_set_potrf_variant(2)
