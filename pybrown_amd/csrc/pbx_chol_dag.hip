@@ -17,157 +17,11 @@
 // its budget raises the error word, after which all remaining waits return
 // at once and the kernel drains; the host then reports the failure.
 #include "pbx_common.h"
+#include "pbx_tile.h"
+
+using namespace pbx_tile;
 
 namespace {
-
-typedef double v4d __attribute__((ext_vector_type(4)));
-constexpr int T64 = 64;
-constexpr int SP = 65;
-
-__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ void stage_rk(double* s, const double* X, int ld) {
-    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
-        int row = e >> 6, col = e & 63;
-        s[col * SP + row] = X[(size_t)row * ld + col];
-    }
-}
-
-__device__ __forceinline__ void mma64(const double* sA, const double* sB, v4d acc[2][2]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 4
-    for (int k = 0; k < T64; k += 4) {
-        const double* pa = sA + (k + lk) * SP;
-        const double* pb = sB + (k + lk) * SP;
-        double a0 = pa[r0 + li], a1 = pa[r0 + 16 + li];
-        double b0 = pb[c0 + li], b1 = pb[c0 + 16 + li];
-        acc[0][0] = mfma(a0, b0, acc[0][0]);
-        acc[0][1] = mfma(a0, b1, acc[0][1]);
-        acc[1][0] = mfma(a1, b0, acc[1][0]);
-        acc[1][1] = mfma(a1, b1, acc[1][1]);
-    }
-}
-
-template <class F>
-__device__ __forceinline__ void acc_for_each(v4d acc[2][2], F f) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y)
-            for (int v = 0; v < 4; ++v) f(r0 + 16 * x + lk + 4 * v, c0 + 16 * y + li, acc[x][y][v]);
-}
-
-__device__ __forceinline__ double bcast(double x, int lane) {
-    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
-    return __hiloint2double(hi, lo);
-}
-
-__device__ __forceinline__ double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    double h = 0.5 * d;
-    double r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    return y;
-}
-
-// Factor the 64 x 64 SPD tile in LDS and form inv(L) in sW (lower, zero
-// above); 8-column panels on wave 0, MFMA trailing updates, inverse rows on
-// waves 1-3 (same algorithm as the launch-based path's v2 tile routine).
-__device__ void tile_factor(double* sL, double* sW, double* sd, double (*red)[64], int col0, int* info) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int e = tid; e < T64 * SP; e += blockDim.x) sW[e] = 0.0;
-    for (int kb = 0; kb < 8; ++kb) {
-        const int c0 = kb * 8;
-        if (w == 0) {
-            const bool act = lane >= c0;
-            double v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = act ? sL[lane * SP + c0 + q] : 0.0;
-            int first_bad = -1;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int j = c0 + q;
-                const double d = bcast(v[q], j);
-                if (!(d > 0.0) && first_bad < 0) first_bad = q;
-                const double y = rsqrt_nr(d);
-                if (lane == j) {
-                    v[q] = d * y;
-                    sd[j] = d * y;
-                } else if (lane > j) {
-                    v[q] = v[q] * y;
-                }
-#pragma unroll
-                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] -= v[q] * bcast(v[q], c0 + q2);
-            }
-            if (act) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
-            }
-            if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
-        }
-        __syncthreads();
-        if (w > 0) {
-            const int t = tid - 64;
-            const int c = t & 63, part = t >> 6;
-            for (int r = c0; r < c0 + 8; ++r) {
-                double s = 0.0;
-                if (c <= r)
-                    for (int k = c + part; k < r; k += 3) s -= sL[r * SP + k] * sW[k * SP + c];
-                red[part][c] = s;
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                if (part == 0 && c <= r) {
-                    double tot = red[0][c] + red[1][c] + red[2][c] + (c == r ? 1.0 : 0.0);
-                    sW[r * SP + c] = tot / sd[r];
-                }
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            }
-        } else {
-            for (int r = 0; r < 16; ++r) {
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            }
-        }
-        if (kb < 7) {
-            const int r0 = (w >> 1) * 32, cq = (w & 1) * 32;
-            const int li = lane & 15, lk = lane >> 4;
-            const int lo = c0 + 8;
-            const bool live = (r0 + 31 >= lo) && !(cq > r0 + 31);
-            v4d acc[2][2];
-            for (int x = 0; x < 2; ++x)
-                for (int y = 0; y < 2; ++y) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
-            if (live) {
-#pragma unroll
-                for (int ks = 0; ks < 8; ks += 4) {
-                    const int k = c0 + ks + lk;
-                    double a0 = sL[(r0 + li) * SP + k], a1 = sL[(r0 + 16 + li) * SP + k];
-                    double b0 = sL[(cq + li) * SP + k], b1 = sL[(cq + 16 + li) * SP + k];
-                    acc[0][0] = mfma(a0, b0, acc[0][0]);
-                    acc[0][1] = mfma(a0, b1, acc[0][1]);
-                    acc[1][0] = mfma(a1, b0, acc[1][0]);
-                    acc[1][1] = mfma(a1, b1, acc[1][1]);
-                }
-            }
-            __syncthreads();
-            if (live)
-                acc_for_each(acc, [&](int r, int c, double v) {
-                    if (r >= lo && c >= lo && c <= r) sL[r * SP + c] -= v;
-                });
-            __syncthreads();
-        }
-    }
-    if (tid < 64) sL[tid * SP + tid] = sd[tid];
-    __syncthreads();
-}
 
 struct DagState {
     int ticket;
@@ -220,12 +74,11 @@ __global__ void dag_reset_kernel(int* state, int T) {
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) state[e] = 0;
 }
 
-__global__ __launch_bounds__(256) void chol_dag_kernel(double* A, int ld, int T, double* work, int* info,
+__global__ __launch_bounds__(256, 2) void chol_dag_kernel(double* A, int ld, int T, double* work, int* info,
                                                        int* state, long long spin_limit) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
-    __shared__ double red[3][64];
     __shared__ int task;
     DagState* st = (DagState*)state;
     int* done = state + 16;
@@ -244,25 +97,32 @@ __global__ __launch_bounds__(256) void chol_dag_kernel(double* A, int ld, int T,
         }
         const int i = j + (t - base);
         bool ok = true;
+        double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
+        double cv[16];
+        acc_load(Aij, ld, cv);
         v4d acc[2][2];
-        for (int x = 0; x < 2; ++x)
-            for (int y = 0; y < 2; ++y) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+        acc_zero(acc);
         for (int k = 0; k < j && ok; ++k) {
             ok = wait_done(done + i * T + k, &st->err, spin_limit);
             if (ok && i != j) ok = wait_done(done + j * T + k, &st->err, spin_limit);
             if (!ok) break;
-            stage_rk(sA, A + (size_t)i * T64 * ld + (size_t)k * T64, ld);
-            if (i != j) stage_rk(sB, A + (size_t)j * T64 * ld + (size_t)k * T64, ld);
+            stage<true>(sA, A + (size_t)i * T64 * ld + (size_t)k * T64, ld);
+            if (i != j) stage<true>(sB, A + (size_t)j * T64 * ld + (size_t)k * T64, ld);
             __syncthreads();
             mma64(sA, i == j ? sA : sB, acc);
             __syncthreads();
         }
         if (!ok) continue;  // error raised: drain the remaining tickets
-        double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
         if (i == j) {
-            acc_for_each(acc, [&](int r, int c, double v) { sA[r * SP + c] = Aij[(size_t)r * ld + c] - v; });
+            int q = 0;
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v, ++q) sA[acc_row(x, v) * SP + acc_col(y)] = cv[q] - acc[x][y][v];
             __syncthreads();
-            tile_factor(sA, sB, sd, red, j * T64, info);
+            tile_factor(sA, sB, sd, j * T64, info);
             double* Wj = work + (size_t)j * T64 * T64;
             for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
                 int r = e >> 6, c = e & 63;
@@ -272,14 +132,19 @@ __global__ __launch_bounds__(256) void chol_dag_kernel(double* A, int ld, int T,
             publish(done + j * T + j);
         } else {
             // C -> operand s[k][r] = C[r][k]
-            acc_for_each(acc, [&](int r, int c, double v) { sA[c * SP + r] = Aij[(size_t)r * ld + c] - v; });
+            int q = 0;
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v, ++q) sA[acc_col(y) * SP + acc_row(x, v)] = cv[q] - acc[x][y][v];
             ok = wait_done(done + j * T + j, &st->err, spin_limit);
             if (!ok) continue;
-            stage_rk(sB, work + (size_t)j * T64 * T64, T64);  // op(B)(kk,c) = W_j[c][kk]
+            stage<true>(sB, work + (size_t)j * T64 * T64, T64);  // op(B)(kk,c) = W_j[c][kk]
             __syncthreads();
             v4d out[2][2];
-            for (int x = 0; x < 2; ++x)
-                for (int y = 0; y < 2; ++y) out[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+            acc_zero(out);
             mma64(sA, sB, out);
             acc_for_each(out, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] = v; });
             publish(done + i * T + j);

@@ -6,18 +6,20 @@
 // ld; only the lower triangle is read.  Work is cut into 64 x 64 tiles; every
 // tile product runs on the FP64 matrix cores (v_mfma_f64_16x16x4_f64): a
 // 256-thread workgroup owns one 64 x 64 output tile, each of its four waves a
-// 32 x 32 quadrant held as 2 x 2 MFMA accumulators.  Operands are staged in
-// LDS k-major (s[k][r]) so each quarter-wave reads 16 consecutive doubles.
+// 32 x 32 quadrant held as 2 x 2 MFMA accumulators (pbx_tile.h).
 //
-// Cholesky, right-looking with look-ahead fused into the update kernel:
+// Cholesky (variants 1 and 2), right-looking with look-ahead fused into the
+// update kernel:
 //   potrf_diag(0)
 //   for k = 0 .. T-2:
 //     trsm(k):   L_ik = A_ik * inv(L_kk)^T           (one workgroup per tile row)
 //     syrk(k):   A_ij -= L_ik L_jk^T  for k < j <= i   (one workgroup per tile)
 //                and the workgroup owning tile (k+1,k+1) factors it in LDS
+// Variant 3 is the single-launch tile graph of pbx_chol_dag.hip.
 // inv(L_kk) is produced with each diagonal factorisation and kept in `work`
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
 #include "pbx_common.h"
+#include "pbx_tile.h"
 
 #define TRY_RC(x)            \
     do {                     \
@@ -25,73 +27,14 @@
         if (rc_) return rc_; \
     } while (0)
 
+using namespace pbx_tile;
+
 namespace {
 
-typedef double v4d __attribute__((ext_vector_type(4)));
-
-constexpr int T64 = 64;
-constexpr int SP = 65;  // LDS row stride in doubles
-
-__device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// Stage a 64 x 64 global tile into LDS as s[k][r] = op(X)(r, k).
-//   rk = true : X is indexed X[r][k] (row r, column k)  -> transposed copy
-//   rk = false: X is indexed X[k][r]                     -> straight copy
-__device__ __forceinline__ void stage(double* s, const double* X, int ld, bool rk) {
-    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
-        int row = e >> 6, col = e & 63;
-        double v = X[(size_t)row * ld + col];
-        if (rk)
-            s[col * SP + row] = v;
-        else
-            s[row * SP + col] = v;
-    }
-}
-
-// acc += op(A) op(B) for one 64-deep slab, operands staged k-major.
-// acc[x][y][v] <-> C[r0 + 16x + lane/16 + 4v][c0 + 16y + lane%16] (the f64 16x16x4
-// accumulator interleaves rows: register v of lane group g holds row g + 4v)
-__device__ __forceinline__ void mma64(const double* sA, const double* sB, v4d acc[2][2]) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 4
-    for (int k = 0; k < T64; k += 4) {
-        const double* pa = sA + (k + lk) * SP;
-        const double* pb = sB + (k + lk) * SP;
-        double a0 = pa[r0 + li], a1 = pa[r0 + 16 + li];
-        double b0 = pb[c0 + li], b1 = pb[c0 + 16 + li];
-        acc[0][0] = mfma(a0, b0, acc[0][0]);
-        acc[0][1] = mfma(a0, b1, acc[0][1]);
-        acc[1][0] = mfma(a1, b0, acc[1][0]);
-        acc[1][1] = mfma(a1, b1, acc[1][1]);
-    }
-}
-
-__device__ __forceinline__ void acc_zero(v4d acc[2][2]) {
-    for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
-}
-
-// Visit every accumulator element with its tile coordinates.
-template <class F>
-__device__ __forceinline__ void acc_for_each(v4d acc[2][2], F f) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    for (int x = 0; x < 2; ++x)
-        for (int y = 0; y < 2; ++y)
-            for (int v = 0; v < 4; ++v) f(r0 + 16 * x + lk + 4 * v, c0 + 16 * y + li, acc[x][y][v]);
-}
-
-// Factor the SPD 64 x 64 tile held in LDS (lower triangle, stride SP) and
-// build the inverse of the factor in sW (lower, zero above).  Column-by-
-// column right-looking elimination; row j of inv(L) is formed in the same
-// step from rows < j.  The diagonal square roots are kept in sd[] until the
-// end so no lane overwrites L[j][j] while others still read it.
-__device__ void tile_potrf_inv(double* sL, double* sW, double* sd, int col0, int* info) {
+// Column-by-column factorisation of the LDS tile with inv(L) formed row by
+// row (two workgroup barriers per column).  Kept as variant 1: the simplest
+// statement of the tile step, used to cross-check the panel version.
+__device__ void tile_factor_columns(double* sL, double* sW, double* sd, int col0, int* info) {
     const int tid = threadIdx.x;
     for (int e = tid; e < T64 * SP; e += blockDim.x) sW[e] = 0.0;
     __syncthreads();
@@ -106,7 +49,6 @@ __device__ void tile_potrf_inv(double* sL, double* sW, double* sd, int col0, int
                 if (!(djj > 0.0)) atomicCAS(info, 0, col0 + j + 1);
             }
         } else if (tid < 128) {
-            // W[j][c] for c <= j: (delta_jc - sum_{c<=k<j} L[j][k] W[k][c]) / L[j][j]
             int c = tid - 64;
             if (c <= j) {
                 double s0 = (c == j) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -122,7 +64,6 @@ __device__ void tile_potrf_inv(double* sL, double* sW, double* sd, int col0, int
             }
         }
         __syncthreads();
-        // trailing update: A[i][c] -= L[i][j] * L[c][j] for j < c <= i
         for (int e = tid; e < T64 * T64; e += blockDim.x) {
             int i = e >> 6, c = e & 63;
             if (c > j && c <= i) sL[i * SP + c] -= sL[i * SP + j] * sL[c * SP + j];
@@ -133,125 +74,12 @@ __device__ void tile_potrf_inv(double* sL, double* sW, double* sd, int col0, int
     __syncthreads();
 }
 
-// Broadcast a double from a uniform lane (v_readlane on both halves).
-__device__ __forceinline__ double bcast(double x, int lane) {
-    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
-    return __hiloint2double(hi, lo);
-}
-
-// 1/sqrt(d): hardware estimate refined by two Newton steps (FMA), then
-// sqrt(d) = d * rsqrt(d).  Within an ulp or two of the correctly rounded
-// values; non-positive or NaN d propagates NaN.
-__device__ __forceinline__ double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    double h = 0.5 * d;
-    double r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    return y;
-}
-
-// Same contract as tile_potrf_inv with far fewer workgroup barriers:
-// 8-column panels are factored by wave 0 alone (lane = row, panel in
-// registers, pivots broadcast with readlane); the trailing update
-// A22 -= P P^T (K = 8) runs on the FP64 matrix cores across all four
-// waves; rows of inv(L) for the panel are formed by waves 1-3 once the
-// panel rows are final.
-__device__ void tile_potrf_inv_v2(double* sL, double* sW, double* sd, int col0, int* info) {
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int e = tid; e < T64 * SP; e += blockDim.x) sW[e] = 0.0;
-    for (int kb = 0; kb < 8; ++kb) {
-        const int c0 = kb * 8;
-        if (w == 0) {
-            const bool act = lane >= c0;
-            double v[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = act ? sL[lane * SP + c0 + q] : 0.0;
-            int first_bad = -1;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int j = c0 + q;
-                const double d = bcast(v[q], j);
-                if (!(d > 0.0) && first_bad < 0) first_bad = q;
-                const double y = rsqrt_nr(d);
-                if (lane == j) {
-                    v[q] = d * y;
-                    sd[j] = d * y;
-                } else if (lane > j) {
-                    v[q] = v[q] * y;
-                }
-#pragma unroll
-                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] -= v[q] * bcast(v[q], c0 + q2);
-            }
-            if (act) {
-#pragma unroll
-                for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
-            }
-            if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
-        }
-        __syncthreads();
-        // inv(L) rows c0..c0+7 (waves 1-3): W[r][c] for c <= r, sequential in r
-        // within the panel; uses L rows r (final) and W rows < r.
-        if (w > 0) {
-            const int t = tid - 64;  // 192 threads: column c = t % 64, split = t / 64
-            const int c = t & 63, part = t >> 6;
-            for (int r = c0; r < c0 + 8; ++r) {
-                double s = 0.0;
-                if (c <= r) {
-                    for (int k = c + part; k < r; k += 3) s -= sL[r * SP + k] * sW[k * SP + c];
-                }
-                // reduce the three partial sums through LDS scratch in sW's pad column
-                __shared__ double red[3][64];
-                red[part][c] = s;
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-                if (part == 0 && c <= r) {
-                    double tot = red[0][c] + red[1][c] + red[2][c] + (c == r ? 1.0 : 0.0);
-                    sW[r * SP + c] = tot / sd[r];
-                }
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            }
-        } else {
-            // wave 0 meets the waves-1..3 barriers of the inverse rows
-            for (int r = 0; r < 16; ++r) {
-                __builtin_amdgcn_s_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-            }
-        }
-        // trailing update of rows/cols >= c0+8 on the matrix cores
-        if (kb < 7) {
-            const int r0 = (w >> 1) * 32, cq = (w & 1) * 32;
-            const int li = lane & 15, lk = lane >> 4;
-            const int lo = c0 + 8;
-            const bool live = (r0 + 31 >= lo) && !(cq > r0 + 31);
-            v4d acc[2][2];
-            acc_zero(acc);
-            if (live) {
-#pragma unroll
-                for (int ks = 0; ks < 8; ks += 4) {
-                    const int k = c0 + ks + lk;
-                    double a0 = sL[(r0 + li) * SP + k], a1 = sL[(r0 + 16 + li) * SP + k];
-                    double b0 = sL[(cq + li) * SP + k], b1 = sL[(cq + 16 + li) * SP + k];
-                    acc[0][0] = mfma(a0, b0, acc[0][0]);
-                    acc[0][1] = mfma(a0, b1, acc[0][1]);
-                    acc[1][0] = mfma(a1, b0, acc[1][0]);
-                    acc[1][1] = mfma(a1, b1, acc[1][1]);
-                }
-            }
-            __syncthreads();
-            if (live) {
-                acc_for_each(acc, [&](int r, int c, double v) {
-                    if (r >= lo && c >= lo && c <= r) sL[r * SP + c] -= v;
-                });
-            }
-            __syncthreads();
-        }
-    }
-    if (tid < 64) sL[tid * SP + tid] = sd[tid];
-    __syncthreads();
+template <int V>
+__device__ __forceinline__ void factor_tile(double* sL, double* sW, double* sd, int col0, int* info) {
+    if (V == 2)
+        tile_factor(sL, sW, sd, col0, info);
+    else
+        tile_factor_columns(sL, sW, sd, col0, info);
 }
 
 // Write the factored diagonal tile (lower incl. diagonal) and its inverse.
@@ -266,23 +94,11 @@ __device__ void store_diag(const double* sL, const double* sW, double* A, int ld
 }
 
 template <int V>
-__device__ __forceinline__ void factor_tile(double* sL, double* sW, double* sd, int col0, int* info) {
-    if (V == 2)
-        tile_potrf_inv_v2(sL, sW, sd, col0, info);
-    else
-        tile_potrf_inv(sL, sW, sd, col0, info);
-}
-
-template <int V>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info) {
     __shared__ double sL[T64 * SP];
     __shared__ double sW[T64 * SP];
     __shared__ double sd[T64];
-    const double* Akk = A + (size_t)k * T64 * ld + (size_t)k * T64;
-    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
-        int r = e >> 6, c = e & 63;
-        sL[r * SP + c] = Akk[(size_t)r * ld + c];
-    }
+    stage<false>(sL, A + (size_t)k * T64 * ld + (size_t)k * T64, ld);
     __syncthreads();
     factor_tile<V>(sL, sW, sd, k * T64, info);
     store_diag(sL, sW, A, ld, k, work);
@@ -294,8 +110,8 @@ __global__ __launch_bounds__(256) void trsm_kernel(double* A, int ld, int k, con
     __shared__ double sB[T64 * SP];
     const int i = k + 1 + blockIdx.x;
     double* Aik = A + (size_t)i * T64 * ld + (size_t)k * T64;
-    stage(sA, Aik, ld, true);                                  // op(A) = A_ik
-    stage(sB, work + (size_t)k * T64 * T64, T64, true);        // op(B)(kk,c) = W[c][kk]
+    stage<true>(sA, Aik, ld);                             // op(A) = A_ik
+    stage<true>(sB, work + (size_t)k * T64 * T64, T64);   // op(B)(kk,c) = W[c][kk]
     __syncthreads();
     v4d acc[2][2];
     acc_zero(acc);
@@ -314,37 +130,46 @@ __device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
 // A_ij -= L_ik L_jk^T over the trailing lower tiles; the workgroup owning
 // (k+1, k+1) (blockIdx 0) then factors that tile.
 template <int V>
-__global__ __launch_bounds__(256) void syrk_kernel(double* A, int ld, int k, double* work, int* info) {
+__global__ __launch_bounds__(256, 2) void syrk_kernel(double* A, int ld, int k, double* work, int* info) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
     int ii, jj;
     tri_decode(blockIdx.x, ii, jj);
     const int i = k + 1 + ii, j = k + 1 + jj;
-    const double* Aik = A + (size_t)i * T64 * ld + (size_t)k * T64;
-    const double* Ajk = A + (size_t)j * T64 * ld + (size_t)k * T64;
     double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
-    stage(sA, Aik, ld, true);
-    if (i != j) stage(sB, Ajk, ld, true);
+    double cv[16];
+    acc_load(Aij, ld, cv);  // issued early: overlaps the staging below
+    stage<true>(sA, A + (size_t)i * T64 * ld + (size_t)k * T64, ld);
+    if (i != j) stage<true>(sB, A + (size_t)j * T64 * ld + (size_t)k * T64, ld);
     __syncthreads();
     v4d acc[2][2];
     acc_zero(acc);
     mma64(sA, i == j ? sA : sB, acc);
     if (blockIdx.x != 0) {
-        if (i == j) {
-            acc_for_each(acc, [&](int r, int c, double v) {
-                if (c <= r) Aij[(size_t)r * ld + c] -= v;
-            });
-        } else {
-            acc_for_each(acc, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] -= v; });
-        }
+        int q = 0;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v, ++q) {
+                    const int r = acc_row(x, v), c = acc_col(y);
+                    if (i != j || c <= r) Aij[(size_t)r * ld + c] = cv[q] - acc[x][y][v];
+                }
         return;
     }
     // look-ahead: update, factor and invert tile (k+1, k+1) in LDS
     __syncthreads();  // all waves done reading sA
     double* sL = sA;
     double* sW = sB;
-    acc_for_each(acc, [&](int r, int c, double v) { sL[r * SP + c] = Aij[(size_t)r * ld + c] - v; });
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) sL[acc_row(x, v) * SP + acc_col(y)] = cv[q] - acc[x][y][v];
     __syncthreads();
     factor_tile<V>(sL, sW, sd, (k + 1) * T64, info);
     store_diag(sL, sW, A, ld, k + 1, work);
@@ -373,15 +198,15 @@ __global__ __launch_bounds__(256) void trtri_row_kernel(const double* A, int ld,
     v4d acc[2][2];
     acc_zero(acc);
     for (int m = j; m < i; ++m) {
-        stage(sA, A + (size_t)i * T64 * ld + (size_t)m * T64, ld, true);    // op(A) = L_im
-        stage(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx, false);  // op(B) = X_mj
+        stage<true>(sA, A + (size_t)i * T64 * ld + (size_t)m * T64, ld);     // op(A) = L_im
+        stage<false>(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);  // op(B) = X_mj
         __syncthreads();
         mma64(sA, sB, acc);
         __syncthreads();
     }
     // X_ij = -W_i * S
     acc_for_each(acc, [&](int r, int c, double v) { sB[r * SP + c] = v; });  // s[k=r][c] = S[r][c]
-    stage(sA, work + (size_t)i * T64 * T64, T64, true);                        // op(A) = W_i
+    stage<true>(sA, work + (size_t)i * T64 * T64, T64);                        // op(A) = W_i
     __syncthreads();
     v4d out[2][2];
     acc_zero(out);
@@ -401,8 +226,8 @@ __global__ __launch_bounds__(256) void lauum_kernel(const double* X, int ldx, do
     v4d acc[2][2];
     acc_zero(acc);
     for (int m = i; m < T; ++m) {
-        stage(sA, X + (size_t)m * T64 * ldx + (size_t)i * T64, ldx, false);  // op(A)(r,k) = X_mi[k][r]
-        stage(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx, false);  // op(B)(k,c) = X_mj[k][c]
+        stage<false>(sA, X + (size_t)m * T64 * ldx + (size_t)i * T64, ldx);  // op(A)(r,k) = X_mi[k][r]
+        stage<false>(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);  // op(B)(k,c) = X_mj[k][c]
         __syncthreads();
         mma64(sA, sB, acc);
         __syncthreads();
@@ -411,8 +236,10 @@ __global__ __launch_bounds__(256) void lauum_kernel(const double* X, int ldx, do
     double* Yji = Y + (size_t)j * T64 * ld + (size_t)i * T64;
     acc_for_each(acc, [&](int r, int c, double v) {
         double s = v * scale;
-        Yij[(size_t)r * ld + c] = s;
-        Yji[(size_t)c * ld + r] = s;
+        if (i != j || c <= r) {
+            Yij[(size_t)r * ld + c] = s;
+            Yji[(size_t)c * ld + r] = s;
+        }
     });
 }
 
@@ -435,9 +262,16 @@ __global__ void copy_lower_kernel(const double* src, double* dst, int np, int ld
     tri_decode(blockIdx.x, ti, tj);
     const double* s = src + (size_t)ti * T64 * ld + (size_t)tj * T64;
     double* d = dst + (size_t)ti * T64 * ld + (size_t)tj * T64;
-    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
-        int r = e >> 6, c = e & 63;
-        d[(size_t)r * ld + c] = s[(size_t)r * ld + c];
+    v2d v[8];
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = threadIdx.x + it * 256;
+        v[it] = *(const v2d*)(s + (size_t)(e >> 5) * ld + (e & 31) * 2);
+    }
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = threadIdx.x + it * 256;
+        *(v2d*)(d + (size_t)(e >> 5) * ld + (e & 31) * 2) = v[it];
     }
 }
 
@@ -451,8 +285,8 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const dou
     v4d acc[2][2];
     acc_zero(acc);
     for (int k = 0; k < K; k += T64) {
-        stage(sA, A + (size_t)ti * T64 * K + k, K, true);
-        stage(sB, B + (size_t)tj * T64 * K + k, K, true);
+        stage<true>(sA, A + (size_t)ti * T64 * K + k, K);
+        stage<true>(sB, B + (size_t)tj * T64 * K + k, K);
         __syncthreads();
         mma64(sA, sB, acc);
         __syncthreads();
@@ -495,8 +329,9 @@ int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s
 }
 
 // In-place lower Cholesky of the np x np matrix A; inverses of the diagonal
-// factor tiles are left in work (np*64 doubles).  *info_dev must be zero on
-// entry; it receives column+1 of the first non-positive pivot.
+// factor tiles are left in work (pbx_dense_work_doubles(np) doubles).
+// *info_dev must be zero on entry; it receives column+1 of the first
+// non-positive pivot.
 int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s) {
     if (np % T64 != 0 || np <= 0) {
         pbx_set_error(PBX_ERR_ARG, "potrf: order %d is not a positive multiple of 64", np);

@@ -123,7 +123,11 @@ __device__ void recip_table_fill(RecipCoef* tab, int n_max, double alpha) {
 }
 
 // Accumulate the Ewald lattice sums for separation sigma (box units).
-// self != 0: sigma = 0, m = 0 image excluded.
+// self: sigma = 0 and the m = 0 image is excluded.  MM / NM >= 0 fix the
+// truncation at compile time (loops unroll; the reciprocal cosines come from
+// three sincospi calls and angle-addition recurrences); -1 reads m_max /
+// n_max at run time and evaluates each cosine directly.
+template <int MM, int NM>
 __device__ void smith_sums(double sx, double sy, double sz, bool self, double alpha, int m_max, int n_max,
                            double wO, double wQ, const RecipCoef* tab, double& sI, double T[6]) {
     const double SQRT_PI = 1.7724538509055160273;
@@ -131,10 +135,15 @@ __device__ void smith_sums(double sx, double sy, double sz, bool self, double al
     const double k1 = 2.0 / SQRT_PI;
     const double k2 = 4.0 * alpha * alpha * alpha / SQRT_PI;
     const double a2 = alpha * alpha;
-    for (int mx = -m_max; mx <= m_max; ++mx) {
-        int rx = m_max - abs(mx);
+    const int mm = MM >= 0 ? MM : m_max;
+    const int nm = NM >= 0 ? NM : n_max;
+#pragma unroll
+    for (int mx = -mm; mx <= mm; ++mx) {
+        const int rx = mm - (mx < 0 ? -mx : mx);
+#pragma unroll
         for (int my = -rx; my <= rx; ++my) {
-            int ry = rx - abs(my);
+            const int ry = rx - (my < 0 ? -my : my);
+#pragma unroll
             for (int mz = -ry; mz <= ry; ++mz) {
                 if (self && mx == 0 && my == 0 && mz == 0) continue;
                 double x = mx + sx, y = my + sy, z = mz + sz;
@@ -158,29 +167,84 @@ __device__ void smith_sums(double sx, double sy, double sz, bool self, double al
             }
         }
     }
-    for (int nx = 0; nx <= n_max; ++nx) {
-        int rx = n_max - nx;
-        for (int ny = (nx == 0 ? 0 : -rx); ny <= rx; ++ny) {
-            int ry = rx - abs(ny);
-            int z0 = (nx == 0 && ny == 0) ? 1 : -ry;
-            for (int nz = z0; nz <= ry; ++nz) {
-                int q = nx * nx + ny * ny + nz * nz;
-                RecipCoef c = tab[q];
-                double cs = self ? 1.0 : cospi(2.0 * (nx * sx + ny * sy + nz * sz));
-                sI += 2.0 * wO * c.A * cs;
-                double cT = 2.0 * cs * (wQ * c.C - wO * c.AB);
-                T[0] += cT * (double)(nx * nx);
-                T[1] += cT * (double)(ny * ny);
-                T[2] += cT * (double)(nz * nz);
-                T[3] += cT * (double)(nx * ny);
-                T[4] += cT * (double)(nx * nz);
-                T[5] += cT * (double)(ny * nz);
+    // reciprocal sum over half of the (n, -n) pairs, doubled (cos is even)
+    if constexpr (NM >= 0) {
+        double cx[NM + 1], sxn[NM + 1], cy[NM + 1], syn[NM + 1], cz[NM + 1], szn[NM + 1];
+        cx[0] = cy[0] = cz[0] = 1.0;
+        sxn[0] = syn[0] = szn[0] = 0.0;
+        if (NM >= 1) {
+            if (self) {
+                cx[1] = cy[1] = cz[1] = 1.0;
+                sxn[1] = syn[1] = szn[1] = 0.0;
+            } else {
+                sincospi(2.0 * sx, &sxn[1], &cx[1]);
+                sincospi(2.0 * sy, &syn[1], &cy[1]);
+                sincospi(2.0 * sz, &szn[1], &cz[1]);
+            }
+        }
+#pragma unroll
+        for (int k = 2; k <= NM; ++k) {
+            cx[k] = cx[k - 1] * cx[1] - sxn[k - 1] * sxn[1];
+            sxn[k] = sxn[k - 1] * cx[1] + cx[k - 1] * sxn[1];
+            cy[k] = cy[k - 1] * cy[1] - syn[k - 1] * syn[1];
+            syn[k] = syn[k - 1] * cy[1] + cy[k - 1] * syn[1];
+            cz[k] = cz[k - 1] * cz[1] - szn[k - 1] * szn[1];
+            szn[k] = szn[k - 1] * cz[1] + cz[k - 1] * szn[1];
+        }
+#pragma unroll
+        for (int nx = 0; nx <= NM; ++nx) {
+            const int rx = NM - nx;
+#pragma unroll
+            for (int ny = (nx == 0 ? 0 : -rx); ny <= rx; ++ny) {
+                const int ay = ny < 0 ? -ny : ny;
+                const int ry = rx - ay;
+                // e^{i(nx tx + ny ty)}
+                const double ysn = ny < 0 ? -syn[ay] : syn[ay];
+                const double pr = cx[nx] * cy[ay] - sxn[nx] * ysn;
+                const double pi = sxn[nx] * cy[ay] + cx[nx] * ysn;
+#pragma unroll
+                for (int nz = ((nx == 0 && ny == 0) ? 1 : -ry); nz <= ry; ++nz) {
+                    const int az = nz < 0 ? -nz : nz;
+                    const double zsn = nz < 0 ? -szn[az] : szn[az];
+                    const double cs = pr * cz[az] - pi * zsn;
+                    const RecipCoef c = tab[nx * nx + ny * ny + nz * nz];
+                    sI += 2.0 * wO * c.A * cs;
+                    const double cT = 2.0 * cs * (wQ * c.C - wO * c.AB);
+                    T[0] += cT * (double)(nx * nx);
+                    T[1] += cT * (double)(ny * ny);
+                    T[2] += cT * (double)(nz * nz);
+                    T[3] += cT * (double)(nx * ny);
+                    T[4] += cT * (double)(nx * nz);
+                    T[5] += cT * (double)(ny * nz);
+                }
+            }
+        }
+    } else {
+        for (int nx = 0; nx <= nm; ++nx) {
+            int rx = nm - nx;
+            for (int ny = (nx == 0 ? 0 : -rx); ny <= rx; ++ny) {
+                int ry = rx - abs(ny);
+                int z0 = (nx == 0 && ny == 0) ? 1 : -ry;
+                for (int nz = z0; nz <= ry; ++nz) {
+                    int q = nx * nx + ny * ny + nz * nz;
+                    RecipCoef c = tab[q];
+                    double cs = self ? 1.0 : cospi(2.0 * (nx * sx + ny * sy + nz * sz));
+                    sI += 2.0 * wO * c.A * cs;
+                    double cT = 2.0 * cs * (wQ * c.C - wO * c.AB);
+                    T[0] += cT * (double)(nx * nx);
+                    T[1] += cT * (double)(ny * ny);
+                    T[2] += cT * (double)(nz * nz);
+                    T[3] += cT * (double)(nx * ny);
+                    T[4] += cT * (double)(nx * nz);
+                    T[5] += cT * (double)(ny * nz);
+                }
             }
         }
     }
 }
 
 // diff_tensor.c:553-575 with the diagonal-only copy of :194-199.
+template <int MM, int NM>
 __device__ void smith_self_block(double a, double L, double alpha, int m_max, int n_max, const RecipCoef* tab,
                                  double m[6]) {
     const double PI = 3.14159265358979323846;
@@ -189,7 +253,7 @@ __device__ void smith_self_block(double a, double L, double alpha, int m_max, in
     double c2 = 3 * a / (4 * L);
     double c3 = a * a * a / (2 * L * L * L);
     double sI = 0.0, T[6] = {0, 0, 0, 0, 0, 0};
-    smith_sums(0.0, 0.0, 0.0, true, alpha, m_max, n_max, c1 * c2, c1 * c3, tab, sI, T);
+    smith_sums<MM, NM>(0.0, 0.0, 0.0, true, alpha, m_max, n_max, c1 * c2, c1 * c3, tab, sI, T);
     double corrO = 1.5 * alpha * a / (SQRT_PI * L);
     double corrQ = alpha * alpha * alpha * a * a * a / (L * L * L * 3.0 * SQRT_PI);
     double diag = c1 * (1.0 - c2 * corrO - c3 * corrQ) + sI;
@@ -200,6 +264,7 @@ __device__ void smith_self_block(double a, double L, double alpha, int m_max, in
 }
 
 // diff_tensor.c:579-648 RPY_Smith_Mij_block
+template <int MM, int NM>
 __device__ void smith_pair_block(double ai, double aj, double rx, double ry, double rz, double L, double alpha,
                                  int m_max, int n_max, const RecipCoef* tab, double m[6]) {
     const double PI = 3.14159265358979323846;
@@ -207,7 +272,7 @@ __device__ void smith_pair_block(double ai, double aj, double rx, double ry, dou
     double c2 = 3 * ai / (4 * L);
     double c3 = (ai == aj) ? ai * ai * ai / (2 * L * L * L) : ai * (ai * ai + aj * aj) / (4 * L * L * L);
     double sI = 0.0, T[6] = {0, 0, 0, 0, 0, 0};
-    smith_sums(rx / L, ry / L, rz / L, false, alpha, m_max, n_max, c1 * c2, c1 * c3, tab, sI, T);
+    smith_sums<MM, NM>(rx / L, ry / L, rz / L, false, alpha, m_max, n_max, c1 * c2, c1 * c3, tab, sI, T);
     m[0] = sI + T[0];
     m[1] = sI + T[1];
     m[2] = sI + T[2];
@@ -254,7 +319,7 @@ struct GeomPacked {
 
 enum Kind { K_RPY = 0, K_SMITH = 1 };
 
-template <int KIND, class Geom>
+template <int KIND, class Geom, int MM = -1, int NM = -1>
 __global__ __launch_bounds__(256) void assemble_tile_kernel(Geom g, const double* __restrict__ a, double L,
                                                             double alpha, int m_max, int n_max, double scale,
                                                             double* __restrict__ M, int ld) {
@@ -276,7 +341,7 @@ __global__ __launch_bounds__(256) void assemble_tile_kernel(Geom g, const double
                 double d = 1.0 / (6 * 3.14159265358979323846 * a[i]);
                 b[0] = b[1] = b[2] = d;
             } else {
-                smith_self_block(a[i], L, alpha, m_max, n_max, tab, b);
+                smith_self_block<MM, NM>(a[i], L, alpha, m_max, n_max, tab, b);
             }
         } else {
             double x, y, z;
@@ -284,7 +349,7 @@ __global__ __launch_bounds__(256) void assemble_tile_kernel(Geom g, const double
             if (KIND == K_RPY)
                 rpy_pair(a[i], a[j], x, y, z, b);
             else
-                smith_pair_block(a[i], a[j], x, y, z, L, alpha, m_max, n_max, tab, b);
+                smith_pair_block<MM, NM>(a[i], a[j], x, y, z, L, alpha, m_max, n_max, tab, b);
         }
         for (int k = 0; k < 6; ++k) b[k] = b[k] * scale;
     }
@@ -355,6 +420,27 @@ __global__ __launch_bounds__(256) void overlaps_kernel(const double* __restrict_
     if (__any(hit) && (threadIdx.x % 64) == 0) atomicOr(flag, 1);
 }
 
+// Launch the Smith kernel with compile-time truncation for m_max, n_max <= 3.
+template <class Geom>
+void launch_smith(Geom g, const double* a, double L, double alpha, int m_max, int n_max, double scale, double* M,
+                  int ld, int tiles, hipStream_t s) {
+#define PBX_SMITH_CASE(MMV, NMV)                                                                              \
+    if (m_max == MMV && n_max == NMV) {                                                                     \
+        hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, Geom, MMV, NMV>), dim3(tiles), dim3(256), 0, s, g, a, \
+                           L, alpha, m_max, n_max, scale, M, ld);                                           \
+        return;                                                                                              \
+    }
+#define PBX_SMITH_ROW(MMV) PBX_SMITH_CASE(MMV, 0) PBX_SMITH_CASE(MMV, 1) PBX_SMITH_CASE(MMV, 2) PBX_SMITH_CASE(MMV, 3)
+    PBX_SMITH_ROW(0)
+    PBX_SMITH_ROW(1)
+    PBX_SMITH_ROW(2)
+    PBX_SMITH_ROW(3)
+#undef PBX_SMITH_ROW
+#undef PBX_SMITH_CASE
+    hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, Geom, -1, -1>), dim3(tiles), dim3(256), 0, s, g, a, L, alpha,
+                       m_max, n_max, scale, M, ld);
+}
+
 inline int tiles_for(int N) {
     int T = (N + TB - 1) / TB;
     return T * (T + 1) / 2;
@@ -385,8 +471,7 @@ int pbx_launch_smith(const double* xyz, const double* a, int N, double L, double
         return PBX_ERR_ARG;
     }
     GeomXYZ g{xyz, L, N};
-    hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, GeomXYZ>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, L,
-                       alpha, m_max, n_max, scale, M, ld);
+    launch_smith(g, a, L, alpha, m_max, n_max, scale, M, ld, tiles_for(N), s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -409,8 +494,7 @@ int pbx_launch_smith_packed(const double* a, const double* P, int N, double L, d
         return PBX_ERR_ARG;
     }
     GeomPacked g{P, N};
-    hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, GeomPacked>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, L,
-                       alpha, m_max, n_max, 1.0, M, ld);
+    launch_smith(g, a, L, alpha, m_max, n_max, 1.0, M, ld, tiles_for(N), s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
