@@ -125,6 +125,12 @@ int pbx_sim_set_noise_source(pbx_sim* sim, pbx_xi_fill_fn fn, void* user);
  * argument/HIP errors. */
 int pbx_sim_propagate(pbx_sim* sim, double dt, int build_Dff, int build_Dnf, int cholesky);
 
+/* Box.propagate followed by the read-back the driver does after every step
+ * (BD.py:168-175): positions (N x 3) and per-label flux counters, copied in
+ * the step's own final synchronisation.  Either output may be NULL. */
+int pbx_sim_propagate_out(pbx_sim* sim, double dt, int build_Dff, int build_Dnf, int cholesky, double* xyz_out,
+                          long long* flux_out);
+
 int pbx_sim_get_positions(pbx_sim* sim, double* xyz);
 int pbx_sim_set_positions(pbx_sim* sim, const double* xyz);
 int pbx_sim_get_flux(pbx_sim* sim, long long* per_label);

@@ -71,6 +71,7 @@ _sig("pbx_sim_create", _i, [ctypes.POINTER(Config), _i, _dp, _dp, _ip, ctypes.PO
 _sig("pbx_sim_destroy", _i, [ctypes.c_void_p])
 _sig("pbx_sim_set_noise_source", _i, [ctypes.c_void_p, XI_FILL, ctypes.c_void_p])
 _sig("pbx_sim_propagate", _i, [ctypes.c_void_p, _d, _i, _i, _i])
+_sig("pbx_sim_propagate_out", _i, [ctypes.c_void_p, _d, _i, _i, _i, _dp, _lp])
 _sig("pbx_sim_get_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_set_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_get_flux", _i, [ctypes.c_void_p, _lp])
@@ -92,6 +93,7 @@ EXPORTED = [
     "pointer_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_spd_inverse_host", "pbx_gemm_nt",
     "pbx_set_potrf_variant",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_propagate",
+    "pbx_sim_propagate_out",
     "pbx_sim_get_positions", "pbx_sim_set_positions", "pbx_sim_get_flux", "pbx_sim_get_matrix",
     "pbx_sim_get_diag", "pbx_sim_draw_count", "pbx_sim_last_attempts", "pbx_sim_sync",
     "pbx_sim_set_timing", "pbx_sim_get_timing",

@@ -17,7 +17,7 @@ class Bead:
 
     def __init__(self, coords, hydrodynamic_radius, label="XXX", hard_core_radius=None, epsilon_LJ=0.0,
                  mobile=True, bead_id=None):
-        self.r = np.array(coords, dtype=np.float64)
+        self._r = np.array(coords, dtype=np.float64)
         self.a = hydrodynamic_radius
         self.label = label
         self.mobile = mobile
@@ -28,6 +28,23 @@ class Bead:
         self.bonded_how = {}
         self.angled_with = []
         self.angled_how = {}
+
+    @property
+    def r(self):
+        """Position (3,).  Inside a Box this is a view of the box's position
+        array; assigning a new array copies it into that view."""
+        return self._r
+
+    @r.setter
+    def r(self, value):
+        if getattr(self, "_bound", False):
+            self._r[:] = value
+        else:
+            self._r = np.array(value, dtype=np.float64)
+
+    def _bind(self, view):
+        self._r = view
+        self._bound = True
 
     def translate(self, vector):
         """bead.py:92-99"""

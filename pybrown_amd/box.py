@@ -154,11 +154,17 @@ class Box:
         self._fill_cb = _lib.XI_FILL(self._fill)
         lib.pbx_sim_set_noise_source(self._sim, self._fill_cb, None)
         self._synced = self._pos.copy()
+        self._counts = np.zeros(max(1, len(self.mobile_labels)), dtype=np.int64)
+        self._pos_ptr = ptr(self._pos.reshape(-1))
+        self._flux_ptr = self._counts.ctypes.data_as(_lib._lp)
 
     # -- plumbing ---------------------------------------------------------
     def _bind_views(self):
         for i, b in enumerate(self.beads):
-            b.r = self._pos[i]
+            if hasattr(b, "_bind"):
+                b._bind(self._pos[i])
+            else:
+                b.r = self._pos[i]
 
     def _fill(self, user, out, count):
         try:
@@ -169,27 +175,21 @@ class Box:
             return 1
 
     def _push_host_positions(self):
-        """Upload positions if the caller moved beads since the last step."""
-        rebound = any(b.r.base is not self._pos for b in self.beads)
-        if rebound:
-            self._pos[:] = np.array([b.r for b in self.beads], dtype=np.float64).reshape(self.N, 3)
-            self._bind_views()
-        if rebound or not np.array_equal(self._pos, self._synced):
+        """Upload positions if the caller moved beads since the last step
+        (bead positions are views of self._pos; see Bead.r)."""
+        if not np.array_equal(self._pos, self._synced):
             rc = lib.pbx_sim_set_positions(self._sim, ptr(self._pos.reshape(-1)))
             _lib.check_rc(rc, "set_positions")
 
     # -- box.py:150-198 ---------------------------------------------------
     def propagate(self, dt, build_Dff=True, build_Dnf=True, cholesky=True):
         self._push_host_positions()
-        rc = lib.pbx_sim_propagate(self._sim, float(dt), int(bool(build_Dff)), int(bool(build_Dnf)),
-                                   int(bool(cholesky)))
+        rc = lib.pbx_sim_propagate_out(self._sim, float(dt), int(bool(build_Dff)), int(bool(build_Dnf)),
+                                       int(bool(cholesky)), self._pos_ptr, self._flux_ptr)
         _lib.check_rc(rc, "propagate")
-        lib.pbx_sim_get_positions(self._sim, ptr(self._pos.reshape(-1)))
         self._synced[:] = self._pos
         if self.is_flux:
-            counts = np.zeros(len(self.mobile_labels), dtype=np.int64)
-            lib.pbx_sim_get_flux(self._sim, counts.ctypes.data_as(_lib._lp))
-            self.net_flux = {lab: int(counts[k]) for k, lab in enumerate(self.mobile_labels)}
+            self.net_flux = {lab: int(self._counts[k]) for k, lab in enumerate(self.mobile_labels)}
 
     # -- state queries ----------------------------------------------------
     @property

@@ -311,7 +311,10 @@ struct PbxSim {
     pbx_xi_fill_fn xi_fill = nullptr;
     void* xi_user = nullptr;
     long long draw_count = 0;
-    int* flag_host = nullptr;  // pinned
+    int* flag_host = nullptr;  // pinned: [0] overlap flag, [1] factorisation status
+    double* pos_host = nullptr;      // pinned copy of positions (end of step)
+    long long* flux_host = nullptr;  // pinned copy of flux counters
+    int info_pending = 0;            // a status copy is queued into flag_host[1]
     int last_attempts = 0;
     int have_D = 0, have_B = 0, have_Rff = 0;
     std::vector<void*> owned;
@@ -352,6 +355,30 @@ int assemble_M(PbxSim* s, double* M, double scale) {
         return pbx_launch_smith(s->pos, s->a, s->N, c.box_length, c.ewald_alpha, c.ewald_real, c.ewald_imag, scale,
                                 M, s->ld, 0, s->stream);
     return pbx_launch_rpy(s->pos, s->a, s->N, c.box_length, scale, M, s->ld, 0, s->stream);
+}
+
+// Queue the factorisation status word for the next synchronisation (no
+// stall: the step keeps going and the word is examined once the stream has
+// been synchronised for another reason).
+int post_info(PbxSim* s) {
+    PBX_HIP(hipMemcpyAsync(s->flag_host + 1, s->info, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    s->info_pending = 1;
+    return PBX_OK;
+}
+
+// After a stream synchronisation: report a queued non-zero status.
+int take_info(PbxSim* s) {
+    if (!s->info_pending) return PBX_OK;
+    s->info_pending = 0;
+    int info = s->flag_host[1];
+    if (!info) return PBX_OK;
+    PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
+    if (info == PBX_INFO_DAG_WAIT) {
+        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        return PBX_ERR_STATE;
+    }
+    pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
+    return info;
 }
 
 int check_info(PbxSim* s) {
@@ -432,7 +459,7 @@ int overlap_now(PbxSim* s, int* hit) {
     PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     PBX_HIP(hipStreamSynchronize(s->stream));
     *hit = s->flag_host[0];
-    return PBX_OK;
+    return take_info(s);
 }
 
 struct StepCtx {
@@ -514,7 +541,7 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
             if (is_lub(s) && bDnf) TRY(build_Dtot(s, bDff ? s->RffMid : s->Rff, s->Dmid));
             if (is_lub(s) || bDff) Dm = s->Dmid;
             if (is_lub(s) && bDff && !bDnf) Dm = s->D;  // self.D untouched by Dff in lub modes
-            if (is_lub(s)) TRY(check_info(s));
+            if (is_lub(s)) TRY(post_info(s));
         }
         // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi
         PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
@@ -590,7 +617,9 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     if ((rc = sim_alloc(s, (void**)&s->info, sizeof(int)))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * s->xi_cap * n3))) return fail(rc);
     if (hipHostMalloc((void**)&s->xi_host, sizeof(double) * s->xi_cap * n3, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&s->flag_host, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&s->flag_host, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->pos_host, sizeof(double) * n3, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&s->flux_host, sizeof(long long) * nl, hipHostMallocDefault) != hipSuccess) {
         pbx_set_error(PBX_ERR_OOM, "hipHostMalloc failed");
         return fail(PBX_ERR_OOM);
     }
@@ -671,6 +700,8 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     for (void* p : s->owned) (void)hipFree(p);
     if (s->xi_host) (void)hipHostFree(s->xi_host);
     if (s->flag_host) (void)hipHostFree(s->flag_host);
+    if (s->pos_host) (void)hipHostFree(s->pos_host);
+    if (s->flux_host) (void)hipHostFree(s->flux_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return PBX_OK;
@@ -724,8 +755,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
             pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
             return PBX_ERR_STATE;
         }
-        int info = check_info(s);
-        if (info) return info;
+        TRY(post_info(s));
     }
     if (cf.scheme == PBX_ERMAK)
         TRY(ermak(s, dt, c));
@@ -744,7 +774,27 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         s->stage_ms[2] += t23;
         s->timed_steps += 1;
     }
+    PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    if (cf.measure_flux) {
+        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
+        PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
+    }
     PBX_HIP(hipStreamSynchronize(s->stream));
+    return take_info(s);
+}
+
+// Box.propagate plus the read-back BD.py needs after every step, in the
+// step's final synchronisation: positions (N x 3) and, when flux is
+// measured, the per-label counters (either pointer may be NULL).
+extern "C" int pbx_sim_propagate_out(pbx_sim* s, double dt, int build_Dff, int build_Dnf, int cholesky,
+                                     double* xyz_out, long long* flux_out) {
+    int rc = pbx_sim_propagate(s, dt, build_Dff, build_Dnf, cholesky);
+    if (rc) return rc;
+    if (xyz_out) memcpy(xyz_out, s->pos_host, sizeof(double) * s->n);
+    if (flux_out && s->cfg.measure_flux) {
+        int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+        memcpy(flux_out, s->flux_host, sizeof(long long) * nl);
+    }
     return PBX_OK;
 }
 
