@@ -11,11 +11,14 @@ seeded N(0,1) stream.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
 """
 import argparse
+import ctypes
 import json
 import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -65,6 +68,8 @@ def main():
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
+    import ctypes
+
     import numpy as np
     import torch
 
@@ -77,29 +82,30 @@ def main():
     if args.potrf:
         assert lib.pbx_set_potrf_variant(args.potrf) == 0
     xyz, a, inp, dt, _ = workload(args.workload, seed=0)
+    N = len(a)
+    n = 3 * N
     box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp, xi_pool_attempts=64)
     for _ in range(args.warmup):
         box.propagate(dt)
     torch.cuda.synchronize()
+    # stage timing: HIP events recorded on the simulation's own stream inside
+    # every timed step (the step ends in a stream synchronisation anyway)
+    lib.pbx_sim_set_timing(box._sim, 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         box.propagate(dt)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    # stage breakdown with HIP events on the simulation's stream (separate steps)
-    import ctypes
-    import numpy as np
-    lib.pbx_sim_set_timing(box._sim, 1)
-    for _ in range(min(10, args.steps)):
-        box.propagate(dt)
     st = np.zeros(3)
     nt = lib.pbx_sim_get_timing(box._sim, st.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     lib.pbx_sim_set_timing(box._sim, 0)
+    ms = (t1 - t0) * 1e3 / args.steps
+    stage = st / max(nt, 1)
     print("stage ms/step: assembly %.3f  cholesky %.3f  propagation %.3f  (over %d steps; wall %.3f)"
-          % (st[0] / nt, st[1] / nt, st[2] / nt, nt, ms), file=sys.stderr)
+          % (stage[0], stage[1], stage[2], nt, ms), file=sys.stderr)
+
     out = {
-        "metric": "steps/s at N=1000 RPY-Ewald",
+        "metric": METRIC[args.workload],
         "value": 1e3 / ms,
         "unit": "steps/s",
         "n_gpus": 1,
@@ -111,10 +117,60 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "rpy_smith N=1000 a=10 phi=0.05 L=437.6 alpha=sqrt(pi) m=n=2 ermak dt=1ps",
-                   "N": len(a), "box_length": inp["box_length"]},
+        "config": {"workload": WORKLOAD_TEXT[args.workload], "N": N, "box_length": round(inp["box_length"], 4)},
+        "stage_ms": {"assembly": stage[0], "cholesky": stage[1], "propagation": stage[2]},
+        "roofline": roofline(args.workload, N, stage),
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out))
+
+
+METRIC = {"c2": "steps/s at N=1000 RPY-Ewald", "c1": "steps/s at N=100 RPY", "c3": "steps/s at N=500 RPY-Ewald+JO midpoint",
+          "c4": "steps/s at N=4000 RPY-Ewald"}
+WORKLOAD_TEXT = {
+    "c1": "rpy N=100 a=5 L=150 ermak dt=1ps flux x=0",
+    "c2": "rpy_smith N=1000 a=10 phi=0.05 L=437.6 alpha=sqrt(pi) m=n=2 ermak dt=1ps flux x=0",
+    "c3": "rpy_smith_lub N=500 a=20/10 phi=0.25 cutoff=0.5 midpoint m=100 dt=0.1ps",
+    "c4": "rpy_smith N=4000 a=10 phi=0.10 L=551.3 alpha=sqrt(pi) m=n=2 ermak dt=1ps",
+}
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+
+
+def smith_flops(N, m=2, nrec=2):
+    """Algorithmic flops of one Smith-Ewald assembly: per pair 63 per
+    real-space image (sqrt/erfc/exp counted as 1) and 10 per reciprocal
+    vector; self blocks skip the m = 0 image."""
+    def l1_ball(k):
+        return sum(1 for x in range(-k, k + 1) for y in range(-k, k + 1) for z in range(-k, k + 1)
+                   if abs(x) + abs(y) + abs(z) <= k)
+    real, recip = l1_ball(m), l1_ball(nrec) - 1
+    pair = real * 63 + recip * 10
+    self_ = (real - 1) * 63 + recip * 10
+    return pair * N * (N - 1) / 2 + self_ * N
+
+
+def roofline(name, N, stage):
+    """Roofline entry for the dominant stage of the step (by measured time)."""
+    n = 3 * N
+    k = int(np.argmax(stage))
+    if k == 1:
+        flops = n ** 3 / 3.0
+        ach = flops / (stage[1] * 1e-3) / 1e12
+        return {"kernel": "cholesky (potrf, n=%d)" % n, "bound": "mfma", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": "n^3/3 = %.3e flop per factorisation" % flops}
+    if k == 0:
+        flops = smith_flops(N) if name in ("c2", "c4", "c3") else 30.0 * N * (N - 1) / 2
+        ach = flops / (stage[0] * 1e-3) / 1e12
+        return {"kernel": "assemble_tile_kernel (mobility)", "bound": "mfma", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "algorithmic": "%.3e flop per assembly" % flops}
+    byts = 4.0 * n * (n + 1)
+    ach = byts / (stage[2] * 1e-3) / 1e9
+    return {"kernel": "propagation (B xi)", "bound": "hbm", "achieved": ach, "peak": 8000.0, "unit": "GB/s",
+            "frac": ach / 8000.0, "traffic": None, "algorithmic": "4n(n+1) = %.3e B per step" % byts}
 
 
 if __name__ == "__main__":
