@@ -123,23 +123,52 @@ __device__ __forceinline__ double rsqrt_nr(double d) {
 
 // Factor the SPD tile held in LDS (lower triangle, stride SP) in place and
 // form W = inv(L) (lower, zero above) in sW.
-//  * 8-column panels: wave 0 factors the panel (lane = row, panel in
-//    registers, pivots broadcast with readlane), then all four waves apply
-//    the K = 8 trailing update on the FP64 matrix cores;
-//  * inv(L) by 16 x 16 blocks: wave 0 inverts the four diagonal blocks
-//    (lane = (block, column)), then block rows 1..3 follow
-//    W_ib = -W_ii sum_{b<=k<i} L_ik W_kb.
+//
+// The trailing matrix and an augmented block R (initially I) live in the
+// MFMA accumulators of the four waves for the whole factorisation, so the
+// elimination applied to A is applied to R as well and R ends as inv(L)
+// (Gauss-Jordan on [L | I]).  Eight 8-column panels:
+//   1. owners copy the panel's columns of A and rows of R to LDS;
+//   2. wave 0 factors the panel (lane = row, pivots broadcast with
+//      readlane), then solves the panel's 8 rows of inv(L) (lane = column);
+//   3. all waves apply A -= P P^T and R -= P W_p (K = 8) on the FP64 matrix
+//      cores, with rows above the panel masked out through the A operand.
 // info receives col0 + column + 1 of the first non-positive pivot.
 __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0, int* info) {
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int e = tid; e < T64 * SP; e += 256) sW[e] = 0.0;
-    for (int kb = 0; kb < 8; ++kb) {
-        const int c0 = kb * 8;
-        if (w == 0) {
-            const bool act = lane >= c0;
-            double v[8];
+    const int r0 = (w >> 1) * 32, cq = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d accL[2][2], accR[2][2];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) v[q] = act ? sL[lane * SP + c0 + q] : 0.0;
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = acc_row(x, v), c = acc_col(y);
+                accL[x][y][v] = sL[r * SP + c];
+                accR[x][y][v] = (r == c) ? 1.0 : 0.0;
+            }
+    __syncthreads();
+    for (int p = 0; p < 8; ++p) {
+        const int c0 = 8 * p;
+        // 1. panel columns of A (rows >= c0) and panel rows of R to LDS
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = acc_row(x, v), c = acc_col(y);
+                    if (c >= c0 && c < c0 + 8 && r >= c0) sL[r * SP + c] = accL[x][y][v];
+                    if (r >= c0 && r < c0 + 8) sW[r * SP + c] = accR[x][y][v];
+                }
+        __syncthreads();
+        // 2. wave 0: panel factorisation, then the panel rows of inv(L)
+        if (w == 0) {
+            double v[8], yq[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = (lane >= c0) ? sL[lane * SP + c0 + q] : 0.0;
             int first_bad = -1;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -147,91 +176,64 @@ __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0,
                 const double d = bcast(v[q], j);
                 if (!(d > 0.0) && first_bad < 0) first_bad = q;
                 const double y = rsqrt_nr(d);
-                if (lane == j) {
-                    v[q] = d * y;
-                    sd[j] = d * y;
-                } else if (lane > j) {
-                    v[q] = v[q] * y;
-                }
+                yq[q] = y;
+                const double lj = d * y;
+                const double vs = v[q] * y;
+                v[q] = (lane == j) ? lj : ((lane > j) ? vs : v[q]);
 #pragma unroll
-                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] -= v[q] * bcast(v[q], c0 + q2);
+                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] = fma(-v[q], bcast(v[q], c0 + q2), v[q2]);
             }
-            if (act) {
+            if (lane >= c0) {
 #pragma unroll
                 for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
             }
+            // diagonal of the panel
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (lane == c0 + q) sd[c0 + q] = v[q];
             if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
+            // W[c0+q][c] = (R[c0+q][c] - sum_{q'<q} L[c0+q][c0+q'] W[c0+q'][c]) / L[c0+q][c0+q]
+            double wv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) wv[q] = sW[(c0 + q) * SP + lane];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+#pragma unroll
+                for (int q2 = 0; q2 < q; ++q2) wv[q] = fma(-bcast(v[q2], c0 + q), wv[q2], wv[q]);
+                wv[q] = wv[q] * yq[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sW[(c0 + q) * SP + lane] = wv[q];
         }
         __syncthreads();
-        if (kb < 7) {
-            const int r0 = (w >> 1) * 32, cq = (w & 1) * 32;
-            const int li = lane & 15, lk = lane >> 4;
+        // 3. trailing updates of A and R on the matrix cores
+        if (p < 7) {
             const int lo = c0 + 8;
-            const bool live = (r0 + 31 >= lo) && !(cq > r0 + 31);
-            v4d acc[2][2];
-            acc_zero(acc);
-            if (live) {
+            const bool rows_live = (r0 + 31 >= lo) && !(cq > r0 + 31);
+            const bool liveL = rows_live && (cq + 31 >= lo);
+            if (rows_live) {
 #pragma unroll
                 for (int ks = 0; ks < 8; ks += 4) {
                     const int k = c0 + ks + lk;
-                    double a0 = sL[(r0 + li) * SP + k], a1 = sL[(r0 + 16 + li) * SP + k];
-                    double b0 = sL[(cq + li) * SP + k], b1 = sL[(cq + 16 + li) * SP + k];
-                    acc[0][0] = mfma(a0, b0, acc[0][0]);
-                    acc[0][1] = mfma(a0, b1, acc[0][1]);
-                    acc[1][0] = mfma(a1, b0, acc[1][0]);
-                    acc[1][1] = mfma(a1, b1, acc[1][1]);
+                    const int ra = r0 + li, rb = r0 + 16 + li;
+                    const double a0 = (ra >= lo) ? -sL[ra * SP + k] : 0.0;
+                    const double a1 = (rb >= lo) ? -sL[rb * SP + k] : 0.0;
+                    const double w0 = sW[k * SP + cq + li], w1 = sW[k * SP + cq + 16 + li];
+                    accR[0][0] = mfma(a0, w0, accR[0][0]);
+                    accR[0][1] = mfma(a0, w1, accR[0][1]);
+                    accR[1][0] = mfma(a1, w0, accR[1][0]);
+                    accR[1][1] = mfma(a1, w1, accR[1][1]);
+                    if (liveL) {
+                        const double b0 = sL[(cq + li) * SP + k], b1 = sL[(cq + 16 + li) * SP + k];
+                        accL[0][0] = mfma(a0, b0, accL[0][0]);
+                        accL[0][1] = mfma(a0, b1, accL[0][1]);
+                        accL[1][0] = mfma(a1, b0, accL[1][0]);
+                        accL[1][1] = mfma(a1, b1, accL[1][1]);
+                    }
                 }
-                acc_for_each(acc, [&](int r, int c, double val) {
-                    if (r >= lo && c >= lo && c <= r) sL[r * SP + c] -= val;
-                });
             }
-            __syncthreads();
         }
     }
-    // diagonal 16 x 16 blocks of inv(L): wave 0, lane = (block b, column c);
-    // rows r >= c by forward substitution with the column kept in registers
-    if (w == 0) {
-        const int b = lane >> 4, c = lane & 15, o = 16 * b;
-        double col[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            double s = (r == c) ? 1.0 : 0.0;
-#pragma unroll
-            for (int k = 0; k < r; ++k)
-                if (k >= c) s -= sL[(o + r) * SP + o + k] * col[k];
-            col[r] = (r >= c) ? s / sd[o + r] : 0.0;
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) sW[(o + r) * SP + o + c] = col[r];
-    }
-    __syncthreads();
-    // block rows 1..3: each thread owns one element (row, col) of the blocks
-    // W_ib, b < i, of the current block row
-    for (int i = 1; i < 4; ++i) {
-        const int nb = i;  // blocks b = 0..i-1, 16 x 16 each
-        const int rr = tid >> 4, cc = tid & 15;  // 16 x 16 positions
-        double S[3] = {0.0, 0.0, 0.0};
-        // S_ib[rr][cc] = sum_{k=b}^{i-1} sum_t L[16i+rr][16k+t] W[16k+t][16b+cc]
-        for (int b = 0; b < nb; ++b) {
-            double s = 0.0;
-            for (int k = b; k < i; ++k)
-#pragma unroll
-                for (int t = 0; t < 16; ++t) s += sL[(16 * i + rr) * SP + 16 * k + t] * sW[(16 * k + t) * SP + 16 * b + cc];
-            S[b] = s;
-        }
-        __syncthreads();
-        __shared__ double sS[3][16][17];
-        for (int b = 0; b < nb; ++b) sS[b][rr][cc] = S[b];
-        __syncthreads();
-        for (int b = 0; b < nb; ++b) {
-            double s = 0.0;
-#pragma unroll
-            for (int t = 0; t < 16; ++t) s += sW[(16 * i + rr) * SP + 16 * i + t] * sS[b][t][cc];
-            sW[(16 * i + rr) * SP + 16 * b + cc] = -s;
-        }
-        __syncthreads();
-    }
-    if (tid < 64) sL[tid * SP + tid] = sd[tid];
     __syncthreads();
 }
 
