@@ -95,7 +95,7 @@ typedef struct pbx_config {
     int measure_flux;
     int n_labels;               /* distinct bead labels (flux counters)     */
     long long max_move_attempts;
-    int xi_pool_attempts;       /* noise draws fetched per refill (0 = 64)  */
+    int xi_pool_attempts;       /* attempts (3N draws) per noise chunk (0 = 64) */
     int reserved;
     double box_length, T, viscosity;
     double ewald_alpha;
@@ -110,7 +110,8 @@ typedef struct pbx_config {
 typedef struct PbxSim pbx_sim;
 
 /* Fills out[0..count) with successive N(0,1) draws; returns 0 on success.
- * The engine calls it whenever its noise pool is empty. */
+ * The engine calls it, on the thread that calls pbx_sim_propagate, for one
+ * chunk (xi_pool_attempts x 3N draws) at a time, ahead of use. */
 typedef int (*pbx_xi_fill_fn)(void* user, double* out, long long count);
 
 /* Box.__init__ (box.py:42-109): uploads radii, positions (N x 3) and label
@@ -119,6 +120,15 @@ int pbx_sim_create(const pbx_config* cfg, int N, const double* radii, const doub
                    pbx_sim** out);
 int pbx_sim_destroy(pbx_sim* sim);
 int pbx_sim_set_noise_source(pbx_sim* sim, pbx_xi_fill_fn fn, void* user);
+/* Native noise source: the stream numpy.random.RandomState(seed).normal(0, 1)
+ * produces (box.py:393-399, :621-629), generated ahead of use on a producer
+ * thread.  Either call discards draws taken from a previous source. */
+int pbx_sim_set_seed(pbx_sim* sim, unsigned int seed);
+/* Move attempts evaluated per batch in the Ermak scheme with overlap checks
+ * (1..32; 0 = adaptive, the default).  Trajectories do not depend on it. */
+int pbx_sim_set_attempt_batch(pbx_sim* sim, int K);
+/* count values of RandomState(seed).normal(0, 1) after skipping `skip`. */
+int pbx_rng_normals(unsigned int seed, long long skip, double* out, long long count);
 
 /* Box.propagate (box.py:150-198).  Returns 0; k > 0 for a non-SPD matrix
  * (pivot k-1); -4 when max_move_attempts is exhausted; other negatives for

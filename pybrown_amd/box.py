@@ -7,8 +7,13 @@ over the device-resident HIP simulation.
 Positions, D, B (and Rff for lubrication) live in HBM; after each propagate
 the new positions are copied once into a shared host array that every
 ``Bead.r`` views, so BD.py-style loops read ``box.beads[k].r`` unchanged.
-N(0,1) draws come from ``numpy.random.RandomState(seed)`` exactly as in
-box.py:393-399 and :621-629 (3N per move attempt, in attempt order).
+N(0,1) draws are the stream of ``numpy.random.RandomState(seed)``
+(box.py:393-399 and :621-629: 3N per move attempt, in attempt order).  By
+default the library generates that stream itself (``noise="native"``: the
+same MT19937 and polar Gaussian, bit for bit), ahead of use on a producer
+thread; ``pseudorandom_number_generator`` is then not advanced.
+``noise="callback"`` draws from ``pseudorandom_number_generator`` instead,
+one chunk of ``xi_pool_attempts`` attempts at a time.
 
 Scope: the hydrodynamic step.  Keywords the step does not cover (walls,
 reactions, force fields, immobile beads, divergence term, force or
@@ -46,7 +51,7 @@ _UNSUPPORTED = ("walls", "external_force_region", "measure_concentration", "reac
 class Box:
     """Simulation box (pyBrown/box.py:30-109)."""
 
-    def __init__(self, beads, input_data, xi_pool_attempts=16):
+    def __init__(self, beads, input_data, xi_pool_attempts=64, noise="native", attempt_batch=0):
         self.inp = dict(DEFAULTS)
         self.inp.update(input_data)
         inp = self.inp
@@ -152,7 +157,16 @@ class Box:
         _lib.check_rc(rc, "Box")
         self._sim = handle.value
         self._fill_cb = _lib.XI_FILL(self._fill)
-        lib.pbx_sim_set_noise_source(self._sim, self._fill_cb, None)
+        seed_ok = isinstance(self.seed, (int, np.integer)) and 0 <= int(self.seed) < 2**32
+        if noise not in ("native", "callback"):
+            raise ValueError("noise must be 'native' or 'callback'")
+        if noise == "native" and seed_ok:
+            _lib.check_rc(lib.pbx_sim_set_seed(self._sim, int(self.seed)), "set_seed")
+            self.noise = "native"
+        else:
+            _lib.check_rc(lib.pbx_sim_set_noise_source(self._sim, self._fill_cb, None), "set_noise_source")
+            self.noise = "callback"
+        _lib.check_rc(lib.pbx_sim_set_attempt_batch(self._sim, int(attempt_batch)), "set_attempt_batch")
         self._synced = self._pos.copy()
         self._counts = np.zeros(max(1, len(self.mobile_labels)), dtype=np.int64)
         self._pos_ptr = ptr(self._pos.reshape(-1))

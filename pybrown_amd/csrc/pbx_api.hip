@@ -6,10 +6,12 @@
 //     bead.c), so pyBrown's ctypes bindings can load this library unchanged.
 //  2. A device-resident simulation handle (pbx_sim_*) that keeps positions,
 //     matrices and factors in HBM and performs Box.propagate
-//     (src/bdsim/pyBrown/box.py:150-349) with HIP kernels; the host supplies
-//     N(0,1) draws through a callback so the reference's RandomState stream
-//     is consumed attempt by attempt exactly as box.py consumes it.
+//     (src/bdsim/pyBrown/box.py:150-349) with HIP kernels.  The N(0,1) draws
+//     come from the native RandomState-equivalent generator (pbx_noise.hip,
+//     run ahead on a producer thread) or from a host callback, and are
+//     consumed attempt by attempt in the order box.py consumes them.
 #include "pbx_common.h"
+#include "pbx_noise.h"
 #include "../../include/pybrown_amd.h"
 
 #include <math.h>
@@ -18,6 +20,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <condition_variable>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 // launchers from pbx_step.hip
@@ -303,14 +308,31 @@ struct PbxSim {
     double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
     double *Wchol = nullptr, *Winv = nullptr;
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
-    // noise pool
+    // noise: chunks of xi_cap attempts (3N draws each) in order; the device
+    // ring holds two chunks (chunk q in slot q % 2), the pinned host side
+    // three (chunk q in buffer q % 3, reusable once xi_ev[q % 3] completes)
     double* xi_dev = nullptr;
-    double* xi_host = nullptr;  // pinned
-    long long xi_cap = 0;       // attempts per refill
-    long long xi_next = 0, xi_have = 0;
+    double* xi_host = nullptr;
+    hipEvent_t xi_ev[3] = {nullptr, nullptr, nullptr};
+    long long xi_cap = 0;
+    long long xi_used = 0;   // attempts consumed from the current source
+    long long xi_taken = 0;  // chunks whose upload has been queued
     pbx_xi_fill_fn xi_fill = nullptr;
     void* xi_user = nullptr;
     long long draw_count = 0;
+    // native source: producer thread filling host chunks ahead of use
+    int native = 0;
+    PbxMT mt;
+    std::thread producer;
+    std::mutex mu;
+    std::condition_variable cv;
+    long long xi_produced = 0;
+    bool stop = false;
+    // batched move attempts (Ermak with overlap checks)
+    double *Pbuf = nullptr, *cand = nullptr;
+    int* cflag = nullptr;
+    int batch_fixed = 0;       // 0: adaptive
+    double att_mean = 2.0;     // running mean of attempts per step
     int* flag_host = nullptr;  // pinned: [0] overlap flag, [1] factorisation status
     double* pos_host = nullptr;      // pinned copy of positions (end of step)
     long long* flux_host = nullptr;  // pinned copy of flux counters
@@ -429,26 +451,100 @@ int factor(PbxSim* s) {
     return PBX_OK;
 }
 
-// next 3N draws of the host N(0,1) stream (one move attempt, box.py:393-399)
-int next_xi(PbxSim* s, const double** out) {
-    if (s->xi_next >= s->xi_have) {
+// ---- noise ---------------------------------------------------------------
+
+size_t chunk_doubles(const PbxSim* s) { return (size_t)s->xi_cap * s->n; }
+
+void producer_main(PbxSim* s) {
+    for (long long q = s->xi_produced;; ++q) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            // host buffer q % 3 last held chunk q - 3: wait until its upload is queued
+            s->cv.wait(lk, [&] { return s->stop || s->xi_taken >= q - 2; });
+            if (s->stop) return;
+        }
+        if (q >= 3) (void)hipEventSynchronize(s->xi_ev[q % 3]);
+        pbx_mt_normals(&s->mt, s->xi_host + (q % 3) * chunk_doubles(s), (long long)chunk_doubles(s));
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->xi_produced = q + 1;
+        }
+        s->cv.notify_all();
+    }
+}
+
+void noise_stop(PbxSim* s) {
+    if (s->producer.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->producer.join();
+    }
+    s->stop = false;
+}
+
+// Drop every drawn-but-unused value and start a new source.
+int noise_reset(PbxSim* s) {
+    noise_stop(s);
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    s->xi_used = s->xi_taken = s->xi_produced = 0;
+    s->native = 0;
+    s->xi_fill = nullptr;
+    return PBX_OK;
+}
+
+// Queue the upload of chunk q into device slot q % 2.
+int noise_take(PbxSim* s, long long q) {
+    double* hb = s->xi_host + (q % 3) * chunk_doubles(s);
+    if (s->native) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.wait(lk, [&] { return s->xi_produced > q; });
+    } else {
         if (!s->xi_fill) {
             pbx_set_error(PBX_ERR_STATE, "no noise source registered");
             return PBX_ERR_STATE;
         }
-        PBX_HIP(hipStreamSynchronize(s->stream));  // previous pool no longer in use
-        long long cnt = s->xi_cap * (long long)s->n;
-        if (s->xi_fill(s->xi_user, s->xi_host, cnt) != 0) {
+        if (q >= 3) PBX_HIP(hipEventSynchronize(s->xi_ev[q % 3]));
+        if (s->xi_fill(s->xi_user, hb, (long long)chunk_doubles(s)) != 0) {
             pbx_set_error(PBX_ERR_STATE, "noise source callback failed");
             return PBX_ERR_STATE;
         }
-        PBX_HIP(hipMemcpyAsync(s->xi_dev, s->xi_host, sizeof(double) * cnt, hipMemcpyHostToDevice, s->stream));
-        s->xi_next = 0;
-        s->xi_have = s->xi_cap;
     }
-    *out = s->xi_dev + s->xi_next * (long long)s->n;
-    s->xi_next += 1;
-    s->draw_count += s->n;
+    PBX_HIP(hipMemcpyAsync(s->xi_dev + (q % 2) * chunk_doubles(s), hb, sizeof(double) * chunk_doubles(s),
+                           hipMemcpyHostToDevice, s->stream));
+    PBX_HIP(hipEventRecord(s->xi_ev[q % 3], s->stream));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->xi_taken = q + 1;
+    }
+    s->cv.notify_all();
+    return PBX_OK;
+}
+
+// Make attempts xi_used .. xi_used + K - 1 resident (K <= xi_cap, so they
+// span at most two chunks; uploading chunk q overwrites chunk q - 2, whose
+// readers precede it on the stream).
+int noise_ensure(PbxSim* s, long long K) {
+    const long long q1 = (s->xi_used + K - 1) / s->xi_cap;
+    while (s->xi_taken <= q1) TRY(noise_take(s, s->xi_taken));
+    return PBX_OK;
+}
+
+int ring_attempts(const PbxSim* s) { return (int)(2 * s->xi_cap); }
+int ring_pos(const PbxSim* s) { return (int)(s->xi_used % (2 * s->xi_cap)); }
+
+void noise_consume(PbxSim* s, long long k) {
+    s->xi_used += k;
+    s->draw_count += k * s->n;
+}
+
+// next 3N draws (one move attempt, box.py:393-399)
+int next_xi(PbxSim* s, const double** out) {
+    TRY(noise_ensure(s, 1));
+    *out = s->xi_dev + (size_t)ring_pos(s) * s->n;
+    noise_consume(s, 1);
     return PBX_OK;
 }
 
@@ -486,27 +582,58 @@ int undo(PbxSim* s, StepCtx& c) {
     return pbx_launch_translate(s->disp, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
 }
 
-// box.py:202-250
+// box.py:202-250.  With overlap checks the attempts run in batches of K:
+// one pass over B yields K candidate moves, one kernel flags those that
+// overlap, and the first clean one is applied; the host reads one index per
+// batch instead of one flag per attempt.  Results do not depend on K.
 int ermak(PbxSim* s, double dt, StepCtx& c) {
     TRY(det(s, s->D, dt, 1.0, c));
-    int attempts = 0;
-    while (true) {
-        ++attempts;
+    if (!s->cfg.check_overlaps) {
         const double* xi;
         TRY(next_xi(s, &xi));
         TRY(stoch(s, s->B, xi, 1.0, c));
-        if (!s->cfg.check_overlaps) break;
-        int hit = 0;
-        TRY(overlap_now(s, &hit));
-        if (!hit) break;
-        TRY(undo(s, c));
+        s->last_attempts = 1;
+        return PBX_OK;
+    }
+    const int nohi = s->cfg.hydrodynamics == PBX_NOHI;
+    const int kmax = pbx_attempt_kmax();
+    int want = s->batch_fixed;
+    if (want <= 0) {
+        want = 2;
+        while (want < 2.0 * s->att_mean && want < kmax) want *= 2;
+    }
+    long long attempts = 0;
+    while (true) {
+        long long K = want;
+        if (K > kmax) K = kmax;
+        if (K > s->xi_cap) K = s->xi_cap;
+        if (K > s->cfg.max_move_attempts - attempts) K = s->cfg.max_move_attempts - attempts;
+        if (K < 1) K = 1;
+        TRY(noise_ensure(s, K));
+        TRY(pbx_launch_attempt_batch(nohi ? nullptr : s->B, nohi ? s->Bd : nullptr, s->ld, s->np, s->N, s->pos,
+                                     s->xi_dev, ring_pos(s), ring_attempts(s), (int)K, c.sq2dt, s->Pbuf, s->cand,
+                                     s->cflag, s->a, s->cfg.box_length, s->cfg.overlap_treshold, s->stream));
+        TRY(pbx_launch_attempt_select(s->cand, s->N, (int)K, s->cflag, s->pos, c.pn, c.pp, c.flux_on, s->label,
+                                      s->flux, s->flag, s->stream));
+        PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+        PBX_HIP(hipStreamSynchronize(s->stream));
+        TRY(take_info(s));
+        const int a = s->flag_host[0];
+        if (a < K) {
+            attempts += a + 1;
+            noise_consume(s, a + 1);
+            break;
+        }
+        attempts += K;
+        noise_consume(s, K);
         if (attempts >= s->cfg.max_move_attempts) {
-            s->last_attempts = attempts;
+            s->last_attempts = (int)attempts;
             pbx_set_error(PBX_ERR_MOVES, "maximal number of move attepts exceeded");
             return PBX_ERR_MOVES;
         }
     }
-    s->last_attempts = attempts;
+    s->last_attempts = (int)attempts;
+    s->att_mean = 0.75 * s->att_mean + 0.25 * (double)attempts;
     return PBX_OK;
 }
 
@@ -615,8 +742,21 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     if ((rc = sim_alloc(s, (void**)&s->flux, sizeof(long long) * nl))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->flag, sizeof(int) * 2))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->info, sizeof(int)))) return fail(rc);
-    if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * s->xi_cap * n3))) return fail(rc);
-    if (hipHostMalloc((void**)&s->xi_host, sizeof(double) * s->xi_cap * n3, hipHostMallocDefault) != hipSuccess ||
+    if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * 2 * s->xi_cap * n3))) return fail(rc);
+    if (cfg->scheme == PBX_ERMAK && cfg->check_overlaps) {
+        const int kmax = pbx_attempt_kmax();
+        if ((rc = sim_alloc(s, (void**)&s->cand, sizeof(double) * (kmax + 1) * n3))) return fail(rc);
+        if ((rc = sim_alloc(s, (void**)&s->cflag, sizeof(int) * kmax))) return fail(rc);
+        if (cfg->hydrodynamics != PBX_NOHI &&
+            (rc = sim_alloc(s, (void**)&s->Pbuf, sizeof(double) * pbx_attempt_partial_doubles(s->np))))
+            return fail(rc);
+    }
+    for (int k = 0; k < 3; ++k)
+        if (hipEventCreateWithFlags(&s->xi_ev[k], hipEventDisableTiming) != hipSuccess) {
+            pbx_set_error(PBX_ERR_HIP, "hipEventCreate failed");
+            return fail(PBX_ERR_HIP);
+        }
+    if (hipHostMalloc((void**)&s->xi_host, sizeof(double) * 3 * s->xi_cap * n3, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->flag_host, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->pos_host, sizeof(double) * n3, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->flux_host, sizeof(long long) * nl, hipHostMallocDefault) != hipSuccess) {
@@ -694,9 +834,12 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
 
 extern "C" int pbx_sim_destroy(pbx_sim* s) {
     if (!s) return PBX_OK;
+    noise_stop(s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int k = 0; k < 4; ++k)
         if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    for (int k = 0; k < 3; ++k)
+        if (s->xi_ev[k]) (void)hipEventDestroy(s->xi_ev[k]);
     for (void* p : s->owned) (void)hipFree(p);
     if (s->xi_host) (void)hipHostFree(s->xi_host);
     if (s->flag_host) (void)hipHostFree(s->flag_host);
@@ -708,9 +851,34 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
 }
 
 extern "C" int pbx_sim_set_noise_source(pbx_sim* s, pbx_xi_fill_fn fn, void* user) {
+    pbx_clear_error();
+    TRY(noise_reset(s));
     s->xi_fill = fn;
     s->xi_user = user;
-    s->xi_next = s->xi_have = 0;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_seed(pbx_sim* s, unsigned int seed) {
+    pbx_clear_error();
+    TRY(noise_reset(s));
+    pbx_mt_seed(&s->mt, seed);
+    s->native = 1;
+    try {
+        s->producer = std::thread(producer_main, s);
+    } catch (...) {
+        s->native = 0;
+        pbx_set_error(PBX_ERR_STATE, "could not start the noise producer thread");
+        return PBX_ERR_STATE;
+    }
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_attempt_batch(pbx_sim* s, int K) {
+    if (K < 0 || K > pbx_attempt_kmax()) {
+        pbx_set_error(PBX_ERR_ARG, "attempt batch must lie in [0, %d] (0 = adaptive)", pbx_attempt_kmax());
+        return PBX_ERR_ARG;
+    }
+    s->batch_fixed = K;
     return PBX_OK;
 }
 

@@ -56,6 +56,22 @@ __device__ __forceinline__ double pbx_min_image(double d, double L) {
     return d;
 }
 
+// clib/bead.c:169-260 pair test: overlap iff no raw axis difference lies in
+// the open skip band (S, L-S) / (-(L-S), -S) and the minimum-image distance
+// is at most S + threshold (S = ai + aj).  d = r_i - r_j.
+__device__ __forceinline__ bool pbx_pair_overlaps(double dx, double dy, double dz, double ai, double aj, double L,
+                                                  double thr) {
+#pragma clang fp contract(off)
+    double d[3] = {dx, dy, dz};
+    const double S = ai + aj;
+    const double Sp = L - S;
+    for (int c = 0; c < 3; ++c)
+        if ((d[c] > S && d[c] < Sp) || (d[c] < -S && d[c] > -Sp)) return false;
+    for (int c = 0; c < 3; ++c) d[c] = pbx_min_image(d[c], L);
+    const double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    return dist <= ai + aj + thr;
+}
+
 // Host-side check shared by every launcher that reduces by a box length.
 static inline bool pbx_box_ok(double L) { return L > 0.0 && L < 1e300; }
 #define PBX_CHECK_BOX(L)                                                                   \
@@ -93,4 +109,17 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
 size_t pbx_dense_work_doubles(int np);
+
+// batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()
+// consecutive attempts (xi of attempt k in ring slot (ring_pos + k) mod
+// ring_att), their overlap flags, then the selection of the first accepted
+int pbx_attempt_kmax();
+int pbx_attempt_slab_width(int np);
+size_t pbx_attempt_partial_doubles(int np);
+int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
+                             const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
+                             double* cand, int* cflag, const double* a, double L, double thr, hipStream_t s);
+int pbx_launch_attempt_select(const double* cand, int N, int K, const int* cflag, double* pos, const double* plane_n,
+                              const double* plane_p, int flux_on, const int* label, long long* flux, int* sel,
+                              hipStream_t s);
 int pbx_gemm_nt_test(const double* A, const double* B, double* C, int M, int N, int K, hipStream_t s);

@@ -402,21 +402,9 @@ __global__ __launch_bounds__(256) void overlaps_kernel(const double* __restrict_
     const int ti = threadIdx.x / TB, tj = threadIdx.x % TB;
     const int i = bi * TB + ti, j = bj * TB + tj;
     bool hit = false;
-    if (i < N && j < i && !(mask && mask[(size_t)j + (size_t)N * i])) {
-        double d[3];
-        for (int c = 0; c < 3; ++c) d[c] = xyz[3 * i + c] - xyz[3 * j + c];
-        double S = a[i] + a[j];
-        double Sp = L - S;
-        bool skip = false;
-        for (int c = 0; c < 3; ++c)
-            if ((d[c] > S && d[c] < Sp) || (d[c] < -S && d[c] > -Sp)) skip = true;
-        if (!skip) {
-#pragma clang fp contract(off)
-            for (int c = 0; c < 3; ++c) d[c] = pbx_min_image(d[c], L);
-            double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-            hit = dist <= a[i] + a[j] + thr;
-        }
-    }
+    if (i < N && j < i && !(mask && mask[(size_t)j + (size_t)N * i]))
+        hit = pbx_pair_overlaps(xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1], xyz[3 * i + 2] - xyz[3 * j + 2],
+                                a[i], a[j], L, thr);
     if (__any(hit) && (threadIdx.x % 64) == 0) atomicOr(flag, 1);
 }
 

@@ -208,6 +208,174 @@ __global__ __launch_bounds__(1024) void trsv_lt_kernel(const double* __restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Batched move attempts (box.py:202-250 loop, evaluated K attempts at a time).
+//
+// A rejected Ermak attempt is undone by translating back, so attempt k starts
+// from base_k = (base_{k-1} + d_{k-1}) - d_{k-1}; the candidate positions of
+// K consecutive attempts follow from K displacement vectors d_k = B xi_k
+// sqrt(2 dt), which one pass over B yields for all k.  The first candidate
+// without an overlap is the accepted move, exactly as in the attempt-by-
+// attempt loop.  Net plane crossings over a chain of translations telescope
+// to side(end) - side(start) per bead.
+
+constexpr int BT = 64;        // B tile edge
+constexpr int BSP = 65;       // LDS stride
+constexpr int KC_MAX = 32;    // candidates per batch
+
+// Partial products P[s][r][k] = sum_{c in slab s, c <= r} B[r][c] xi_k[c] for
+// row block i = blockIdx.x and column slab s = blockIdx.y (S tiles wide).
+// Wave w computes rows 16w..16w+15 of the block for all KC candidates with
+// v_mfma_f64_16x16x4_f64 (A = B rows from LDS, B operand = xi tile from LDS).
+template <int KC>
+__global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restrict__ B, int ld, int np, int n,
+                                                          const double* __restrict__ ring, int ring_pos, int ring_att,
+                                                          int K, int S, double* __restrict__ P) {
+    typedef double v4d __attribute__((ext_vector_type(4)));
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    __shared__ double sB[BT * BSP];
+    __shared__ double sX[KC * BSP];
+    const int i = blockIdx.x, sl = blockIdx.y;
+    const int j0 = sl * S;
+    if (j0 > i) return;
+    const int j1 = min(j0 + S, i + 1);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    v4d acc[KC / 16];
+#pragma unroll
+    for (int cb = 0; cb < KC / 16; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const double* Bblk = B + (size_t)i * BT * ld;
+    for (int j = j0; j < j1; ++j) {
+        // issue all global loads of the tile and its xi block, then store
+        v2d vb[8];
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int e = tid + it * 256;
+            vb[it] = *(const v2d*)(Bblk + (size_t)(e >> 5) * ld + (size_t)j * BT + (e & 31) * 2);
+        }
+        double vx[KC / 4];
+#pragma unroll
+        for (int it = 0; it < KC / 4; ++it) {
+            const int e = tid + it * 256;
+            const int k = e >> 6, c = j * BT + (e & 63);
+            int slot = ring_pos + k;
+            if (slot >= ring_att) slot -= ring_att;
+            vx[it] = (k < K && c < n) ? ring[(size_t)slot * n + c] : 0.0;
+        }
+        __syncthreads();  // previous tile fully consumed
+#pragma unroll
+        for (int it = 0; it < 8; ++it) {
+            const int e = tid + it * 256;
+            const int r = e >> 5, c = (e & 31) * 2;
+            // diagonal tile: only c <= r belongs to the lower-triangular B
+            sB[r * BSP + c] = (j == i && c > r) ? 0.0 : vb[it].x;
+            sB[r * BSP + c + 1] = (j == i && c + 1 > r) ? 0.0 : vb[it].y;
+        }
+#pragma unroll
+        for (int it = 0; it < KC / 4; ++it) {
+            const int e = tid + it * 256;
+            sX[(e >> 6) * BSP + (e & 63)] = vx[it];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int c = 0; c < BT; c += 4) {
+            const double a = sB[(16 * w + li) * BSP + c + lk];
+#pragma unroll
+            for (int cb = 0; cb < KC / 16; ++cb)
+                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sX[(16 * cb + li) * BSP + c + lk], acc[cb], 0, 0, 0);
+        }
+    }
+    // accumulator register v of lane group lk holds row lk + 4v, column li
+    double* Ps = P + (size_t)sl * np * KC;
+#pragma unroll
+    for (int cb = 0; cb < KC / 16; ++cb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = i * BT + 16 * w + lk + 4 * v;
+            Ps[(size_t)r * KC + 16 * cb + li] = acc[cb][v];
+        }
+}
+
+// Candidate positions of K attempts from the current positions: thread per
+// coordinate; d_k = (sum of the slab partials in slab order) * sqrt(2 dt)
+// (B xi, box.py:371), or ((Bd xi) sqrt(2 dt)) for the diagonal nohi factor
+// (box.py:369).  cand[k][r] = base + d_k, base <- cand - d_k; row K keeps the
+// chain end.  Also clears the K overlap flags.
+__global__ void cand_kernel(const double* __restrict__ pos, const double* __restrict__ P, int np, int KC, int S, int n,
+                            int K, double sq2dt, const double* __restrict__ Bd, const double* __restrict__ ring,
+                            int ring_pos, int ring_att, double* __restrict__ cand, int* __restrict__ cflag) {
+    // d_k is rounded on its own before it is added, as the reference forms
+    // the displacement array first (no fused multiply-add into base + d)
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < K) cflag[r] = 0;
+    if (r >= n) return;
+    double base = pos[r];
+    const int ns = (r / BT) / S + 1;
+    for (int k = 0; k < K; ++k) {
+        double d;
+        if (Bd) {
+            int slot = ring_pos + k;
+            if (slot >= ring_att) slot -= ring_att;
+            d = ((Bd[r] * ring[(size_t)slot * n + r]) * sq2dt) * 1.0;
+        } else {
+            double acc = 0.0;
+            for (int s = 0; s < ns; ++s) acc += P[((size_t)s * np + r) * KC + k];
+            d = (acc * sq2dt) * 1.0;
+        }
+        const double c = base + d;
+        cand[(size_t)k * n + r] = c;
+        base = c - d;
+    }
+    cand[(size_t)K * n + r] = base;
+}
+
+// Overlap flags of K candidates: lower 16 x 16 bead tiles, one pair per
+// thread, candidates in order.
+__global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __restrict__ cand, const double* __restrict__ a,
+                                                             int N, int K, double L, double thr, int* cflag) {
+    const int t = blockIdx.x;
+    int bi = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+    while (bi * (bi + 1) / 2 > t) --bi;
+    const int bj = t - bi * (bi + 1) / 2;
+    const int i = bi * 16 + (threadIdx.x >> 4), j = bj * 16 + (threadIdx.x & 15);
+    const bool live = i < N && j < i;
+    const double ai = live ? a[i] : 0.0, aj = live ? a[j] : 0.0;
+    const int n = 3 * N;
+    for (int k = 0; k < K; ++k) {
+        bool hit = false;
+        if (live) {
+            const double* C = cand + (size_t)k * n;
+            hit = pbx_pair_overlaps(C[3 * i] - C[3 * j], C[3 * i + 1] - C[3 * j + 1], C[3 * i + 2] - C[3 * j + 2], ai,
+                                    aj, L, thr);
+        }
+        if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(&cflag[k], 1);
+    }
+}
+
+// Apply the first candidate without an overlap (or the chain end when all K
+// overlapped): positions, per-label net crossings, and the index in *sel.
+__global__ void select_kernel(const double* __restrict__ cand, int N, int K, const int* __restrict__ cflag, double* pos,
+                              FluxPlane f, const int* __restrict__ label, long long* flux, int* sel) {
+    int acc = K;
+    for (int k = K - 1; k >= 0; --k)
+        if (!cflag[k]) acc = k;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) *sel = acc;
+    if (b >= N) return;
+    const double* src = cand + (size_t)acc * 3 * N;
+    const double x0 = pos[3 * b], y0 = pos[3 * b + 1], z0 = pos[3 * b + 2];
+    const double x1 = src[3 * b], y1 = src[3 * b + 1], z1 = src[3 * b + 2];
+    pos[3 * b] = x1;
+    pos[3 * b + 1] = y1;
+    pos[3 * b + 2] = z1;
+    if (f.on) {
+        const int s0 = plane_side(f, x0, y0, z0), s1 = plane_side(f, x1, y1, z1);
+        if (s0 != s1) atomicAdd((unsigned long long*)&flux[label[b]], (unsigned long long)(long long)(s1 - s0));
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -297,6 +465,78 @@ int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, cons
 
 int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, hipStream_t s) {
     hipLaunchKernelGGL(trsv_lt_kernel, dim3(1), dim3(1024), 0, s, Lm, np, ld, W, x, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched attempts
+
+int pbx_attempt_slab_width(int np) {
+    const int T = np / BT;
+    int S = (T + 11) / 12;
+    return S < 2 ? 2 : S;
+}
+
+size_t pbx_attempt_partial_doubles(int np) {
+    const int T = np / BT;
+    const int S = pbx_attempt_slab_width(np);
+    const int nslab = (T + S - 1) / S;
+    return (size_t)nslab * np * KC_MAX;
+}
+
+int pbx_attempt_kmax() { return KC_MAX; }
+
+int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
+                             const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
+                             double* cand, int* cflag, const double* a, double L, double thr, hipStream_t s) {
+    if (K < 1 || K > KC_MAX || ring_pos < 0 || ring_pos >= ring_att) {
+        pbx_set_error(PBX_ERR_ARG, "attempt batch: K=%d ring_pos=%d ring_att=%d", K, ring_pos, ring_att);
+        return PBX_ERR_ARG;
+    }
+    PBX_CHECK_BOX(L);
+    const int n = 3 * N;
+    const int S = pbx_attempt_slab_width(np);
+    const int KC = K <= 16 ? 16 : 32;
+    if (!Bd) {
+        if (np % BT || n > np) {
+            pbx_set_error(PBX_ERR_ARG, "attempt batch: bad order np=%d n=%d", np, n);
+            return PBX_ERR_ARG;
+        }
+        const int T = np / BT;
+        dim3 grid(T, (T + S - 1) / S);
+        if (KC == 16)
+            hipLaunchKernelGGL(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
+                               K, S, P);
+        else
+            hipLaunchKernelGGL(stoch_batch_kernel<32>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
+                               K, S, P);
+        PBX_HIP(hipGetLastError());
+    }
+    const int nthr = n > K ? n : K;
+    hipLaunchKernelGGL(cand_kernel, dim3((nthr + 255) / 256), dim3(256), 0, s, pos, P, np, KC, S, n, K, sq2dt, Bd,
+                       ring, ring_pos, ring_att, cand, cflag);
+    PBX_HIP(hipGetLastError());
+    if (N > 1) {
+        const int T16 = (N + 15) / 16;
+        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(T16 * (T16 + 1) / 2), dim3(256), 0, s, cand, a, N, K, L, thr,
+                           cflag);
+        PBX_HIP(hipGetLastError());
+    }
+    return PBX_OK;
+}
+
+int pbx_launch_attempt_select(const double* cand, int N, int K, const int* cflag, double* pos, const double* plane_n,
+                              const double* plane_p, int flux_on, const int* label, long long* flux, int* sel,
+                              hipStream_t s) {
+    FluxPlane f;
+    for (int c = 0; c < 3; ++c) {
+        f.n[c] = plane_n ? plane_n[c] : 0.0;
+        f.p[c] = plane_p ? plane_p[c] : 0.0;
+    }
+    f.on = flux_on;
+    hipLaunchKernelGGL(select_kernel, dim3((N + 255) / 256), dim3(256), 0, s, cand, N, K, cflag, pos, f, label, flux,
+                       sel);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

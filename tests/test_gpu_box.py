@@ -15,17 +15,17 @@ pytestmark = pytest.mark.gpu
 FLUX = {"normal": [1.0, 0.0, 0.0], "plane_point": [0.0, 0.0, 0.0]}
 
 
-def _gpu_box(xyz, a, inp, labels=None):
+def _gpu_box(xyz, a, inp, labels=None, **kw):
     from pybrown_amd.bead import Bead
     from pybrown_amd.box import Box
     labels = labels or ["XXX"] * len(a)
     beads = [Bead(p, r, label=lab) for p, r, lab in zip(xyz, a, labels)]
-    return Box(beads, inp)
+    return Box(beads, inp, **kw)
 
 
-def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None):
+def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None, **kw):
     labels = labels or ["XXX"] * len(a)
-    g = _gpu_box(xyz, a, inp, labels)
+    g = _gpu_box(xyz, a, inp, labels, **kw)
     o = OracleBox(xyz, a, labels, inp)
     worst = 0.0
     for j in range(steps):
@@ -155,3 +155,76 @@ def test_single_launch_cholesky_trajectory():
         _run_pair(xyz, a, inp, 10, 1.0)
     finally:
         lib.pbx_set_potrf_variant(2)
+
+
+def _trajectory(xyz, a, inp, steps, dt, **kw):
+    g = _gpu_box(xyz, a, inp, **kw)
+    out = []
+    for _ in range(steps):
+        g.propagate(dt)
+        out.append((np.array([b.r for b in g.beads]), dict(getattr(g, "net_flux", {})), g.last_move_attempts))
+    return out, g.draw_count
+
+
+def test_attempt_batch_size_does_not_change_the_trajectory():
+    # rejection-heavy: batches of 1, 3 and 32 attempts and the adaptive size
+    # give bit-identical positions, flux, attempt and draw counts
+    xyz, a = _grid([-2.3, 0.0, 2.3], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy", seed=3, measure_flux=FLUX)
+    ref, ref_draws = _trajectory(xyz, a, inp, 30, 0.02, attempt_batch=1)
+    assert sum(t[2] for t in ref) > 30
+    for K in (3, 32, 0):
+        got, draws = _trajectory(xyz, a, inp, 30, 0.02, attempt_batch=K)
+        assert draws == ref_draws
+        for (p0, f0, n0), (p1, f1, n1) in zip(ref, got):
+            assert np.array_equal(p0, p1) and f0 == f1 and n0 == n1
+
+
+def test_nohi_batched_rejections_bit_exact():
+    xyz, a = _grid([-2.2, 0.0, 2.2], 1.0)
+    inp = dict(BASE, hydrodynamics="nohi", seed=4, measure_flux=FLUX)
+    g, o, worst = _run_pair(xyz, a, inp, 40, 0.02, tol=0.0)
+    assert sum(o.attempt_log) > len(o.attempt_log)
+
+
+def test_callback_noise_matches_native():
+    xyz, a = _grid([-2.3, 0.0, 2.3], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy_smith", ewald_real=1, ewald_imag=1, seed=11, measure_flux=FLUX)
+    nat, d0 = _trajectory(xyz, a, inp, 20, 0.02, noise="native", xi_pool_attempts=5)
+    cb, d1 = _trajectory(xyz, a, inp, 20, 0.02, noise="callback", xi_pool_attempts=7)
+    assert d0 == d1
+    for (p0, f0, n0), (p1, f1, n1) in zip(nat, cb):
+        assert np.array_equal(p0, p1) and f0 == f1 and n0 == n1
+
+
+def test_max_move_attempts_exhausted_matches_oracle():
+    # gaps of 0.1 A: the first step of most seeds needs more than 4 attempts;
+    # pick one on the CPU oracle
+    xyz, a = _grid([-2.1, 0.0, 2.1], 1.0)
+    inp = None
+    for seed in range(200):
+        trial = dict(BASE, hydrodynamics="rpy", seed=seed, max_move_attempts=4, measure_flux=FLUX)
+        o = OracleBox(xyz, a, ["XXX"] * len(a), trial)
+        try:
+            o.propagate(0.02)
+        except AssertionError:
+            inp = trial
+            break
+    assert inp is not None
+    g = _gpu_box(xyz, a, inp)
+    with pytest.raises(AssertionError, match="move attepts"):
+        g.propagate(0.02)
+    assert g.last_move_attempts == 4 and g.draw_count == o.draw_count
+    pos = np.array([b.r for b in g.beads])
+    from pybrown_amd._lib import lib, ptr
+    lib.pbx_sim_get_positions(g._sim, ptr(pos.reshape(-1)))
+    # positions are the chain end before the wrap (no wrap after a failed step)
+    assert np.abs(pos - o.r).max() <= 1e-10 * np.abs(o.r).max()
+
+
+def test_c2_full_size_first_steps():
+    # the benchmark workload (N = 1000, Smith-Ewald, ~20 attempts per step)
+    from pybrown_amd.workloads import workload
+    xyz, a, inp, dt, _ = workload("c2", seed=0)
+    g, o, _ = _run_pair(xyz, a, inp, 3, dt)
+    print("attempts:", o.attempt_log)
