@@ -745,7 +745,8 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * 2 * s->xi_cap * n3))) return fail(rc);
     if (cfg->scheme == PBX_ERMAK && cfg->check_overlaps) {
         const int kmax = pbx_attempt_kmax();
-        if ((rc = sim_alloc(s, (void**)&s->cand, sizeof(double) * (kmax + 1) * n3))) return fail(rc);
+        // K + 1 candidate rows, then N displacement bounds
+        if ((rc = sim_alloc(s, (void**)&s->cand, sizeof(double) * ((kmax + 1) * n3 + N)))) return fail(rc);
         if ((rc = sim_alloc(s, (void**)&s->cflag, sizeof(int) * kmax))) return fail(rc);
         if (cfg->hydrodynamics != PBX_NOHI &&
             (rc = sim_alloc(s, (void**)&s->Pbuf, sizeof(double) * pbx_attempt_partial_doubles(s->np))))

@@ -245,15 +245,15 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
 #pragma unroll
     for (int cb = 0; cb < KC / 16; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
     const double* Bblk = B + (size_t)i * BT * ld;
-    for (int j = j0; j < j1; ++j) {
-        // issue all global loads of the tile and its xi block, then store
-        v2d vb[8];
+    // global loads of tile j into registers (B tile and its xi block)
+    v2d vb[8];
+    double vx[KC / 4];
+    auto fetch = [&](int j) {
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int e = tid + it * 256;
             vb[it] = *(const v2d*)(Bblk + (size_t)(e >> 5) * ld + (size_t)j * BT + (e & 31) * 2);
         }
-        double vx[KC / 4];
 #pragma unroll
         for (int it = 0; it < KC / 4; ++it) {
             const int e = tid + it * 256;
@@ -262,6 +262,9 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             if (slot >= ring_att) slot -= ring_att;
             vx[it] = (k < K && c < n) ? ring[(size_t)slot * n + c] : 0.0;
         }
+    };
+    fetch(j0);
+    for (int j = j0; j < j1; ++j) {
         __syncthreads();  // previous tile fully consumed
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
@@ -277,6 +280,7 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             sX[(e >> 6) * BSP + (e & 63)] = vx[it];
         }
         __syncthreads();
+        if (j + 1 < j1) fetch(j + 1);  // in flight during this tile's products
 #pragma unroll 4
         for (int c = 0; c < BT; c += 4) {
             const double a = sB[(16 * w + li) * BSP + c + lk];
@@ -296,56 +300,113 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
         }
 }
 
-// Candidate positions of K attempts from the current positions: thread per
-// coordinate; d_k = (sum of the slab partials in slab order) * sqrt(2 dt)
+// Candidate positions of K attempts from the current positions.  A
+// workgroup takes 8 beads; thread (bead, k) forms d_k for the bead's three
+// coordinates: the slab partials summed in slab order, times sqrt(2 dt)
 // (B xi, box.py:371), or ((Bd xi) sqrt(2 dt)) for the diagonal nohi factor
-// (box.py:369).  cand[k][r] = base + d_k, base <- cand - d_k; row K keeps the
-// chain end.  Also clears the K overlap flags.
-__global__ void cand_kernel(const double* __restrict__ pos, const double* __restrict__ P, int np, int KC, int S, int n,
-                            int K, double sq2dt, const double* __restrict__ Bd, const double* __restrict__ ring,
-                            int ring_pos, int ring_att, double* __restrict__ cand, int* __restrict__ cflag) {
+// (box.py:369).  Then per coordinate the chain cand[k] = base + d_k,
+// base <- cand[k] - d_k; row K of cand keeps the chain end.  delta[b] bounds
+// the distance of every candidate position of bead b from its current one
+// (the overlap kernel's pair filter).  Block 0 also clears the K flags.
+constexpr int CB = 8;  // beads per workgroup
+__global__ __launch_bounds__(256) void cand_kernel(const double* __restrict__ pos, const double* __restrict__ P, int np,
+                                                   int KC, int S, int N, int K, double sq2dt,
+                                                   const double* __restrict__ Bd, const double* __restrict__ ring,
+                                                   int ring_pos, int ring_att, double* __restrict__ cand,
+                                                   double* __restrict__ delta, int* __restrict__ cflag) {
     // d_k is rounded on its own before it is added, as the reference forms
     // the displacement array first (no fused multiply-add into base + d)
 #pragma clang fp contract(off)
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < K) cflag[r] = 0;
-    if (r >= n) return;
-    double base = pos[r];
-    const int ns = (r / BT) / S + 1;
-    for (int k = 0; k < K; ++k) {
-        double d;
-        if (Bd) {
-            int slot = ring_pos + k;
-            if (slot >= ring_att) slot -= ring_att;
-            d = ((Bd[r] * ring[(size_t)slot * n + r]) * sq2dt) * 1.0;
-        } else {
-            double acc = 0.0;
-            for (int s = 0; s < ns; ++s) acc += P[((size_t)s * np + r) * KC + k];
-            d = (acc * sq2dt) * 1.0;
+    __shared__ double sd[CB][3][KC_MAX + 1];
+    __shared__ double sm[CB][3];
+    const int tid = threadIdx.x;
+    const int k = tid & 31, bl = tid >> 5;
+    const int b = blockIdx.x * CB + bl;
+    const int n = 3 * N;
+    if (blockIdx.x == 0 && tid < K) cflag[tid] = 0;
+    if (b < N && k < K) {
+        int slot = ring_pos + k;
+        if (slot >= ring_att) slot -= ring_att;
+        for (int c = 0; c < 3; ++c) {
+            const int r = 3 * b + c;
+            double d;
+            if (Bd) {
+                d = ((Bd[r] * ring[(size_t)slot * n + r]) * sq2dt) * 1.0;
+            } else {
+                const int ns = (r / BT) / S + 1;
+                double acc = 0.0;
+                for (int s = 0; s < ns; ++s) acc += P[((size_t)s * np + r) * KC + k];
+                d = (acc * sq2dt) * 1.0;
+            }
+            sd[bl][c][k] = d;
         }
-        const double c = base + d;
-        cand[(size_t)k * n + r] = c;
-        base = c - d;
     }
-    cand[(size_t)K * n + r] = base;
+    __syncthreads();
+    // chain: thread (bead, coordinate) walks the K candidates
+    if (k < 3 && b < N) {
+        const int r = 3 * b + k;
+        const double p0 = pos[r];
+        double base = p0, far = 0.0;
+        for (int q = 0; q < K; ++q) {
+            const double d = sd[bl][k][q];
+            const double c = base + d;
+            sd[bl][k][q] = c;
+            far = fmax(far, fabs(c - p0));
+            base = c - d;
+        }
+        sd[bl][k][K] = base;
+        sm[bl][k] = fmax(far, fabs(base - p0));
+    }
+    __syncthreads();
+    if (k == 0 && b < N) {
+        const double m0 = sm[bl][0], m1 = sm[bl][1], m2 = sm[bl][2];
+        delta[b] = sqrt(m0 * m0 + m1 * m1 + m2 * m2) * (1.0 + 1e-12);
+    }
+    // coalesced write-out: for each candidate row, the 3 * CB contiguous
+    // coordinates of this workgroup's beads
+    const int nb = min(CB, N - blockIdx.x * CB);
+    for (int e = tid; e < (K + 1) * 3 * CB; e += 256) {
+        const int q = e / (3 * CB), o = e % (3 * CB);
+        const int bb = o / 3, c = o % 3;
+        if (bb < nb) cand[(size_t)q * n + 3 * (blockIdx.x * CB) + o] = sd[bb][c][q];
+    }
 }
 
 // Overlap flags of K candidates: lower 16 x 16 bead tiles, one pair per
-// thread, candidates in order.
-__global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __restrict__ cand, const double* __restrict__ a,
-                                                             int N, int K, double L, double thr, int* cflag) {
+// thread, candidates in order.  A pair can overlap in some candidate only if
+// its current minimum-image distance is at most S + thr + delta_i + delta_j
+// (the torus distance moves by at most the two displacements, and the
+// reference test flags only pairs within S + thr); other pairs skip the loop.
+__global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __restrict__ cand,
+                                                             const double* __restrict__ pos,
+                                                             const double* __restrict__ delta,
+                                                             const double* __restrict__ a, int N, int K, double L,
+                                                             double thr, int* cflag) {
     const int t = blockIdx.x;
     int bi = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
     while (bi * (bi + 1) / 2 > t) --bi;
     const int bj = t - bi * (bi + 1) / 2;
     const int i = bi * 16 + (threadIdx.x >> 4), j = bj * 16 + (threadIdx.x & 15);
-    const bool live = i < N && j < i;
-    const double ai = live ? a[i] : 0.0, aj = live ? a[j] : 0.0;
+    bool near = false;
+    double ai = 0.0, aj = 0.0;
+    if (i < N && j < i) {
+        ai = a[i];
+        aj = a[j];
+        double s2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            const double d = pbx_min_image(pos[3 * i + c] - pos[3 * j + c], L);
+            s2 += d * d;
+        }
+        const double reach = ai + aj + thr + delta[i] + delta[j];
+        // a relative margin far above the rounding of these few operations
+        near = !(sqrt(s2) > reach * (1.0 + 1e-9) + 1e-9 * L);
+    }
+    if (!__any(near)) return;
     const int n = 3 * N;
     for (int k = 0; k < K; ++k) {
         bool hit = false;
-        if (live) {
+        if (near) {
             const double* C = cand + (size_t)k * n;
             hit = pbx_pair_overlaps(C[3 * i] - C[3 * j], C[3 * i + 1] - C[3 * j + 1], C[3 * i + 2] - C[3 * j + 2], ai,
                                     aj, L, thr);
@@ -513,14 +574,14 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
                                K, S, P);
         PBX_HIP(hipGetLastError());
     }
-    const int nthr = n > K ? n : K;
-    hipLaunchKernelGGL(cand_kernel, dim3((nthr + 255) / 256), dim3(256), 0, s, pos, P, np, KC, S, n, K, sq2dt, Bd,
-                       ring, ring_pos, ring_att, cand, cflag);
+    double* delta = cand + (size_t)(K + 1) * n;
+    hipLaunchKernelGGL(cand_kernel, dim3((N + CB - 1) / CB), dim3(256), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
+                       ring_pos, ring_att, cand, delta, cflag);
     PBX_HIP(hipGetLastError());
     if (N > 1) {
         const int T16 = (N + 15) / 16;
-        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(T16 * (T16 + 1) / 2), dim3(256), 0, s, cand, a, N, K, L, thr,
-                           cflag);
+        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(T16 * (T16 + 1) / 2), dim3(256), 0, s, cand, pos, delta, a, N, K,
+                           L, thr, cflag);
         PBX_HIP(hipGetLastError());
     }
     return PBX_OK;
