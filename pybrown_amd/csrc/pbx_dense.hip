@@ -16,6 +16,11 @@
 //     syrk(k):   A_ij -= L_ik L_jk^T  for k < j <= i   (one workgroup per tile)
 //                and the workgroup owning tile (k+1,k+1) factors it in LDS
 // Variant 3 is the single-launch tile graph of pbx_chol_dag.hip.
+// Variant 4 (default) folds trsm(k+1) into syrk(k): the workgroups of tile
+// column k+1 keep their updated tile in registers, wait (bounded) for the
+// look-ahead workgroup to publish inv(L_{k+1,k+1}), and solve in place, so
+// each panel costs one launch; its diagonal tiles use the wave-specialised
+// factorisation (tile_factor_ws).
 // inv(L_kk) is produced with each diagonal factorisation and kept in `work`
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
 #include "pbx_common.h"
@@ -76,7 +81,9 @@ __device__ void tile_factor_columns(double* sL, double* sW, double* sd, int col0
 
 template <int V>
 __device__ __forceinline__ void factor_tile(double* sL, double* sW, double* sd, int col0, int* info) {
-    if (V == 2)
+    if (V == 5)
+        tile_factor_ws(sL, sW, sd, col0, info);
+    else if (V == 2)
         tile_factor(sL, sW, sd, col0, info);
     else
         tile_factor_columns(sL, sW, sd, col0, info);
@@ -93,11 +100,22 @@ __device__ void store_diag(const double* sL, const double* sW, double* A, int ld
     }
 }
 
+// Look-ahead state words (variant 4), in the work area after the diagonal
+// inverses: [LA_FLAG] = index of the last diagonal tile published,
+// [LA_ERR] = a dependency wait ran out of budget.  (Words 2 and 3 of the
+// single-launch graph's state block, which that variant does not use.)
+constexpr int LA_FLAG = 2, LA_ERR = 3;
+
 template <int V>
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info) {
+__global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info,
+                                                         int* la) {
     __shared__ double sL[T64 * SP];
     __shared__ double sW[T64 * SP];
     __shared__ double sd[T64];
+    if (la && threadIdx.x == 0) {
+        la[LA_FLAG] = 0;
+        la[LA_ERR] = 0;
+    }
     stage<false>(sL, A + (size_t)k * T64 * ld + (size_t)k * T64, ld);
     __syncthreads();
     factor_tile<V>(sL, sW, sd, k * T64, info);
@@ -173,6 +191,106 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(double* A, int ld, int k, 
     __syncthreads();
     factor_tile<V>(sL, sW, sd, (k + 1) * T64, info);
     store_diag(sL, sW, A, ld, k + 1, work);
+}
+
+// Wait until *flag >= target (thread 0 polls with device-scope acquire),
+// bounded; false if the budget ran out or another wait already failed.
+__device__ bool wait_ge_agent(const int* flag, int target, int* err) {
+    __shared__ int ok;
+    if (threadIdx.x == 0) {
+        int good = 1;
+        long long spins = 0;
+        while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1 << 22)) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                good = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok = good;
+    }
+    __syncthreads();
+    return ok != 0;
+}
+
+// syrk(k) with trsm(k+1) folded in (variant 4).  Workgroup 0 updates and
+// factors tile (k+1, k+1) and publishes it; workgroups of tiles (i, k+1),
+// i > k+1, wait for that and write L_{i,k+1} = C W_{k+1}^T instead of C.
+// Only T-k-2 workgroups ever wait, and only on workgroup 0, which waits on
+// nothing, so every wait ends.
+template <int V>
+__global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, int k, double* work, int* info,
+                                                           int* la) {
+    __shared__ double sA[T64 * SP];
+    __shared__ double sB[T64 * SP];
+    __shared__ double sd[T64];
+    int ii, jj;
+    tri_decode(blockIdx.x, ii, jj);
+    const int i = k + 1 + ii, j = k + 1 + jj;
+    double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
+    double cv[16];
+    acc_load(Aij, ld, cv);
+    stage<true>(sA, A + (size_t)i * T64 * ld + (size_t)k * T64, ld);
+    if (i != j) stage<true>(sB, A + (size_t)j * T64 * ld + (size_t)k * T64, ld);
+    __syncthreads();
+    v4d acc[2][2];
+    acc_zero(acc);
+    mma64(sA, i == j ? sA : sB, acc);
+    if (blockIdx.x == 0) {
+        __syncthreads();
+        double* sL = sA;
+        double* sW = sB;
+        int q = 0;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v, ++q) sL[acc_row(x, v) * SP + acc_col(y)] = cv[q] - acc[x][y][v];
+        __syncthreads();
+        factor_tile<V>(sL, sW, sd, (k + 1) * T64, info);
+        store_diag(sL, sW, A, ld, k + 1, work);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(la + LA_FLAG, k + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if (jj != 0) {
+        int q = 0;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v, ++q) {
+                    const int r = acc_row(x, v), c = acc_col(y);
+                    if (i != j || c <= r) Aij[(size_t)r * ld + c] = cv[q] - acc[x][y][v];
+                }
+        return;
+    }
+    // tile (i, k+1): op(A)(r, kk) = C[r][kk] in sA, then C W^T
+    __syncthreads();
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) sA[acc_col(y) * SP + acc_row(x, v)] = cv[q] - acc[x][y][v];
+    if (!wait_ge_agent(la + LA_FLAG, k + 1, la + LA_ERR)) return;
+    stage<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
+    __syncthreads();
+    v4d out[2][2];
+    acc_zero(out);
+    mma64(sA, sB, out);
+    acc_for_each(out, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] = v; });
+}
+
+// An exhausted look-ahead wait becomes the factorisation status word.
+__global__ void la_finish_kernel(const int* la, int* info) {
+    if (threadIdx.x == 0 && la[LA_ERR] != 0) atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
 }
 
 // ---------------------------------------------------------------------------
@@ -305,10 +423,11 @@ size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64 + (pbx_dag_state
 
 // Cholesky variant: 1 = launch per panel, column-by-column diagonal tiles;
 // 2 = launch per panel, 8-column panel diagonal tiles; 3 = single persistent
-// launch over the tile dependency graph (pbx_chol_dag.hip).
-static int g_potrf_variant = 2;
+// launch over the tile dependency graph (pbx_chol_dag.hip); 4 = one launch
+// per panel with the solve folded in and wave-specialised diagonal tiles.
+static int g_potrf_variant = 4;
 extern "C" int pbx_set_potrf_variant(int v) {
-    if (v < 1 || v > 3) return PBX_ERR_ARG;
+    if (v < 1 || v > 4) return PBX_ERR_ARG;
     g_potrf_variant = v;
     return PBX_OK;
 }
@@ -343,11 +462,23 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         TRY_RC(pbx_potrf_dag(A, np, ld, work, info_dev, state, 0, s));
         return PBX_OK;
     }
+    if (g_potrf_variant == 4) {
+        int* la = (int*)(work + (size_t)np * T64);
+        hipLaunchKernelGGL(potrf_diag_kernel<5>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, la);
+        if (T > 1) hipLaunchKernelGGL(trsm_kernel, dim3(T - 1), dim3(256), 0, s, A, ld, 0, work);
+        for (int k = 0; k + 1 < T; ++k) {
+            int cnt = (T - 1 - k) * (T - k) / 2;
+            hipLaunchKernelGGL(syrk_trsm_kernel<5>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev, la);
+        }
+        hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
+        PBX_HIP(hipGetLastError());
+        return PBX_OK;
+    }
     const bool v2 = g_potrf_variant == 2;
     if (v2)
-        hipLaunchKernelGGL(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);
+        hipLaunchKernelGGL(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
     else
-        hipLaunchKernelGGL(potrf_diag_kernel<1>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev);
+        hipLaunchKernelGGL(potrf_diag_kernel<1>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
     for (int k = 0; k + 1 < T; ++k) {
         hipLaunchKernelGGL(trsm_kernel, dim3(T - 1 - k), dim3(256), 0, s, A, ld, k, work);
         int cnt = (T - 1 - k) * (T - k) / 2;

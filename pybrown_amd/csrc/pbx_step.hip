@@ -301,15 +301,15 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
 }
 
 // Candidate positions of K attempts from the current positions.  A
-// workgroup takes 8 beads; thread (bead, k) forms d_k for the bead's three
+// workgroup takes 4 beads; thread (bead, k) forms d_k for the bead's three
 // coordinates: the slab partials summed in slab order, times sqrt(2 dt)
 // (B xi, box.py:371), or ((Bd xi) sqrt(2 dt)) for the diagonal nohi factor
 // (box.py:369).  Then per coordinate the chain cand[k] = base + d_k,
 // base <- cand[k] - d_k; row K of cand keeps the chain end.  delta[b] bounds
 // the distance of every candidate position of bead b from its current one
-// (the overlap kernel's pair filter).  Block 0 also clears the K flags.
-constexpr int CB = 8;  // beads per workgroup
-__global__ __launch_bounds__(256) void cand_kernel(const double* __restrict__ pos, const double* __restrict__ P, int np,
+// (the overlap kernel's pair filter).  Block 0 also clears the overlap mask.
+constexpr int CB = 4;  // beads per workgroup (32 threads each)
+__global__ __launch_bounds__(128) void cand_kernel(const double* __restrict__ pos, const double* __restrict__ P, int np,
                                                    int KC, int S, int N, int K, double sq2dt,
                                                    const double* __restrict__ Bd, const double* __restrict__ ring,
                                                    int ring_pos, int ring_att, double* __restrict__ cand,
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void cand_kernel(const double* __restrict__ po
     const int k = tid & 31, bl = tid >> 5;
     const int b = blockIdx.x * CB + bl;
     const int n = 3 * N;
-    if (blockIdx.x == 0 && tid < K) cflag[tid] = 0;
+    if (blockIdx.x == 0 && tid == 0) cflag[0] = 0;
     if (b < N && k < K) {
         int slot = ring_pos + k;
         if (slot >= ring_att) slot -= ring_att;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void cand_kernel(const double* __restrict__ po
     // coalesced write-out: for each candidate row, the 3 * CB contiguous
     // coordinates of this workgroup's beads
     const int nb = min(CB, N - blockIdx.x * CB);
-    for (int e = tid; e < (K + 1) * 3 * CB; e += 256) {
+    for (int e = tid; e < (K + 1) * 3 * CB; e += 128) {
         const int q = e / (3 * CB), o = e % (3 * CB);
         const int bb = o / 3, c = o % 3;
         if (bb < nb) cand[(size_t)q * n + 3 * (blockIdx.x * CB) + o] = sd[bb][c][q];
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) void cand_kernel(const double* __restrict__ po
 }
 
 // Overlap flags of K candidates: lower 16 x 16 bead tiles, one pair per
-// thread, candidates in order.  A pair can overlap in some candidate only if
+// thread; bit k of cflag[0] is set when candidate k overlaps.  A pair can overlap in some candidate only if
 // its current minimum-image distance is at most S + thr + delta_i + delta_j
 // (the torus distance moves by at most the two displacements, and the
 // reference test flags only pairs within S + thr); other pairs skip the loop.
@@ -404,6 +404,8 @@ __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __res
     }
     if (!__any(near)) return;
     const int n = 3 * N;
+    unsigned m = 0;  // bit k: some pair of this wave overlaps in candidate k
+#pragma unroll 4
     for (int k = 0; k < K; ++k) {
         bool hit = false;
         if (near) {
@@ -411,17 +413,18 @@ __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __res
             hit = pbx_pair_overlaps(C[3 * i] - C[3 * j], C[3 * i + 1] - C[3 * j + 1], C[3 * i + 2] - C[3 * j + 2], ai,
                                     aj, L, thr);
         }
-        if (__any(hit) && (threadIdx.x & 63) == 0) atomicOr(&cflag[k], 1);
+        if (__any(hit)) m |= 1u << k;
     }
+    if (m && (threadIdx.x & 63) == 0) atomicOr((unsigned*)cflag, m);
 }
 
 // Apply the first candidate without an overlap (or the chain end when all K
 // overlapped): positions, per-label net crossings, and the index in *sel.
 __global__ void select_kernel(const double* __restrict__ cand, int N, int K, const int* __restrict__ cflag, double* pos,
                               FluxPlane f, const int* __restrict__ label, long long* flux, int* sel) {
-    int acc = K;
-    for (int k = K - 1; k >= 0; --k)
-        if (!cflag[k]) acc = k;
+    const unsigned all = K >= 32 ? 0xffffffffu : ((1u << K) - 1u);
+    const unsigned clean = ~(unsigned)cflag[0] & all;
+    const int acc = clean ? __builtin_ctz(clean) : K;
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b == 0) *sel = acc;
     if (b >= N) return;
@@ -575,7 +578,7 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
         PBX_HIP(hipGetLastError());
     }
     double* delta = cand + (size_t)(K + 1) * n;
-    hipLaunchKernelGGL(cand_kernel, dim3((N + CB - 1) / CB), dim3(256), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
+    hipLaunchKernelGGL(cand_kernel, dim3((N + CB - 1) / CB), dim3(32 * CB), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
                        ring_pos, ring_att, cand, delta, cflag);
     PBX_HIP(hipGetLastError());
     if (N > 1) {

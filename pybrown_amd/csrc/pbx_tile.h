@@ -237,4 +237,206 @@ __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0,
     __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// Wave-specialised tile factorisation: same contract as tile_factor (L in sL,
+// inv(L) in sW, diagonal in sd, status in info), with the four waves working
+// concurrently on one 64 x 64 tile instead of in lock step:
+//   wave 0  the serial L chain: factors 8-column panel p (lane = row, pivots
+//           broadcast with readlane), publishes it, then applies panel p to
+//           panel p+1's columns itself (row entries read as LDS broadcasts);
+//   waves 2, 3  hold the trailing L blocks in MFMA accumulators, apply each
+//           published panel and stage the columns of panel p+2 (updated
+//           through p) one panel ahead of wave 0;
+//   wave 1  forms the rows of inv(L) a panel behind the L chain
+//           (Gauss-Jordan on [L | I]: W rows of panel p by substitution with
+//           the panel's 8 x 8 diagonal block, then R -= L_panel W_p).
+// Hand-offs are LDS counters with workgroup-scope release/acquire; every
+// wait targets a counter released by an earlier step of another wave, so the
+// waits always end (each is also bounded).  Needs exactly 256 threads.
+
+__device__ __forceinline__ void lds_wait_ge(int* f, int target) {
+    int it = 0;
+    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target && ++it < (1 << 24))
+        __builtin_amdgcn_s_sleep(1);
+}
+
+__device__ __forceinline__ void lds_release_add(int* f, int v) {
+    __hip_atomic_fetch_add(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int col0, int* info) {
+    __shared__ int fl[2];      // [0]: panels published by wave 0; [1]: staged panel inputs (2 per panel)
+    __shared__ double sy[64];  // 1 / L_jj
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    // trailing L blocks (block row X, block column Y) of waves 2 and 3; the
+    // split balances the number of panel updates each block receives
+    const int BX2[5] = {3, 2, 3, 1, 3}, BY2[5] = {3, 1, 1, 0, 0};
+    const int BX3[5] = {2, 3, 1, 0, 2}, BY3[5] = {2, 2, 1, 0, 0};
+    v4d acc[10];
+    int bx[5], by[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        bx[s] = (w == 2) ? BX2[s] : BX3[s];
+        by[s] = (w == 2) ? BY2[s] : BY3[s];
+    }
+    if (tid == 0) {
+        fl[0] = 0;
+        fl[1] = 0;
+    }
+    if (w >= 2) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[s][v] = sL[(16 * bx[s] + lk + 4 * v) * SP + 16 * by[s] + li];
+    } else if (w == 1) {
+        // R = I; lower block (X, Y) is acc[X (X + 1) / 2 + Y]
+#pragma unroll
+        for (int X = 0; X < 4; ++X)
+#pragma unroll
+            for (int Y = 0; Y <= X; ++Y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[X * (X + 1) / 2 + Y][v] = (X == Y && lk + 4 * v == li) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (w == 0) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = sL[lane * SP + q];
+        for (int p = 0; p < 8; ++p) {
+            const int c0 = 8 * p;
+            int first_bad = -1;
+            double yq[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int j = c0 + q;
+                const double d = bcast(v[q], j);
+                if (!(d > 0.0) && first_bad < 0) first_bad = q;
+                const double y = rsqrt_nr(d);
+                yq[q] = y;
+                // the whole column is scaled: entries above the pivot are dead
+                v[q] = v[q] * y;
+#pragma unroll
+                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] = fma(-v[q], bcast(v[q], c0 + q2), v[q2]);
+            }
+            if (lane >= c0) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (lane == c0 + q) sd[c0 + q] = v[q];
+                if (lane == q) sy[c0 + q] = yq[q];
+            }
+            if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
+            if (lane == 0) lds_release_add(&fl[0], 1);
+            if (p == 7) break;
+            // panel p+1's columns (updated through p-1 by waves 2 and 3) less
+            // panel p's contribution
+            if (p > 0) lds_wait_ge(&fl[1], 2 * p);
+            const int n0 = c0 + 8;
+            double nv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) nv[q] = (lane >= n0) ? sL[lane * SP + n0 + q] : 0.0;
+#pragma unroll
+            for (int q2 = 0; q2 < 8; ++q2) {
+                const double* Lr = sL + (n0 + q2) * SP + c0;
+                double t = nv[q2];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) t = fma(-v[q], Lr[q], t);
+                nv[q2] = t;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) v[q] = nv[q];
+        }
+    } else if (w >= 2) {
+        for (int p = 0; p < 7; ++p) {
+            const int c0 = 8 * p, lo = c0 + 8;
+            lds_wait_ge(&fl[0], p + 1);
+            const int n2 = c0 + 16;  // first column of panel p+2
+            // pass 0: blocks holding panel p+2's columns, which are staged;
+            // pass 1: the rest
+#pragma unroll
+            for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+                for (int s = 0; s < 5; ++s) {
+                    const int X = bx[s], Y = by[s];
+                    const bool live = 16 * Y + 15 >= lo;
+                    const bool early = (n2 < 64) && (16 * Y <= n2) && (n2 < 16 * Y + 16);
+                    if (!live || early != (pass == 0)) continue;
+#pragma unroll
+                    for (int ks = 0; ks < 8; ks += 4) {
+                        const int k = c0 + ks + lk;
+                        const int ra = 16 * X + li;
+                        const double a0 = (ra >= lo) ? -sL[ra * SP + k] : 0.0;
+                        const double b0 = sL[(16 * Y + li) * SP + k];
+                        acc[s] = mfma(a0, b0, acc[s]);
+                    }
+                    if (early) {
+                        const int c = 16 * Y + li;
+                        if (c >= n2 && c < n2 + 8)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) {
+                                const int r = 16 * X + lk + 4 * v;
+                                if (r >= n2) sL[r * SP + c] = acc[s][v];
+                            }
+                    }
+                }
+                if (pass == 0 && lane == 0) lds_release_add(&fl[1], 1);
+            }
+        }
+    } else {
+        for (int p = 0; p < 8; ++p) {
+            const int c0 = 8 * p, lo = c0 + 8;
+            lds_wait_ge(&fl[0], p + 1);
+            // R rows c0..c0+7 (columns <= c0+7 lie in block row X = c0 / 16)
+            const int X = c0 >> 4;
+#pragma unroll
+            for (int XX = 0; XX < 4; ++XX) {
+                if (XX != X) continue;
+#pragma unroll
+                for (int Y = 0; Y <= XX; ++Y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 16 * XX + lk + 4 * v, c = 16 * Y + li;
+                        if (r >= c0 && r < c0 + 8) sW[r * SP + c] = acc[XX * (XX + 1) / 2 + Y][v];
+                    }
+            }
+            __builtin_amdgcn_wave_barrier();
+            // W rows: lane = column, substitution with the panel's diagonal block
+            double wv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) wv[q] = (lane < c0 + 8) ? sW[(c0 + q) * SP + lane] : 0.0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double* Lr = sL + (c0 + q) * SP + c0;
+                double t = wv[q];
+#pragma unroll
+                for (int q2 = 0; q2 < q; ++q2) t = fma(-Lr[q2], wv[q2], t);
+                wv[q] = t * sy[c0 + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sW[(c0 + q) * SP + lane] = wv[q];
+            __builtin_amdgcn_wave_barrier();
+            if (p == 7) break;
+            // R[rows >= lo] -= L[rows, panel] W_p over blocks with columns <= c0+7
+#pragma unroll
+            for (int XB = 0; XB < 4; ++XB)
+#pragma unroll
+                for (int YB = 0; YB <= XB; ++YB) {
+                    if (16 * XB + 15 < lo || 16 * YB > c0 + 7) continue;
+#pragma unroll
+                    for (int ks = 0; ks < 8; ks += 4) {
+                        const int k = c0 + ks + lk;
+                        const int ra = 16 * XB + li;
+                        const double a0 = (ra >= lo) ? -sL[ra * SP + k] : 0.0;
+                        const double b0 = sW[k * SP + 16 * YB + li];
+                        acc[XB * (XB + 1) / 2 + YB] = mfma(a0, b0, acc[XB * (XB + 1) / 2 + YB]);
+                    }
+                }
+        }
+    }
+    __syncthreads();
+}
+
 }  // namespace pbx_tile

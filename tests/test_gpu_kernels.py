@@ -137,12 +137,12 @@ def test_lubrication_accumulates(orc):
     np.testing.assert_allclose(C.reshape(9, 9), Co, rtol=1e-12, atol=1e-12 * np.abs(Co).max())
 
 
-@pytest.fixture(params=[1, 2, 3], ids=["potrf_v1", "potrf_v2", "potrf_dag"])
+@pytest.fixture(params=[1, 2, 3, 4], ids=["potrf_v1", "potrf_v2", "potrf_dag", "potrf_v4"])
 def potrf_variant(request):
     from pybrown_amd._lib import lib
     assert lib.pbx_set_potrf_variant(request.param) == 0
     yield request.param
-    lib.pbx_set_potrf_variant(2)
+    lib.pbx_set_potrf_variant(4)
 
 
 @pytest.mark.parametrize("n", [64, 100, 300, 1000, 3000])
