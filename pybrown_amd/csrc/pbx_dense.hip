@@ -417,6 +417,22 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const dou
 
 size_t pbx_dag_state_ints(int np);
 int pbx_potrf_dag(double* A, int np, int ld, double* work, int* info_dev, int* state, int grid, hipStream_t s);
+bool pbx_persist_fits(int np, int cus);
+int pbx_potrf_persist(double* A, int np, int ld, double* work, int* info_dev, int* state, int cus, hipStream_t s);
+
+// compute units of the current device (queried once)
+static int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 1;
+    }
+    return cus;
+}
 
 // work = [T inverse tiles of the diagonal factor][single-launch DAG state]
 size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64 + (pbx_dag_state_ints(np) + 1) / 2; }
@@ -424,10 +440,12 @@ size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64 + (pbx_dag_state
 // Cholesky variant: 1 = launch per panel, column-by-column diagonal tiles;
 // 2 = launch per panel, 8-column panel diagonal tiles; 3 = single persistent
 // launch over the tile dependency graph (pbx_chol_dag.hip); 4 = one launch
-// per panel with the solve folded in and wave-specialised diagonal tiles.
+// per panel with the solve folded in and wave-specialised diagonal tiles;
+// 5 = persistent launch with tiles owned in registers (pbx_chol_persist.hip;
+// falls back to 4 when the tiles do not fit the compute units).
 static int g_potrf_variant = 4;
 extern "C" int pbx_set_potrf_variant(int v) {
-    if (v < 1 || v > 4) return PBX_ERR_ARG;
+    if (v < 1 || v > 5) return PBX_ERR_ARG;
     g_potrf_variant = v;
     return PBX_OK;
 }
@@ -462,7 +480,12 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         TRY_RC(pbx_potrf_dag(A, np, ld, work, info_dev, state, 0, s));
         return PBX_OK;
     }
-    if (g_potrf_variant == 4) {
+    if (g_potrf_variant == 5 && pbx_persist_fits(np, device_cus())) {
+        int* state = (int*)(work + (size_t)np * T64);
+        TRY_RC(pbx_potrf_persist(A, np, ld, work, info_dev, state, device_cus(), s));
+        return PBX_OK;
+    }
+    if (g_potrf_variant >= 4) {
         int* la = (int*)(work + (size_t)np * T64);
         hipLaunchKernelGGL(potrf_diag_kernel<5>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, la);
         if (T > 1) hipLaunchKernelGGL(trsm_kernel, dim3(T - 1), dim3(256), 0, s, A, ld, 0, work);

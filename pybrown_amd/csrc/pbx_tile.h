@@ -254,10 +254,17 @@ __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0,
 // wait targets a counter released by an earlier step of another wave, so the
 // waits always end (each is also bounded).  Needs exactly 256 threads.
 
-__device__ __forceinline__ void lds_wait_ge(int* f, int target) {
+// Bounded wait on an LDS counter; an exhausted budget is reported through
+// the status word (the factorisation is then not valid).
+__device__ __forceinline__ void lds_wait_ge(int* f, int target, int* info) {
     int it = 0;
-    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target && ++it < (1 << 24))
+    while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < target) {
+        if (++it >= (1 << 24)) {
+            atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
+            return;
+        }
         __builtin_amdgcn_s_sleep(1);
+    }
 }
 
 __device__ __forceinline__ void lds_release_add(int* f, int v) {
@@ -333,7 +340,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
             if (p == 7) break;
             // panel p+1's columns (updated through p-1 by waves 2 and 3) less
             // panel p's contribution
-            if (p > 0) lds_wait_ge(&fl[1], 2 * p);
+            if (p > 0) lds_wait_ge(&fl[1], 2 * p, info);
             const int n0 = c0 + 8;
             double nv[8];
 #pragma unroll
@@ -352,7 +359,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
     } else if (w >= 2) {
         for (int p = 0; p < 7; ++p) {
             const int c0 = 8 * p, lo = c0 + 8;
-            lds_wait_ge(&fl[0], p + 1);
+            lds_wait_ge(&fl[0], p + 1, info);
             const int n2 = c0 + 16;  // first column of panel p+2
             // pass 0: blocks holding panel p+2's columns, which are staged;
             // pass 1: the rest
@@ -388,7 +395,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
     } else {
         for (int p = 0; p < 8; ++p) {
             const int c0 = 8 * p, lo = c0 + 8;
-            lds_wait_ge(&fl[0], p + 1);
+            lds_wait_ge(&fl[0], p + 1, info);
             // R rows c0..c0+7 (columns <= c0+7 lie in block row X = c0 / 16)
             const int X = c0 >> 4;
 #pragma unroll
