@@ -61,7 +61,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1, 2, 3; 0 = library default)")
+    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-5; 0 = library default)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -78,6 +80,16 @@ def main():
     from pybrown_amd.workloads import workload
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": nbytes, "copies": 3}))
+        return
     from pybrown_amd._lib import lib
     if args.potrf:
         assert lib.pbx_set_potrf_variant(args.potrf) == 0
@@ -151,6 +163,19 @@ def smith_flops(N, m=2, nrec=2):
     return pair * N * (N - 1) / 2 + self_ * N
 
 
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of `kernel_key` from the committed counter summary
+    (profiles/hbm_traffic.json, written from rocprofv3 FETCH_SIZE and
+    WRITE_SIZE runs scaled by a plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(kernel_key)
+    except (OSError, ValueError):
+        return None
+    return None if entry is None else entry.get("hbm_bytes")
+
+
 def roofline(name, N, stage):
     """Roofline entry for the dominant stage of the step (by measured time)."""
     n = 3 * N
@@ -159,7 +184,8 @@ def roofline(name, N, stage):
         flops = n ** 3 / 3.0
         ach = flops / (stage[1] * 1e-3) / 1e12
         return {"kernel": "cholesky (potrf, n=%d)" % n, "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic("cholesky_%s_n%d" % (name, n)),
                 "algorithmic": "n^3/3 = %.3e flop per factorisation" % flops}
     if k == 0:
         flops = smith_flops(N) if name in ("c2", "c4", "c3") else 30.0 * N * (N - 1) / 2
