@@ -46,6 +46,10 @@ void JO_R_lubrication_correction_matrix(double* as, double* pointers, int number
 /* clib/bead.c:27 -- minimum-image pointer matrix, N x N x 3; diagonal entries
  * keep the caller's values. */
 void pointer_pbc_matrix(double* positions, int number_of_beads, double box_length, double* pointers);
+/* bead.c:99 -- pointers[i][j] = wrap(r_immobile_j - r_mobile_i), shape
+ * (number_of_mobile, number_of_immobile, 3); every entry written. */
+void pointer_immobile_pbc_matrix(double* positions_mobile, double* positions_immobile, int number_of_mobile,
+                                 int number_of_immobile, double box_length, double* pointers);
 
 /* clib/bead.c:169 -- 1 if any non-bonded pair overlaps, else 0 (-1 on a HIP
  * error).  connection_matrix is N x N int32 (may be NULL = no bonds). */
@@ -127,6 +131,15 @@ int pbx_sim_set_seed(pbx_sim* sim, unsigned int seed);
 /* Move attempts evaluated per batch in the Ermak scheme with overlap checks
  * (1..32; 0 = adaptive, the default).  Trajectories do not depend on it. */
 int pbx_sim_set_attempt_batch(pbx_sim* sim, int K);
+/* Checkpoint: the device state (positions, flux, D, B[, Rff], diagonal
+ * inverses, counters) and the noise state (unconsumed draws; the native
+ * generator's state) as one byte buffer; restoring it into a simulation
+ * created with the same configuration resumes bit for bit (pickling a Box,
+ * test_box.py:53-117).  With a noise callback the caller restores its own
+ * generator and registers the callback before pbx_sim_restore. */
+long long pbx_sim_snapshot_size(pbx_sim* sim);
+int pbx_sim_snapshot(pbx_sim* sim, void* buf, long long bytes);
+int pbx_sim_restore(pbx_sim* sim, const void* buf, long long bytes);
 /* count values of RandomState(seed).normal(0, 1) after skipping `skip`. */
 int pbx_rng_normals(unsigned int seed, long long skip, double* out, long long count);
 

@@ -13,6 +13,7 @@
  *
  * Pinning (what checks each function, see tests/test_oracle_*.py):
  *   orc_pointer_pbc_matrix      bit-exact vs the compiled reference (oracle/_ref)
+ *   orc_pointer_immobile_...    bit-exact vs the compiled reference (oracle/_ref)
  *   orc_check_overlaps          exact vs oracle/_ref on random configurations
  *   orc_rpy_m_matrix            bit-exact vs oracle/_ref; pecnut golden values
  *   orc_rpy_smith_m_matrix      bit-exact vs oracle/_ref; BD_BOX golden ratios
@@ -103,6 +104,16 @@ void orc_pointer_pbc_matrix(const double *xyz, int N, double L, double *P3)
 				lo[c] = -d[c];
 			}
 		}
+}
+
+/* clib/bead.c:99-158 pointer_immobile_pbc_matrix: P3[i][j] = wrap(r_immobile_j -
+ * r_mobile_i), layout (n_mobile, n_immobile, 3), same wrap loops. */
+void orc_pointer_immobile_pbc_matrix(const double *xm, const double *xi, int nm, int ni, double L, double *P3)
+{
+	for (int j = 0; j < ni; ++j)
+		for (int i = 0; i < nm; ++i)
+			for (int c = 0; c < 3; ++c)
+				P3[3 * j + 3 * (size_t)i * ni + c] = min_image(xi[3 * j + c] - xm[3 * i + c], L);
 }
 
 /* pyBrown/diffusion.py:40 -- the packed pair list handed to every mobility

@@ -59,6 +59,7 @@ class Oracle:
             p = "orc_"
             self._names = dict(
                 pointer=p + "pointer_pbc_matrix", overlaps=p + "check_overlaps",
+                immobile=p + "pointer_immobile_pbc_matrix",
                 rpy=p + "rpy_m_matrix", smith=p + "rpy_smith_m_matrix",
                 jo2b=p + "jo_2b_r_matrix", lub=p + "jo_r_lubrication_correction_matrix")
         elif lib == "ref":
@@ -67,13 +68,16 @@ class Oracle:
             self.lib = _lazy_open(REF_PATH)
             self._names = dict(
                 pointer="pointer_pbc_matrix", overlaps="check_overlaps",
-                rpy="RPY_M_matrix", smith="RPY_Smith_M_matrix",
+                immobile="pointer_immobile_pbc_matrix", rpy="RPY_M_matrix", smith="RPY_Smith_M_matrix",
                 jo2b="JO_2B_R_matrix", lub=None)
         else:
             raise ValueError(lib)
         L = self.lib
         f = getattr(L, self._names["pointer"])
         f.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp]
+        f.restype = None
+        f = getattr(L, self._names["immobile"])
+        f.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
         f.restype = None
         f = getattr(L, self._names["overlaps"])
         f.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip]
@@ -114,6 +118,15 @@ class Oracle:
         out = np.zeros(3 * N * N)
         getattr(self.lib, self._names["pointer"])(_d(xyz), N, float(L), _d(out))
         return out.reshape(N, N, 3)
+
+    def pointer_immobile_pbc_matrix(self, xyz_mobile, xyz_immobile, L):
+        """bead.c:99-158; returns (N_mob, N_immob, 3) with rik[i][j] = wrap(r_j - r_i)."""
+        xm = np.ascontiguousarray(xyz_mobile, dtype=np.float64).reshape(-1)
+        xi = np.ascontiguousarray(xyz_immobile, dtype=np.float64).reshape(-1)
+        nm, ni = xm.size // 3, xi.size // 3
+        out = np.zeros(3 * nm * ni)
+        getattr(self.lib, self._names["immobile"])(_d(xm), _d(xi), nm, ni, float(L), _d(out))
+        return out.reshape(nm, ni, 3)
 
     @staticmethod
     def packed(rij):

@@ -60,6 +60,7 @@ _sig("RPY_Smith_M_matrix", None, [_dp, _dp, _d, _d, _i, _i, _i, _dp])
 _sig("JO_2B_R_matrix", None, [_d, _d, _d, _d, _d, _dp])
 _sig("JO_R_lubrication_correction_matrix", None, [_dp, _dp, _i, _d, _i, _dp])
 _sig("pointer_pbc_matrix", None, [_dp, _i, _d, _dp])
+_sig("pointer_immobile_pbc_matrix", None, [_dp, _dp, _i, _i, _d, _dp])
 _sig("check_overlaps", _i, [_dp, _dp, _i, _d, _d, _ip])
 # dense
 _sig("pbx_cholesky", _i, [_dp, _i, _dp])
@@ -72,6 +73,9 @@ _sig("pbx_sim_destroy", _i, [ctypes.c_void_p])
 _sig("pbx_sim_set_noise_source", _i, [ctypes.c_void_p, XI_FILL, ctypes.c_void_p])
 _sig("pbx_sim_set_seed", _i, [ctypes.c_void_p, ctypes.c_uint])
 _sig("pbx_sim_set_attempt_batch", _i, [ctypes.c_void_p, _i])
+_sig("pbx_sim_snapshot_size", ctypes.c_longlong, [ctypes.c_void_p])
+_sig("pbx_sim_snapshot", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
+_sig("pbx_sim_restore", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
 _sig("pbx_rng_normals", _i, [ctypes.c_uint, ctypes.c_longlong, _dp, ctypes.c_longlong])
 _sig("pbx_sim_propagate", _i, [ctypes.c_void_p, _d, _i, _i, _i])
 _sig("pbx_sim_propagate_out", _i, [ctypes.c_void_p, _d, _i, _i, _i, _dp, _lp])
@@ -93,10 +97,11 @@ _sig("pbx_version", ctypes.c_char_p, [])
 
 EXPORTED = [
     "RPY_M_matrix", "RPY_Smith_M_matrix", "JO_2B_R_matrix", "JO_R_lubrication_correction_matrix",
-    "pointer_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_spd_inverse_host", "pbx_gemm_nt",
+    "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_spd_inverse_host", "pbx_gemm_nt",
     "pbx_set_potrf_variant",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
-    "pbx_sim_set_attempt_batch", "pbx_rng_normals", "pbx_sim_propagate",
+    "pbx_sim_set_attempt_batch", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
+    "pbx_sim_restore", "pbx_sim_propagate",
     "pbx_sim_propagate_out",
     "pbx_sim_get_positions", "pbx_sim_set_positions", "pbx_sim_get_flux", "pbx_sim_get_matrix",
     "pbx_sim_get_diag", "pbx_sim_draw_count", "pbx_sim_last_attempts", "pbx_sim_sync",

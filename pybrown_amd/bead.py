@@ -89,6 +89,17 @@ def compute_pointer_pbc_matrix(beads, box_length):
     return P.reshape(N, N, 3)
 
 
+def compute_pointer_immobile_pbc_matrix(mobile_beads, immobile_beads, box_length):
+    """bead.py:307-335: (N_mob, N_immob, 3) with rik[i][j] = wrap(r_j - r_i)."""
+    nm, ni = len(mobile_beads), len(immobile_beads)
+    xm, xi = _xyz(mobile_beads), _xyz(immobile_beads)
+    P = np.zeros(3 * nm * ni)
+    if nm and ni:
+        lib.pointer_immobile_pbc_matrix(ptr(xm), ptr(xi), nm, ni, float(box_length), ptr(P))
+        _lib.check_void("pointer_immobile_pbc_matrix")
+    return P.reshape(nm, ni, 3)
+
+
 def build_connection_matrix(beads):
     """bead.py:401-413 (bonds and immobile pairs are never tested)."""
     N = len(beads)

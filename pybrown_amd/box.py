@@ -67,6 +67,7 @@ class Box:
         if inp["propagation_scheme"] not in _lib.SCHEME_CODES:
             raise ValueError("unknown propagation_scheme %r" % inp["propagation_scheme"])
 
+        self._init_args = (xi_pool_attempts, noise, attempt_batch)
         self.beads = beads
         self.box_length = inp["box_length"]
         # box.py:621-629
@@ -235,6 +236,34 @@ class Box:
     @property
     def Rff(self):
         return self._matrix(2)
+
+    # -- checkpoint / restart (test_box.py:53-117 pickles a Box mid-run) ----
+    def snapshot(self):
+        """Device and noise state as bytes (pbx_sim_snapshot)."""
+        import ctypes
+        size = int(lib.pbx_sim_snapshot_size(self._sim))
+        buf = ctypes.create_string_buffer(size)
+        _lib.check_rc(lib.pbx_sim_snapshot(self._sim, buf, size), "snapshot")
+        return buf.raw
+
+    def restore(self, data):
+        """Resume from snapshot() of a Box with the same beads and keywords."""
+        _lib.check_rc(lib.pbx_sim_restore(self._sim, data, len(data)), "restore")
+        _lib.check_rc(lib.pbx_sim_get_positions(self._sim, ptr(self._pos.reshape(-1))), "get_positions")
+        self._synced[:] = self._pos
+
+    def __getstate__(self):
+        return {"beads": self.beads, "input_data": dict(self.inp, seed=self.seed), "init": self._init_args,
+                "net_flux": getattr(self, "net_flux", None), "rng": self.pseudorandom_number_generator,
+                "snapshot": self.snapshot()}
+
+    def __setstate__(self, st):
+        xi, noise, batch = st["init"]
+        self.__init__(st["beads"], st["input_data"], xi_pool_attempts=xi, noise=noise, attempt_batch=batch)
+        self.pseudorandom_number_generator = st["rng"]
+        self.restore(st["snapshot"])
+        if st["net_flux"] is not None:
+            self.net_flux = st["net_flux"]
 
     def close(self):
         if getattr(self, "_sim", None):

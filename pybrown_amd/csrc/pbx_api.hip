@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <condition_variable>
 #include <mutex>
 #include <thread>
@@ -194,6 +195,25 @@ extern "C" void pointer_pbc_matrix(double* positions, int number_of_beads, doubl
     report(run(), "pointer_pbc_matrix");
 }
 
+// bead.c:99 pointer_immobile_pbc_matrix
+extern "C" void pointer_immobile_pbc_matrix(double* positions_mobile, double* positions_immobile, int number_of_mobile,
+                                            int number_of_immobile, double box_length, double* pointers) {
+    pbx_clear_error();
+    auto run = [&]() -> int {
+        int nm = number_of_mobile, ni = number_of_immobile;
+        if (nm <= 0 || ni <= 0) return PBX_OK;
+        size_t bytes = sizeof(double) * 3 * (size_t)nm * ni;
+        DevBuf dm, di, dP;
+        TRY(up(dm, positions_mobile, sizeof(double) * 3 * nm));
+        TRY(up(di, positions_immobile, sizeof(double) * 3 * ni));
+        TRY(dP.alloc(bytes));
+        TRY(pbx_launch_immobile_pointers(dm.as<double>(), di.as<double>(), nm, ni, box_length, dP.as<double>(), 0));
+        PBX_HIP(hipMemcpy(pointers, dP.p, bytes, hipMemcpyDeviceToHost));
+        return PBX_OK;
+    };
+    report(run(), "pointer_immobile_pbc_matrix");
+}
+
 // bead.c:169 check_overlaps; returns 1 on overlap, 0 otherwise (-1 on error)
 extern "C" int check_overlaps(double* positions, double* as, int number_of_beads, double box_length,
                               double overlap_treshold, int* connection_matrix) {
@@ -323,6 +343,9 @@ struct PbxSim {
     // native source: producer thread filling host chunks ahead of use
     int native = 0;
     PbxMT mt;
+    PbxMT mt_after[4];        // generator state after chunk q, in slot q % 4
+    PbxMT mt_start;           // state at chunk mt_base (seed or restore point)
+    long long mt_base = 0;
     std::thread producer;
     std::mutex mu;
     std::condition_variable cv;
@@ -467,6 +490,7 @@ void producer_main(PbxSim* s) {
         pbx_mt_normals(&s->mt, s->xi_host + (q % 3) * chunk_doubles(s), (long long)chunk_doubles(s));
         {
             std::lock_guard<std::mutex> lk(s->mu);
+            s->mt_after[q % 4] = s->mt;
             s->xi_produced = q + 1;
         }
         s->cv.notify_all();
@@ -863,6 +887,8 @@ extern "C" int pbx_sim_set_seed(pbx_sim* s, unsigned int seed) {
     pbx_clear_error();
     TRY(noise_reset(s));
     pbx_mt_seed(&s->mt, seed);
+    s->mt_start = s->mt;
+    s->mt_base = 0;
     s->native = 1;
     try {
         s->producer = std::thread(producer_main, s);
@@ -870,6 +896,187 @@ extern "C" int pbx_sim_set_seed(pbx_sim* s, unsigned int seed) {
         s->native = 0;
         pbx_set_error(PBX_ERR_STATE, "could not start the noise producer thread");
         return PBX_ERR_STATE;
+    }
+    return PBX_OK;
+}
+
+// ---- snapshot / restore ------------------------------------------------------
+// Layout: header | positions 3N | flux n_labels | unconsumed draws |
+// [generator state] | D, B [, Rff] (np x ld each) | diagonal inverses.
+
+namespace {
+struct SnapHeader {
+    char magic[8];
+    int N, np, hydro, scheme, n_labels, native;
+    int have_D, have_B, have_Rff, last_attempts;
+    long long draw_count, xi_used, xi_taken, xi_cap;
+    long long pending, nmat, wdoubles;
+    double att_mean;
+};
+const char SNAP_MAGIC[8] = {'P', 'B', 'X', 'S', 'N', 'A', 'P', '1'};
+
+int snap_nmat(const PbxSim* s) {
+    if (s->cfg.hydrodynamics == PBX_NOHI) return 0;
+    return is_lub(s) ? 3 : 2;
+}
+size_t snap_bytes(const PbxSim* s, long long pending) {
+    const int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    const size_t mat = sizeof(double) * (size_t)s->np * s->ld;
+    const size_t w = s->Wchol ? sizeof(double) * pbx_dense_work_doubles(s->np) : 0;
+    return sizeof(SnapHeader) + sizeof(double) * s->n + sizeof(long long) * nl + sizeof(double) * pending * s->n +
+           (s->native ? sizeof(PbxMT) : 0) + mat * snap_nmat(s) + w;
+}
+long long pending_attempts(const PbxSim* s) { return s->xi_taken * s->xi_cap - s->xi_used; }
+
+// copy attempts [a0, a0 + k) of the device ring to/from host memory
+int ring_copy(PbxSim* s, long long a0, long long k, double* host, bool to_host) {
+    for (long long a = a0; a < a0 + k; ++a) {
+        double* dev = s->xi_dev + (size_t)(a % (2 * s->xi_cap)) * s->n;
+        double* h = host + (size_t)(a - a0) * s->n;
+        if (to_host)
+            PBX_HIP(hipMemcpy(h, dev, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+        else
+            PBX_HIP(hipMemcpy(dev, h, sizeof(double) * s->n, hipMemcpyHostToDevice));
+    }
+    return PBX_OK;
+}
+}  // namespace
+
+extern "C" long long pbx_sim_snapshot_size(pbx_sim* s) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    return (long long)snap_bytes(s, pending_attempts(s));
+}
+
+extern "C" int pbx_sim_snapshot(pbx_sim* s, void* buf, long long bytes) {
+    pbx_clear_error();
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    SnapHeader h;
+    memset(&h, 0, sizeof(h));
+    memcpy(h.magic, SNAP_MAGIC, 8);
+    PbxMT mt_now;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        h.xi_taken = s->xi_taken;
+        if (s->native) mt_now = (s->xi_taken == s->mt_base) ? s->mt_start : s->mt_after[(s->xi_taken - 1) % 4];
+    }
+    h.N = s->N;
+    h.np = s->np;
+    h.hydro = s->cfg.hydrodynamics;
+    h.scheme = s->cfg.scheme;
+    h.n_labels = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    h.native = s->native;
+    h.have_D = s->have_D;
+    h.have_B = s->have_B;
+    h.have_Rff = s->have_Rff;
+    h.last_attempts = s->last_attempts;
+    h.draw_count = s->draw_count;
+    h.xi_used = s->xi_used;
+    h.xi_cap = s->xi_cap;
+    h.pending = pending_attempts(s);
+    h.nmat = snap_nmat(s);
+    h.wdoubles = s->Wchol ? (long long)pbx_dense_work_doubles(s->np) : 0;
+    h.att_mean = s->att_mean;
+    if (!buf || bytes < (long long)snap_bytes(s, h.pending)) {
+        pbx_set_error(PBX_ERR_ARG, "snapshot buffer too small (%lld < %zu bytes)", bytes, snap_bytes(s, h.pending));
+        return PBX_ERR_ARG;
+    }
+    char* p = (char*)buf;
+    memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    PBX_HIP(hipMemcpy(p, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    p += sizeof(double) * s->n;
+    PBX_HIP(hipMemcpy(p, s->flux, sizeof(long long) * h.n_labels, hipMemcpyDeviceToHost));
+    p += sizeof(long long) * h.n_labels;
+    TRY(ring_copy(s, s->xi_used, h.pending, (double*)p, true));
+    p += sizeof(double) * h.pending * s->n;
+    if (s->native) {
+        memcpy(p, &mt_now, sizeof(PbxMT));
+        p += sizeof(PbxMT);
+    }
+    double* mats[3] = {s->D, s->B, s->Rff};
+    const size_t mat = sizeof(double) * (size_t)s->np * s->ld;
+    for (int m = 0; m < h.nmat; ++m) {
+        PBX_HIP(hipMemcpy(p, mats[m], mat, hipMemcpyDeviceToHost));
+        p += mat;
+    }
+    if (h.wdoubles) PBX_HIP(hipMemcpy(p, s->Wchol, sizeof(double) * h.wdoubles, hipMemcpyDeviceToHost));
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
+    pbx_clear_error();
+    SnapHeader h;
+    if (!buf || bytes < (long long)sizeof(h)) {
+        pbx_set_error(PBX_ERR_ARG, "snapshot too short");
+        return PBX_ERR_ARG;
+    }
+    memcpy(&h, buf, sizeof(h));
+    const int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    if (memcmp(h.magic, SNAP_MAGIC, 8) || h.N != s->N || h.np != s->np || h.hydro != s->cfg.hydrodynamics ||
+        h.scheme != s->cfg.scheme || h.n_labels != nl || h.xi_cap != s->xi_cap || h.nmat != snap_nmat(s) ||
+        h.pending < 0 || h.pending > 2 * h.xi_cap) {
+        pbx_set_error(PBX_ERR_ARG, "snapshot does not match this simulation (size, mode or noise chunking)");
+        return PBX_ERR_ARG;
+    }
+    if (!h.native && !s->xi_fill) {
+        pbx_set_error(PBX_ERR_STATE, "snapshot uses a noise callback: register one before restoring");
+        return PBX_ERR_STATE;
+    }
+    {
+        const int was = s->native;
+        s->native = h.native;  // the expected size depends on it
+        const bool ok = bytes >= (long long)snap_bytes(s, h.pending);
+        s->native = was;
+        if (!ok) {
+            pbx_set_error(PBX_ERR_ARG, "snapshot too short");
+            return PBX_ERR_ARG;
+        }
+    }
+    // stop the producer, keep a registered callback
+    pbx_xi_fill_fn fill = s->xi_fill;
+    void* user = s->xi_user;
+    TRY(noise_reset(s));
+    s->xi_fill = fill;
+    s->xi_user = user;
+    s->native = h.native;
+    const char* p = (const char*)buf + sizeof(h);
+    PBX_HIP(hipMemcpy(s->pos, p, sizeof(double) * s->n, hipMemcpyHostToDevice));
+    p += sizeof(double) * s->n;
+    PBX_HIP(hipMemcpy(s->flux, p, sizeof(long long) * nl, hipMemcpyHostToDevice));
+    p += sizeof(long long) * nl;
+    s->xi_used = h.xi_used;
+    s->xi_taken = h.xi_taken;
+    s->xi_produced = h.xi_taken;
+    TRY(ring_copy(s, h.xi_used, h.pending, (double*)p, false));
+    p += sizeof(double) * h.pending * s->n;
+    if (h.native) {
+        memcpy(&s->mt, p, sizeof(PbxMT));
+        p += sizeof(PbxMT);
+        s->mt_start = s->mt;
+        s->mt_base = h.xi_taken;
+    }
+    double* mats[3] = {s->D, s->B, s->Rff};
+    const size_t mat = sizeof(double) * (size_t)s->np * s->ld;
+    for (int m = 0; m < h.nmat; ++m) {
+        PBX_HIP(hipMemcpy(mats[m], p, mat, hipMemcpyHostToDevice));
+        p += mat;
+    }
+    if (h.wdoubles && s->Wchol)
+        PBX_HIP(hipMemcpy(s->Wchol, p, sizeof(double) * std::min<long long>(h.wdoubles, pbx_dense_work_doubles(s->np)),
+                          hipMemcpyHostToDevice));
+    s->have_D = h.have_D;
+    s->have_B = h.have_B;
+    s->have_Rff = h.have_Rff;
+    s->last_attempts = h.last_attempts;
+    s->draw_count = h.draw_count;
+    s->att_mean = h.att_mean;
+    if (h.native) {
+        try {
+            s->producer = std::thread(producer_main, s);
+        } catch (...) {
+            pbx_set_error(PBX_ERR_STATE, "could not start the noise producer thread");
+            return PBX_ERR_STATE;
+        }
     }
     return PBX_OK;
 }

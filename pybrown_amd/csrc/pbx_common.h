@@ -94,6 +94,8 @@ int pbx_launch_rpy_packed(const double* a, const double* P, int N, double* M, in
 int pbx_launch_smith_packed(const double* a, const double* P, int N, double L, double alpha, int m_max, int n_max,
                             double* M, int ld, hipStream_t s);
 int pbx_launch_pointers(const double* xyz, int N, double L, double* P3, hipStream_t s);
+int pbx_launch_immobile_pointers(const double* xm, const double* xi, int nm, int ni, double L, double* P3,
+                                 hipStream_t s);
 int pbx_launch_overlaps(const double* xyz, const double* a, int N, double L, double thr, const int* mask,
                         int* flag, hipStream_t s);
 

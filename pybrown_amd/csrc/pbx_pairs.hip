@@ -391,6 +391,16 @@ __global__ void pointers_kernel(const double* __restrict__ xyz, int N, double L,
     }
 }
 
+// clib/bead.c:99-158: P3[i][j] = wrap(r_immobile_j - r_mobile_i), layout
+// (n_mobile, n_immobile, 3); thread per (i, j), j fastest (coalesced writes).
+__global__ void immobile_pointers_kernel(const double* __restrict__ xm, const double* __restrict__ xi, int nm, int ni,
+                                         double L, double* __restrict__ P3) {
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nm * ni) return;
+    const int i = (int)(t / ni), j = (int)(t % ni);
+    for (int c = 0; c < 3; ++c) P3[3 * t + c] = pbx_min_image(xi[3 * j + c] - xm[3 * i + c], L);
+}
+
 // clib/bead.c:169-260: overlap iff no raw axis difference lies in the open
 // skip band (S, L-S) / (-(L-S), -S) and the minimum-image distance is at most
 // S + threshold (S = a_i + a_j).
@@ -492,6 +502,17 @@ int pbx_launch_pointers(const double* xyz, int N, double L, double* P3, hipStrea
     PBX_CHECK_BOX(L);
     size_t total = (size_t)N * N;
     hipLaunchKernelGGL(pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xyz, N, L, P3);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_immobile_pointers(const double* xm, const double* xi, int nm, int ni, double L, double* P3,
+                                 hipStream_t s) {
+    if (nm <= 0 || ni <= 0) return PBX_OK;
+    PBX_CHECK_BOX(L);
+    size_t total = (size_t)nm * ni;
+    hipLaunchKernelGGL(immobile_pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xm, xi, nm, ni,
+                       L, P3);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -11,6 +11,8 @@ on a machine where the reference tree is absent.  Cases (seeded):
              overlapping and contained pairs                diff_tensor.c:73
   smith      N=16, L in {20, 150}, m = n in {0..3}          diff_tensor.c:170
   jo2b       24 (ai, aj, r) near-contact pairs              diff_tensor.c:270
+  immobile   12 mobile x 7 immobile beads, L=20, positions
+             up to 2.5 L out of the box (wrap loops)        bead.c:99
 
     make -C oracle && python tests/golden/make_ref_vectors.py
 """
@@ -71,6 +73,12 @@ def main():
         jo_in.append([ai, aj, d[0], d[1], d[2]])
         jo_out.append(ref.jo_2b_r_matrix(ai, aj, d))
     v["jo_in"], v["jo_out"] = np.array(jo_in), np.array(jo_out)
+
+    r2 = np.random.RandomState(20241203)  # own stream: earlier cases unchanged
+    xm = r2.uniform(-50.0, 50.0, (12, 3))
+    xi = r2.uniform(-50.0, 50.0, (7, 3))
+    v["imm_xm"], v["imm_xi"], v["imm_L"] = xm, xi, 20.0
+    v["imm_P3"] = ref.pointer_immobile_pbc_matrix(xm, xi, 20.0)
     np.savez_compressed(OUT, **v)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
 

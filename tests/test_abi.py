@@ -24,7 +24,7 @@ def _declared():
 def test_header_declares_reference_entry_points():
     names = _declared()
     for ref in ("RPY_M_matrix", "RPY_Smith_M_matrix", "JO_2B_R_matrix", "JO_R_lubrication_correction_matrix",
-                "pointer_pbc_matrix", "check_overlaps"):
+                "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps"):
         assert ref in names
 
 

@@ -228,3 +228,59 @@ def test_c2_full_size_first_steps():
     xyz, a, inp, dt, _ = workload("c2", seed=0)
     g, o, _ = _run_pair(xyz, a, inp, 3, dt)
     print("attempts:", o.attempt_log)
+
+
+def _restart_pair(xyz, a, inp, M, dt, flags=None, **kw):
+    """test_box.py:53-117: pickle after M steps, run M//2 more; unpickle the
+    copy and run the same M//2 steps; positions, flux and counters agree bit
+    for bit."""
+    import pickle
+    g = _gpu_box(xyz, a, inp, **kw)
+    f = flags or (lambda j: (True, True, True))
+    for j in range(M):
+        g.propagate(dt, *f(j))
+    saved = pickle.dumps(g)
+    for j in range(M, M + M // 2):
+        g.propagate(dt, *f(j))
+    original = np.array([b.r for b in g.beads])
+    flux0, draws0 = dict(getattr(g, "net_flux", {})), g.draw_count
+    g.close()
+    h = pickle.loads(saved)
+    for j in range(M, M + M // 2):
+        h.propagate(dt, *f(j))
+    restarted = np.array([b.r for b in h.beads])
+    assert np.array_equal(original, restarted)
+    assert dict(getattr(h, "net_flux", {})) == flux0 and h.draw_count == draws0
+
+
+@pytest.mark.parametrize("M", [4, 7, 11])
+def test_restart_nohi_seed_sync(M):
+    # test_box.py:53-82 design: 125 beads of radius 0.1 on a unit grid, seed 1
+    xyz = np.array([[i, j, k] for i in range(5) for j in range(5) for k in range(5)], float)
+    a = np.full(125, 0.1)
+    inp = dict(BASE, hydrodynamics="nohi", seed=1, measure_flux=FLUX)
+    _restart_pair(xyz, a, inp, M, 0.001)
+
+
+@pytest.mark.parametrize("noise", ["native", "callback"])
+def test_restart_smith_cadence(noise):
+    # Cholesky every second step: the restored B must be the saved one
+    xyz, a = _grid([-2.3, 0.0, 2.3], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy_smith", ewald_real=1, ewald_imag=1, seed=9, measure_flux=FLUX)
+    _restart_pair(xyz, a, inp, 9, 0.02, flags=lambda j: (j % 2 == 0, True, j % 2 == 0), noise=noise,
+                  xi_pool_attempts=5)
+
+
+def test_restart_midpoint_lub():
+    xyz, a = _grid([-2.2, 0.0, 2.2], 1.0)
+    inp = dict(BASE, hydrodynamics="rpy_smith_lub", propagation_scheme="midpoint", lubrication_cutoff=10,
+               ewald_real=1, ewald_imag=1, m_midpoint=100.0, seed=2)
+    _restart_pair(xyz, a, inp, 6, 1e-10, flags=lambda j: (j % 3 == 0, j % 2 == 0, True))
+
+
+def test_restore_rejects_a_different_simulation():
+    xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
+    g = _gpu_box(xyz, a, dict(BASE, hydrodynamics="rpy"))
+    h = _gpu_box(xyz, a, dict(BASE, hydrodynamics="nohi"))
+    with pytest.raises(RuntimeError):
+        h.restore(g.snapshot())

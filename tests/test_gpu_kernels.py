@@ -37,6 +37,14 @@ def test_pointers_bit_exact(ref_vectors):
     assert np.array_equal(compute_pointer_pbc_matrix(beads, float(v["ptr_L"])), v["ptr_P3"])
 
 
+def test_immobile_pointers_bit_exact(ref_vectors):
+    from pybrown_amd.bead import Bead, compute_pointer_immobile_pbc_matrix
+    v = ref_vectors
+    mob = [Bead(x, 1.0) for x in v["imm_xm"]]
+    imm = [Bead(x, 1.0, mobile=False) for x in v["imm_xi"]]
+    assert np.array_equal(compute_pointer_immobile_pbc_matrix(mob, imm, float(v["imm_L"])), v["imm_P3"])
+
+
 def test_overlaps_exact(ref_vectors):
     from pybrown_amd.bead import Bead, check_overlaps
     v = ref_vectors

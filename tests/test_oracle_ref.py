@@ -12,6 +12,24 @@ def test_pointers_fixture(orc, ref_vectors):
     assert np.array_equal(orc.pointer_pbc_matrix(v["ptr_xyz"], float(v["ptr_L"])), v["ptr_P3"])
 
 
+def test_immobile_pointers_fixture(orc, ref_vectors):
+    v = ref_vectors
+    got = orc.pointer_immobile_pbc_matrix(v["imm_xm"], v["imm_xi"], float(v["imm_L"]))
+    assert np.array_equal(got, v["imm_P3"])
+
+
+def test_immobile_pointers_closed_form(orc):
+    # every component lies in (-L/2, L/2] and differs from the raw
+    # difference by a whole number of boxes
+    rng = np.random.RandomState(3)
+    xm, xi, L = rng.uniform(-40, 40, (9, 3)), rng.uniform(-40, 40, (5, 3)), 13.0
+    P = orc.pointer_immobile_pbc_matrix(xm, xi, L)
+    raw = xi[None, :, :] - xm[:, None, :]
+    k = (raw - P) / L
+    assert np.allclose(k, np.round(k), atol=1e-12)
+    assert np.all(P <= L / 2) and np.all(P > -L / 2)
+
+
 def test_overlaps_fixture(orc, ref_vectors):
     v = ref_vectors
     got = [orc.check_overlaps(x, a, float(v["ovl_L"]), float(t)) for x, a, t in zip(v["ovl_xyz"], v["ovl_a"], v["ovl_thr"])]
