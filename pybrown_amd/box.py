@@ -67,6 +67,8 @@ class Box:
         if inp["propagation_scheme"] not in _lib.SCHEME_CODES:
             raise ValueError("unknown propagation_scheme %r" % inp["propagation_scheme"])
 
+        if any(b.bonded_with or b.angled_with for b in beads):
+            raise NotImplementedError("bonds and angles (force fields) are outside the hydrodynamic step")
         self._init_args = (xi_pool_attempts, noise, attempt_batch)
         self.beads = beads
         self.box_length = inp["box_length"]
@@ -116,6 +118,7 @@ class Box:
             self.planes = Plane(self.flux_plane_point, self.flux_normal)
             self.net_flux = {lab: 0 for lab in self.mobile_labels}
         self.is_concentration = False
+        self.is_energy = "enr_write_freq" in inp  # box.py:77-79; E stays 0 (no force field)
         self.is_wall = False
         self.end_simulation = False
         self.E = 0.0

@@ -155,9 +155,11 @@ __global__ __launch_bounds__(256, 2) void chol_dag_kernel(double* A, int ld, int
 
 }  // namespace
 
+// State block shared by the single-launch factorisations (this file and
+// pbx_chol_persist.hip): 16 words, T x T tile flags, T column counters.
 size_t pbx_dag_state_ints(int np) {
     int T = np / T64;
-    return 16 + (size_t)T * T;
+    return 16 + (size_t)T * T + T;
 }
 
 // In-place lower Cholesky (same contract as pbx_potrf) in one persistent

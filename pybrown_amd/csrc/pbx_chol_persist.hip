@@ -44,7 +44,7 @@ namespace {
 
 constexpr int PER_CU = 1;    // resident workgroups per compute unit
 constexpr int ST_ERR = 4;    // state word: a wait ran out of budget
-constexpr int ST_FLAGS = 16; // state words: T x T completion flags
+constexpr int ST_FLAGS = 16; // state words: T x T completion flags, then T column counters
 
 __device__ bool wait_flag(const int* flag, int target, int* err) {
     __shared__ int ok;
@@ -81,7 +81,7 @@ __device__ __forceinline__ void tile_of(int t, int& i, int& j) {
 }
 
 __global__ void persist_reset_kernel(int* state, int T) {
-    const int n = ST_FLAGS + T * T;
+    const int n = ST_FLAGS + T * T + T;
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) state[e] = 0;
 }
 
@@ -96,6 +96,7 @@ __global__ __launch_bounds__(256, 1) void potrf_persist_kernel(double* A, int ld
     __shared__ double sd[T64];
     int* err = state + ST_ERR;
     int* done = state + ST_FLAGS;
+    int* colcnt = done + T * T;  // L tiles published below the diagonal, per column
     const int W = gridDim.x - 1;
     const int ntiles = T * (T + 1) / 2;
 
@@ -125,6 +126,8 @@ __global__ __launch_bounds__(256, 1) void potrf_persist_kernel(double* A, int ld
                     sA[c * SP + r] = v;  // op(A)(r, kk) = L[r][kk]
                 });
                 publish_flag(done + k * T + k - 1, 2);  // includes the barrier for sA
+                if (threadIdx.x == 0)
+                    __hip_atomic_fetch_add(colcnt + k - 1, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 // A_kk - L L^T into sB (row-major) for the factorisation
                 v4d S[2][2];
                 acc_zero(S);
@@ -164,6 +167,7 @@ __global__ __launch_bounds__(256, 1) void potrf_persist_kernel(double* A, int ld
     }
     for (int k = 0; k <= kend; ++k) {
         // phase 1: finish tiles whose updates are complete
+        bool have_w = false;
         for (int t = 1 + w; t < ntiles; t += W) {
             int i, j;
             tile_of(t, i, j);
@@ -174,7 +178,8 @@ __global__ __launch_bounds__(256, 1) void potrf_persist_kernel(double* A, int ld
                 continue;
             }
             // L_ij = A_ij W_j^T  (A_ij holds every update by now)
-            if (!wait_flag(done + j * T + j, 2, err)) return;
+            if (!have_w && !wait_flag(done + j * T + j, 2, err)) return;
+            have_w = true;
             double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
             stage<true>(sA, Aij, ld);
             stage<true>(sB, work + (size_t)j * T64 * T64, T64);
@@ -184,15 +189,18 @@ __global__ __launch_bounds__(256, 1) void potrf_persist_kernel(double* A, int ld
             mma64(sA, sB, out);
             acc_for_each(out, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] = v; });
             publish_flag(done + i * T + j, 2);
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(colcnt + j, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // phase 2: A_ij -= L_ik L_jk^T on the remaining tiles, in row order
+        // phase 2: A_ij -= L_ik L_jk^T on the remaining tiles, in row order,
+        // once every L tile of column k is published (one wait per panel)
+        bool have_col = false;
         for (int t = 1 + w; t < ntiles; t += W) {
             int i, j;
             tile_of(t, i, j);
             const int nu = (i == j) ? j - 1 : j;
             if (k >= nu) continue;
-            if (!wait_flag(done + i * T + k, 2, err)) return;
-            if (i != j && !wait_flag(done + j * T + k, 2, err)) return;
+            if (!have_col && !wait_flag(colcnt + k, T - 1 - k, err)) return;
+            have_col = true;
             double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
             double cv[16];
             acc_load(Aij, ld, cv);
