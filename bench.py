@@ -61,7 +61,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-5; 0 = library default)")
+    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-6; 0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -181,11 +181,12 @@ def roofline(name, N, stage):
     n = 3 * N
     k = int(np.argmax(stage))
     if k == 1:
+        from pybrown_amd._lib import lib
         flops = n ** 3 / 3.0
         ach = flops / (stage[1] * 1e-3) / 1e12
         return {"kernel": "cholesky (potrf, n=%d)" % n, "bound": "mfma", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic("cholesky_%s_n%d" % (name, n)),
+                "traffic": measured_traffic("cholesky_%s_n%d_v%d" % (name, n, lib.pbx_get_potrf_variant())),
                 "algorithmic": "n^3/3 = %.3e flop per factorisation" % flops}
     if k == 0:
         flops = smith_flops(N) if name in ("c2", "c4", "c3") else 30.0 * N * (N - 1) / 2

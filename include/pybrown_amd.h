@@ -69,10 +69,18 @@ int pbx_cholesky(const double* A, int n, double* L);
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
 
 /* Selects the Cholesky used by every factorisation in this process:
+ * 0 = the library default (6);
  * 1 = one launch per panel, column-by-column diagonal tiles;
- * 2 = one launch per panel, 8-column-panel diagonal tiles (default);
- * 3 = one persistent launch over the tile dependency graph. */
+ * 2 = one launch per panel, 8-column-panel diagonal tiles;
+ * 3 = one persistent launch over the tile dependency graph;
+ * 4 = one launch per panel, the solve of the next tile column folded in,
+ *     wave-specialised diagonal tiles;
+ * 5 = one persistent launch, tiles owned by workgroups;
+ * 6 = two tile columns per launch (rank-128 trailing updates).
+ * Returns PBX_ERR_ARG for any other value. */
 int pbx_set_potrf_variant(int variant);
+/* The Cholesky variant in use (1-6). */
+int pbx_get_potrf_variant(void);
 
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
  * FP64 MFMA tile product used by the factorisations, for testing. */

@@ -16,11 +16,15 @@
 //     syrk(k):   A_ij -= L_ik L_jk^T  for k < j <= i   (one workgroup per tile)
 //                and the workgroup owning tile (k+1,k+1) factors it in LDS
 // Variant 3 is the single-launch tile graph of pbx_chol_dag.hip.
-// Variant 4 (default) folds trsm(k+1) into syrk(k): the workgroups of tile
+// Variant 4 folds trsm(k+1) into syrk(k): the workgroups of tile
 // column k+1 keep their updated tile in registers, wait (bounded) for the
 // look-ahead workgroup to publish inv(L_{k+1,k+1}), and solve in place, so
 // each panel costs one launch; its diagonal tiles use the wave-specialised
 // factorisation (tile_factor_ws).
+// Variant 5 is the persistent launch of pbx_chol_persist.hip.  Variant 6
+// (default) factors two tile columns per launch (potrf2_kernel below): one
+// rank-128 pass over the trailing matrix per two columns instead of two
+// rank-64 passes, and half the launches.
 // inv(L_kk) is produced with each diagonal factorisation and kept in `work`
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
 #include "pbx_common.h"
@@ -294,6 +298,311 @@ __global__ void la_finish_kernel(const int* la, int* info) {
 }
 
 // ---------------------------------------------------------------------------
+// Variant 6: two tile columns per launch.  Launch (c0, P, nc) applies panels
+// c0-P .. c0-1 (P <= 2: one rank-64P update per tile, so the trailing matrix
+// is read and written once per two columns) and factors columns c0 .. c0+nc-1
+// (nc <= 2).  Workgroups, in dispatch order:
+//   0         tile (c0, c0), which arrives fully updated: factor, publish
+//             inv(L) ([LA_FLAG] = c0+1); then L_{c0+1,c0} = C W^T, tile
+//             (c0+1, c0+1) less L_{c0+1,c0} L_{c0+1,c0}^T, factor, publish
+//             ([LA_FLAG] = c0+2).  L tiles are stored after each publish:
+//             only the inverses are waited for;
+//   1         (nc = 2) tile (c0+1, c0) through the panels, written to a
+//             scratch tile (its place in A receives L_{c0+1,c0} from workgroup
+//             0 while the rows still read the updated value);
+//   2         (nc = 2, P > 0) tile (c0+1, c0+1) through the panels; both
+//             counted in [LA_UPD];
+//   row i     (i >= c0+nc) tiles (i, c0) and (i, c0+1): update, then
+//             L_{i,c0} = C W_{c0}^T, its own copy of L_{c0+1,c0},
+//             C_{i,c0+1} -= L_{i,c0} L_{c0+1,c0}^T, L_{i,c0+1} = C W_{c0+1}^T.
+//             Row c0+2 also brings tile (c0+2, c0+2) fully up to date (the
+//             panels, then L_{i,c0} L_{i,c0}^T and L_{i,c0+1} L_{i,c0+1}^T), so
+//             the next launch's diagonal chain starts without an update;
+//   the rest  trailing tiles (i, j), j >= c0+nc, except (c0+2, c0+2): the
+//             panel update.
+// Workgroup 0 waits only on workgroups 1 and 2, which wait on nothing; row
+// workgroups wait only on workgroups 0, 1 and 2.  Lower indices are
+// dispatched first, so every wait ends (each is also bounded).
+constexpr int LA_UPD = 4;
+
+__global__ void la_reset_kernel(int* la) {
+    if (threadIdx.x >= LA_FLAG && threadIdx.x <= LA_UPD) la[threadIdx.x] = 0;
+}
+
+__device__ __forceinline__ void publish_agent(int* flag, int v) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// S += sum_{m = m0 .. m0+P-1} L_im L_jm^T (i == j: one operand).  Other
+// waves may still read sA/sB on return: synchronise before reusing them.
+__device__ __forceinline__ void panel_update(const double* A, int ld, int i, int j, int m0, int P, double* sA,
+                                             double* sB, v4d S[2][2]) {
+    const double* Ai = A + (size_t)i * T64 * ld;
+    const double* Aj = A + (size_t)j * T64 * ld;
+    for (int m = m0; m < m0 + P; ++m) {
+        if (m > m0) __syncthreads();
+        stage<true>(sA, Ai + (size_t)m * T64, ld);
+        if (i != j) stage<true>(sB, Aj + (size_t)m * T64, ld);
+        __syncthreads();
+        mma64(sA, i == j ? sA : sB, S);
+    }
+}
+
+// s[c][r] = cv - S at this thread's accumulator positions (op(A) layout)
+__device__ __forceinline__ void put_diff_t(double* s, const double cv[16], v4d S[2][2]) {
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) s[acc_col(y) * SP + acc_row(x, v)] = cv[q] - S[x][y][v];
+}
+
+// s[r][c] = cv - S (row-major tile for the factorisation)
+__device__ __forceinline__ void put_diff(double* s, const double cv[16], v4d S[2][2]) {
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) s[acc_row(x, v) * SP + acc_col(y)] = cv[q] - S[x][y][v];
+}
+
+// s[c][r] = X[r][c] for an accumulator tile (op(A) layout)
+__device__ __forceinline__ void put_t(double* s, v4d X[2][2]) {
+    acc_for_each(X, [&](int r, int c, double v) { s[c * SP + r] = v; });
+}
+
+// X = cv - S in global memory (lower part only when `lower`)
+__device__ __forceinline__ void store_diff(double* X, int ld, const double cv[16], v4d S[2][2], bool lower) {
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) {
+                const int r = acc_row(x, v), c = acc_col(y);
+                if (!lower || c <= r) X[(size_t)r * ld + c] = cv[q] - S[x][y][v];
+            }
+}
+
+// Lower part of the diagonal tile X -= op(A) op(A)^T, op(A) in sA (block
+// synchronised by the caller); the load of X overlaps the product.
+__device__ __forceinline__ void syrk_store(double* X, int ld, const double* sA) {
+    double cv[16];
+    acc_load(X, ld, cv);
+    v4d S[2][2];
+    acc_zero(S);
+    mma64(sA, sA, S);
+    store_diff(X, ld, cv, S, true);
+}
+
+__device__ __forceinline__ void store_w(const double* sW, double* Wk) {
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) Wk[e] = sW[(e >> 6) * SP + (e & 63)];
+}
+
+__device__ __forceinline__ void store_l(const double* sL, double* Akk, int ld) {
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (c <= r) Akk[(size_t)r * ld + c] = sL[r * SP + c];
+    }
+}
+
+template <int V, int OCC>
+__global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
+                                                        int upd_target, double* work, double* scr, int* info,
+                                                        int* la) {
+    __shared__ double sA[T64 * SP];
+    __shared__ double sB[T64 * SP];
+    __shared__ double sd[T64];
+    const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
+    const int nw = T - c0 - nc;
+    const int b = blockIdx.x;
+    auto tile = [&](int i, int j) { return A + (size_t)i * T64 * ld + (size_t)j * T64; };
+    int* err = la + LA_ERR;
+    double* W0 = work + (size_t)c0 * T64 * T64;
+    double* W1 = W0 + T64 * T64;
+
+    if (b == 0) {
+        stage<false>(sA, tile(c0, c0), ld);
+        __syncthreads();
+        factor_tile<V>(sA, sB, sd, c0 * T64, info);
+        store_w(sB, W0);
+        publish_agent(la + LA_FLAG, c0 + 1);
+        store_l(sA, tile(c0, c0), ld);
+        if (nc == 1) return;
+        if (!wait_ge_agent(la + LA_UPD, upd_target, err)) return;
+        // W^T in sB (op(B)(kk, c) = W[c][kk]); W is lower triangular
+        for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+            const int r = e >> 6, c = e & 63;
+            if (c > r) {
+                const double t = sB[r * SP + c];
+                sB[r * SP + c] = sB[c * SP + r];
+                sB[c * SP + r] = t;
+            }
+        }
+        double* Aoff = tile(c0 + 1, c0);
+        double cv[16];
+        acc_load(tile(c0 + 1, c0 + 1), ld, cv);
+        stage<true>(sA, scr, T64);
+        __syncthreads();
+        v4d Lo[2][2];
+        acc_zero(Lo);
+        mma64(sA, sB, Lo);
+        acc_for_each(Lo, [&](int r, int c, double v) { Aoff[(size_t)r * ld + c] = v; });
+        __syncthreads();
+        put_t(sA, Lo);
+        __syncthreads();
+        v4d S[2][2];
+        acc_zero(S);
+        mma64(sA, sA, S);
+        put_diff(sB, cv, S);
+        __syncthreads();
+        factor_tile<V>(sB, sA, sd, (c0 + 1) * T64, info);
+        store_w(sA, W1);
+        publish_agent(la + LA_FLAG, c0 + 2);
+        store_l(sB, tile(c0 + 1, c0 + 1), ld);
+        return;
+    }
+    if (b <= nh) {
+        // tiles (c0+1, c0) (to the scratch tile) and (c0+1, c0+1)
+        const int i = c0 + 1, j = (b == 1) ? c0 : c0 + 1;
+        double* Aij = tile(i, j);
+        double cv[16];
+        acc_load(Aij, ld, cv);
+        v4d S[2][2];
+        acc_zero(S);
+        if (P > 0) panel_update(A, ld, i, j, c0 - P, P, sA, sB, S);
+        if (b == 1)
+            store_diff(scr, T64, cv, S, false);
+        else
+            store_diff(Aij, ld, cv, S, true);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(la + LA_UPD, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if (b <= nh + nw) {
+        const int i = c0 + nc + (b - 1 - nh);
+        double* A0 = tile(i, c0);
+        if (nc == 1) {
+            double cv[16];
+            acc_load(A0, ld, cv);
+            v4d S[2][2];
+            acc_zero(S);
+            if (P > 0) {
+                panel_update(A, ld, i, c0, c0 - P, P, sA, sB, S);
+                __syncthreads();
+            }
+            put_diff_t(sA, cv, S);
+            if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
+            stage<true>(sB, W0, T64);
+            __syncthreads();
+            v4d out[2][2];
+            acc_zero(out);
+            mma64(sA, sB, out);
+            acc_for_each(out, [&](int r, int c, double v) { A0[(size_t)r * ld + c] = v; });
+            return;
+        }
+        double* A1 = tile(i, c0 + 1);
+        double* Aii = tile(i, i);
+        const bool diag = (i == c0 + 2);  // also finishes tile (i, i) for the next launch
+        if (diag && P > 0) {
+            double cv[16];
+            acc_load(Aii, ld, cv);
+            v4d S[2][2];
+            acc_zero(S);
+            panel_update(A, ld, i, i, c0 - P, P, sA, sB, S);
+            store_diff(Aii, ld, cv, S, true);
+            __syncthreads();
+        }
+        // tile (i, c0+1) through the panels, back in place (each thread later
+        // reads back the values it wrote)
+        if (P > 0) {
+            double cv[16];
+            acc_load(A1, ld, cv);
+            v4d S[2][2];
+            acc_zero(S);
+            panel_update(A, ld, i, c0 + 1, c0 - P, P, sA, sB, S);
+            store_diff(A1, ld, cv, S, false);
+            __syncthreads();
+        }
+        // tile (i, c0): update, then L_{i,c0} = C W_{c0}^T
+        v4d L0[2][2];
+        {
+            double cv[16];
+            acc_load(A0, ld, cv);
+            v4d S[2][2];
+            acc_zero(S);
+            if (P > 0) {
+                panel_update(A, ld, i, c0, c0 - P, P, sA, sB, S);
+                __syncthreads();
+            }
+            put_diff_t(sA, cv, S);
+        }
+        if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
+        stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+        __syncthreads();
+        acc_zero(L0);
+        mma64(sA, sB, L0);
+        acc_for_each(L0, [&](int r, int c, double v) { A0[(size_t)r * ld + c] = v; });
+        // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
+        if (!wait_ge_agent(la + LA_UPD, upd_target, err)) return;
+        stage<true>(sA, scr, T64);
+        __syncthreads();
+        v4d Lo[2][2];
+        acc_zero(Lo);
+        mma64(sA, sB, Lo);
+        __syncthreads();
+        // C_{i,c0+1} -= L_{i,c0} L_{c0+1,c0}^T
+        put_t(sA, L0);
+        put_t(sB, Lo);
+        __syncthreads();
+        {
+            double c1[16];
+            acc_load(A1, ld, c1);
+            v4d S[2][2];
+            acc_zero(S);
+            mma64(sA, sB, S);
+            if (diag) syrk_store(Aii, ld, sA);
+            __syncthreads();
+            put_diff_t(sA, c1, S);
+        }
+        if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err)) return;
+        stage<true>(sB, W1, T64);
+        __syncthreads();
+        v4d L1[2][2];
+        acc_zero(L1);
+        mma64(sA, sB, L1);
+        acc_for_each(L1, [&](int r, int c, double v) { A1[(size_t)r * ld + c] = v; });
+        if (diag) {
+            __syncthreads();
+            put_t(sA, L1);
+            __syncthreads();
+            syrk_store(Aii, ld, sA);
+        }
+        return;
+    }
+    // trailing tile (i, j), j >= c0 + nc; with two columns, tile
+    // (c0+2, c0+2) (first in this order) belongs to row c0+2
+    int ii, jj;
+    tri_decode(b - 1 - nh - nw + (nc == 2 ? 1 : 0), ii, jj);
+    const int i = c0 + nc + ii, j = c0 + nc + jj;
+    double* Aij = tile(i, j);
+    double cv[16];
+    acc_load(Aij, ld, cv);
+    v4d S[2][2];
+    acc_zero(S);
+    panel_update(A, ld, i, j, c0 - P, P, sA, sB, S);
+    store_diff(Aij, ld, cv, S, i == j);
+}
+
+// ---------------------------------------------------------------------------
 // Triangular inverse X = L^{-1} (lower) from a factored A and its diagonal
 // inverses W_i: X_ii = W_i; X_ij = -W_i * sum_{j<=m<i} L_im X_mj  (i > j).
 // One launch per tile row i, one workgroup per j < i.
@@ -435,20 +744,30 @@ static int device_cus() {
 }
 
 // work = [T inverse tiles of the diagonal factor][single-launch DAG state]
-size_t pbx_dense_work_doubles(int np) { return (size_t)np * T64 + (pbx_dag_state_ints(np) + 1) / 2; }
+//        [one scratch tile, 16-byte aligned]
+static size_t dense_scratch_offset(int np) {
+    const size_t o = (size_t)np * T64 + (pbx_dag_state_ints(np) + 1) / 2;
+    return (o + 1) & ~(size_t)1;
+}
+size_t pbx_dense_work_doubles(int np) { return dense_scratch_offset(np) + (size_t)T64 * T64; }
 
 // Cholesky variant: 1 = launch per panel, column-by-column diagonal tiles;
 // 2 = launch per panel, 8-column panel diagonal tiles; 3 = single persistent
 // launch over the tile dependency graph (pbx_chol_dag.hip); 4 = one launch
 // per panel with the solve folded in and wave-specialised diagonal tiles;
 // 5 = persistent launch with tiles owned in registers (pbx_chol_persist.hip;
-// falls back to 4 when the tiles do not fit the compute units).
-static int g_potrf_variant = 4;
+// falls back to 4 when the tiles do not fit the compute units); 6 (default)
+// = two tile columns per launch (rank-128 trailing updates); 0 selects the
+// default.
+constexpr int kDefaultPotrf = 6;
+static int g_potrf_variant = kDefaultPotrf;
+static int g_own_cu_tiles = 64;
 extern "C" int pbx_set_potrf_variant(int v) {
-    if (v < 1 || v > 5) return PBX_ERR_ARG;
-    g_potrf_variant = v;
+    if (v < 0 || v > 6) return PBX_ERR_ARG;
+    g_potrf_variant = v ? v : kDefaultPotrf;
     return PBX_OK;
 }
+extern "C" int pbx_get_potrf_variant(void) { return g_potrf_variant; }
 
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s) {
     if (np == n) return PBX_OK;
@@ -483,6 +802,36 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
     if (g_potrf_variant == 5 && pbx_persist_fits(np, device_cus())) {
         int* state = (int*)(work + (size_t)np * T64);
         TRY_RC(pbx_potrf_persist(A, np, ld, work, info_dev, state, device_cus(), s));
+        return PBX_OK;
+    }
+    if (g_potrf_variant == 6) {
+        int* la = (int*)(work + (size_t)np * T64);
+        // Up to 64 tile columns the trailing updates of a launch finish well
+        // within its diagonal chain, so each workgroup gets a compute unit of
+        // its own (dynamic LDS beyond half the CU's 160 KB): nothing shares
+        // the chain's CU.  Larger orders are throughput-bound and keep two
+        // workgroups per CU.
+        const bool own_cu = T <= g_own_cu_tiles;
+        const size_t dyn_lds = own_cu ? 24576 : 0;
+        hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
+        int upd_target = 0;
+        for (int c0 = 0, P = 0; c0 < T;) {
+            const int nc = T - c0 < 2 ? T - c0 : 2;
+            const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
+            const int nw = T - c0 - nc;
+            const int ntr = (P > 0 && nw > 0) ? nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0) : 0;
+            upd_target += nh;
+            if (own_cu)
+                hipLaunchKernelGGL((potrf2_kernel<5, 1>), dim3(1 + nh + nw + ntr), dim3(256), dyn_lds, s, A, ld, T, c0,
+                                   P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+            else
+                hipLaunchKernelGGL((potrf2_kernel<5, 2>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0, P,
+                                   nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+            P = nc;
+            c0 += nc;
+        }
+        hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
+        PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
     if (g_potrf_variant >= 4) {

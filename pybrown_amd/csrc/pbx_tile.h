@@ -26,15 +26,20 @@ __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
 // into LDS.  rk = true: s[col][row] = X[row][col] (transposed); rk = false:
 // s[row][col] = X[row][col].  All eight 16-byte loads per thread are issued
 // before any LDS store so their latencies overlap.
-template <bool RK>
-__device__ __forceinline__ void stage(double* s, const double* X, int ld) {
-    v2d v[8];
+// The two halves are also available separately (stage_fetch into registers,
+// stage_put into LDS) so a caller can fetch the next operand while the
+// matrix cores work on the current one.
+__device__ __forceinline__ void stage_fetch(v2d v[8], const double* X, int ld) {
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const int e = threadIdx.x + it * 256;  // 2048 pairs
         const int row = e >> 5, col = (e & 31) * 2;
         v[it] = *(const v2d*)(X + (size_t)row * ld + col);
     }
+}
+
+template <bool RK>
+__device__ __forceinline__ void stage_put(double* s, const v2d v[8]) {
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const int e = threadIdx.x + it * 256;
@@ -47,6 +52,13 @@ __device__ __forceinline__ void stage(double* s, const double* X, int ld) {
             s[row * SP + col + 1] = v[it].y;
         }
     }
+}
+
+template <bool RK>
+__device__ __forceinline__ void stage(double* s, const double* X, int ld) {
+    v2d v[8];
+    stage_fetch(v, X, ld);
+    stage_put<RK>(s, v);
 }
 
 // acc += op(A) op(B) over a 64-deep slab; wave w owns the 32 x 32 quadrant

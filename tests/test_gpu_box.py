@@ -154,7 +154,7 @@ def test_single_launch_cholesky_trajectory():
     try:
         _run_pair(xyz, a, inp, 10, 1.0)
     finally:
-        lib.pbx_set_potrf_variant(4)
+        lib.pbx_set_potrf_variant(0)
 
 
 def _trajectory(xyz, a, inp, steps, dt, **kw):
