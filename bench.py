@@ -64,11 +64,17 @@ def main():
     ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-6; 0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--requests", type=int, default=48, help="c5: requests taken from the head of the stream")
+    ap.add_argument("--slots", type=int, default=16, help="c5: concurrent simulations")
+    ap.add_argument("--request-steps", type=int, default=200, help="c5: steps per request")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        serve_bench(args)
+        return
 
     import ctypes
 
@@ -135,6 +141,71 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
+    print(json.dumps(out))
+
+
+def serve_bench(args):
+    """Served request stream (c5): the first --requests requests of the
+    stream (pybrown_amd.serving), run (1) one at a time, (2) served with all
+    arrived at the start (throughput X) and (3) served with Poisson arrivals
+    at rate 1.5 X, which gives the reported sims/s and p50/p99 time to
+    trajectory."""
+    # hardware queues the HIP runtime spreads the simulations' streams over;
+    # read once when the runtime starts, so set before anything touches it
+    os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.slots, 4), 16))
+    from pybrown_amd.serving import Server, make_requests_parallel, stream_params, summary
+    params = stream_params(args.requests)
+    t = time.perf_counter()
+    reqs = make_requests_parallel(params, steps=args.request_steps, workers=min(16, os.cpu_count() or 1))
+    print("c5: %d requests placed in %.1f s (N %d..%d, %s)" % (
+        len(reqs), time.perf_counter() - t, min(r.N for r in reqs), max(r.N for r in reqs),
+        {m: sum(r.mode == m for r in reqs) for m in ("rpy_smith", "rpy", "rpy_smith_lub")}), file=sys.stderr)
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    budget = 250e9
+    small = sorted(reqs, key=lambda r: r.N)[:2]
+    Server(slots=2, hbm_budget=budget).run(small, realtime=False)  # warm-up: code objects, allocator
+    one = Server(slots=1, hbm_budget=budget).run(reqs, realtime=False)
+    base = summary(one)
+    print("c5 one at a time: %s" % base, file=sys.stderr)
+    for r in sorted(one, key=lambda r: r.start - r.end)[:8]:
+        print("   request %4d  N %5d  %-14s service %.3f s" % (r.rid, r.N, r.mode, r.end - r.start), file=sys.stderr)
+    burst = summary(Server(slots=args.slots, hbm_budget=budget).run(reqs, realtime=False))
+    print("c5 served, all at start: %s" % burst, file=sys.stderr)
+    rate = 1.5 * burst["sims_per_s"]
+    for r, p in zip(reqs, stream_params(args.requests, rate=rate)):
+        r.arrival = p["arrival"]
+    res = Server(slots=args.slots, hbm_budget=budget).run(reqs, realtime=True)
+    served = summary(res)
+    print("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served), file=sys.stderr)
+    bad = [r.status for r in res if r.status != "ok"]
+    assert not bad, bad[:3]
+    out = {
+        "metric": "served sims/s + p99 latency (request stream)",
+        "value": served["sims_per_s"],
+        "unit": "sims/s",
+        "n_gpus": 1,
+        "steps": args.request_steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "request stream: first %d requests, N log-uniform 32..3000, phi 0.02..0.30, "
+                               "60%% rpy_smith / 25%% rpy (open box) / 15%% rpy_smith_lub, ermak dt=1ps, "
+                               "%d steps each, flux x=0, Poisson arrivals at 1.5x the burst throughput"
+                               % (args.requests, args.request_steps),
+                   "requests": args.requests, "slots": args.slots, "hbm_budget_bytes": budget,
+                   "arrival_rate_per_s": rate, "gpu_max_hw_queues": os.environ["GPU_MAX_HW_QUEUES"]},
+        "p50_s": served["p50_s"],
+        "p99_s": served["p99_s"],
+        "one_at_a_time_sims_per_s": base["sims_per_s"],
+        "burst_sims_per_s": burst["sims_per_s"],
+        "speedup_vs_one_at_a_time": served["sims_per_s"] / base["sims_per_s"],
+        "roofline": None,
+    }
     print(json.dumps(out))
 
 

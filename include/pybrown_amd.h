@@ -162,6 +162,19 @@ int pbx_sim_propagate(pbx_sim* sim, double dt, int build_Dff, int build_Dnf, int
 int pbx_sim_propagate_out(pbx_sim* sim, double dt, int build_Dff, int build_Dnf, int cholesky, double* xyz_out,
                           long long* flux_out);
 
+/* The BD.py:164-185 driver loop without file output, in one call: steps
+ * j = j0 .. j0+steps-1 of Box.propagate with the cadence flags of BD.py:177
+ * (j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0).  flux_out
+ * (steps x n_labels, or NULL) receives each step's per-label flux (the .flx
+ * line after it); xyz_out (or NULL) the positions before each step j with
+ * j % xyz_every == 0, one N x 3 frame per such step (the .xyz frames).
+ * Results equal the same steps through pbx_sim_propagate_out. */
+int pbx_sim_run(pbx_sim* sim, double dt, long long j0, long long steps, int diff_freq, int lub_freq, int chol_freq,
+                long long* flux_out, double* xyz_out, int xyz_every);
+/* Device memory held by the simulation, in bytes (what admission control
+ * counts against an HBM budget). */
+long long pbx_sim_device_bytes(pbx_sim* sim);
+
 int pbx_sim_get_positions(pbx_sim* sim, double* xyz);
 int pbx_sim_set_positions(pbx_sim* sim, const double* xyz);
 int pbx_sim_get_flux(pbx_sim* sim, long long* per_label);

@@ -80,6 +80,8 @@ _sig("pbx_sim_restore", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong
 _sig("pbx_rng_normals", _i, [ctypes.c_uint, ctypes.c_longlong, _dp, ctypes.c_longlong])
 _sig("pbx_sim_propagate", _i, [ctypes.c_void_p, _d, _i, _i, _i])
 _sig("pbx_sim_propagate_out", _i, [ctypes.c_void_p, _d, _i, _i, _i, _dp, _lp])
+_sig("pbx_sim_run", _i, [ctypes.c_void_p, _d, ctypes.c_longlong, ctypes.c_longlong, _i, _i, _i, _lp, _dp, _i])
+_sig("pbx_sim_device_bytes", ctypes.c_longlong, [ctypes.c_void_p])
 _sig("pbx_sim_get_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_set_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_get_flux", _i, [ctypes.c_void_p, _lp])
@@ -105,6 +107,8 @@ EXPORTED = [
     "pbx_sim_set_attempt_batch", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
     "pbx_sim_restore", "pbx_sim_propagate",
     "pbx_sim_propagate_out",
+    "pbx_sim_run",
+    "pbx_sim_device_bytes",
     "pbx_sim_get_positions", "pbx_sim_set_positions", "pbx_sim_get_flux", "pbx_sim_get_matrix",
     "pbx_sim_get_diag", "pbx_sim_draw_count", "pbx_sim_last_attempts", "pbx_sim_sync",
     "pbx_sim_set_timing", "pbx_sim_get_timing",

@@ -209,6 +209,33 @@ class Box:
         if self.is_flux:
             self.net_flux = {lab: int(self._counts[k]) for k, lab in enumerate(self.mobile_labels)}
 
+    def run(self, dt, steps, j0=0, diff_freq=1, lub_freq=1, chol_freq=1, xyz_every=0):
+        """`steps` iterations of the BD.py:164-185 loop (propagate with its
+        cadence flags) inside the library, without returning per step.
+        Returns (flux, frames): flux[t, l] is label l's flux of step j0+t
+        (zeros without measure_flux); frames holds the positions before each
+        step j with j % xyz_every == 0 (None when xyz_every is 0)."""
+        self._push_host_positions()
+        nl = max(1, len(self.mobile_labels))
+        flux = np.zeros((int(steps), nl), dtype=np.int64)
+        frames = None
+        if xyz_every:
+            nf = sum(1 for j in range(j0, j0 + steps) if j % xyz_every == 0)
+            frames = np.zeros((nf, self.N, 3))
+        rc = lib.pbx_sim_run(self._sim, float(dt), int(j0), int(steps), int(diff_freq), int(lub_freq),
+                             int(chol_freq), flux.ctypes.data_as(_lib._lp),
+                             ptr(frames.reshape(-1)) if frames is not None else None, int(xyz_every))
+        _lib.check_rc(rc, "run")
+        _lib.check_rc(lib.pbx_sim_get_positions(self._sim, ptr(self._pos.reshape(-1))), "get_positions")
+        self._synced[:] = self._pos
+        if self.is_flux and steps:
+            self.net_flux = {lab: int(flux[-1, k]) for k, lab in enumerate(self.mobile_labels)}
+        return flux, frames
+
+    @property
+    def device_bytes(self):
+        return int(lib.pbx_sim_device_bytes(self._sim))
+
     # -- state queries ----------------------------------------------------
     @property
     def draw_count(self):

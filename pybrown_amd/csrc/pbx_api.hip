@@ -363,6 +363,7 @@ struct PbxSim {
     int last_attempts = 0;
     int have_D = 0, have_B = 0, have_Rff = 0;
     std::vector<void*> owned;
+    size_t device_bytes = 0;         // sum of the allocations in `owned`
     // optional stage timing with HIP events on the simulation stream
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -379,6 +380,7 @@ int sim_alloc(PbxSim* s, void** p, size_t bytes) {
         return PBX_ERR_OOM;
     }
     s->owned.push_back(*p);
+    s->device_bytes += bytes;
     return PBX_OK;
 }
 
@@ -1173,6 +1175,43 @@ extern "C" int pbx_sim_propagate_out(pbx_sim* s, double dt, int build_Dff, int b
     }
     return PBX_OK;
 }
+
+// The driver loop of BD.py:164-185 without file output, inside the library:
+// steps j = j0 .. j0+steps-1, each Box.propagate with the cadence flags of
+// BD.py:177.  flux_out[t * n_labels + l] = label l's flux of step j0+t (the
+// value BD.py writes on .flx line j0+t+1); xyz_out receives the positions
+// before every step j with j % xyz_every == 0 (BD.py:170-171), one N x 3
+// frame each, in order.
+extern "C" int pbx_sim_run(pbx_sim* s, double dt, long long j0, long long steps, int diff_freq, int lub_freq,
+                           int chol_freq, long long* flux_out, double* xyz_out, int xyz_every) {
+    pbx_clear_error();
+    if (!s || steps < 0 || j0 < 0 || diff_freq < 1 || lub_freq < 1 || chol_freq < 1 || (xyz_out && xyz_every < 1)) {
+        pbx_set_error(PBX_ERR_ARG, "run: bad arguments");
+        return PBX_ERR_ARG;
+    }
+    const int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    double* frame = xyz_out;
+    for (long long t = 0; t < steps; ++t) {
+        const long long j = j0 + t;
+        if (xyz_out && j % xyz_every == 0) {
+            if (t == 0)
+                TRY(pbx_sim_get_positions(s, frame));
+            else
+                memcpy(frame, s->pos_host, sizeof(double) * s->n);  // positions after step j-1
+            frame += s->n;
+        }
+        TRY(pbx_sim_propagate(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0));
+        if (flux_out) {
+            if (s->cfg.measure_flux)
+                memcpy(flux_out + t * nl, s->flux_host, sizeof(long long) * nl);
+            else
+                memset(flux_out + t * nl, 0, sizeof(long long) * nl);
+        }
+    }
+    return PBX_OK;
+}
+
+extern "C" long long pbx_sim_device_bytes(pbx_sim* s) { return s ? (long long)s->device_bytes : 0; }
 
 // Stage timing with HIP events recorded on the simulation's stream:
 // [0] mobility assembly (+ lubrication and inverses), [1] Cholesky,

@@ -604,8 +604,14 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
 
 // ---------------------------------------------------------------------------
 // Triangular inverse X = L^{-1} (lower) from a factored A and its diagonal
-// inverses W_i: X_ii = W_i; X_ij = -W_i * sum_{j<=m<i} L_im X_mj  (i > j).
-// One launch per tile row i, one workgroup per j < i.
+// inverses W_i, by recursive doubling over aligned tile blocks:
+//   level b = 1, 2, 4, ...: for each block pair B1 = [s, s+b),
+//   B2 = [s+b, min(s+2b, T)) (s a multiple of 2b) whose halves are already
+//   inverted,  X_21 = -X_22 (L_21 X_11).
+// Two launches per level: T1 = L_21 X_11 (tile (p, q) sums over k = q..b-1,
+// X_11 being lower triangular), stored in X's unused upper triangle at tile
+// (s+q, s+b+p); then X_21 = -X_22 T1 (k = 0..p).  About 2T tile products on
+// the critical path instead of T^2/2 for a row-by-row substitution.
 
 __global__ void copy_diag_inv_kernel(double* X, int ldx, const double* work) {
     const int i = blockIdx.x;
@@ -617,29 +623,56 @@ __global__ void copy_diag_inv_kernel(double* X, int ldx, const double* work) {
     }
 }
 
-__global__ __launch_bounds__(256) void trtri_row_kernel(const double* A, int ld, double* X, int ldx, int i,
-                                                        const double* work) {
+// acc += sum over k of op(A)_k op(B)_k, with the next pair of operand tiles
+// fetched into registers while the current product runs.  RKA/RKB select
+// the staging layout: true = the tile itself as that operand of the product
+// for A (op(A) = tile), false = the tile itself for B (op(B) = tile).
+template <class FA, class FB>
+__device__ __forceinline__ void chain_product(int k0, int k1, FA tileA, FB tileB, int lda, int ldb, double* sA,
+                                              double* sB, v4d acc[2][2]) {
+    if (k0 > k1) return;
+    v2d fa[8], fb[8];
+    stage_fetch(fa, tileA(k0), lda);
+    stage_fetch(fb, tileB(k0), ldb);
+    for (int k = k0; k <= k1; ++k) {
+        if (k > k0) __syncthreads();
+        stage_put<true>(sA, fa);   // op(A)(r, kk) = tileA[r][kk]
+        stage_put<false>(sB, fb);  // op(B)(kk, c) = tileB[kk][c]
+        __syncthreads();
+        if (k < k1) {
+            stage_fetch(fa, tileA(k + 1), lda);
+            stage_fetch(fb, tileB(k + 1), ldb);
+        }
+        mma64(sA, sB, acc);
+    }
+}
+
+template <int PHASE>
+__global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int ld, double* X, int ldx, int T, int b) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
-    const int j = blockIdx.x;  // j < i
+    const int pair = blockIdx.x / (b * b), rem = blockIdx.x - pair * b * b;
+    const int p = rem / b, q = rem - p * b;
+    const int s = pair * 2 * b;
+    const int h2 = (s + 2 * b < T ? s + 2 * b : T) - (s + b);
+    if (p >= h2) return;  // whole workgroup: B2 is shorter than b
+    auto Lt = [&](int i, int j) { return A + (size_t)i * T64 * ld + (size_t)j * T64; };
+    auto Xt = [&](int i, int j) { return X + (size_t)i * T64 * ldx + (size_t)j * T64; };
     v4d acc[2][2];
     acc_zero(acc);
-    for (int m = j; m < i; ++m) {
-        stage<true>(sA, A + (size_t)i * T64 * ld + (size_t)m * T64, ld);     // op(A) = L_im
-        stage<false>(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);  // op(B) = X_mj
-        __syncthreads();
-        mma64(sA, sB, acc);
-        __syncthreads();
+    if (PHASE == 0) {
+        // T1(p, q) = sum_{k=q}^{b-1} L(s+b+p, s+k) X(s+k, s+q)
+        chain_product(q, b - 1, [&](int k) { return Lt(s + b + p, s + k); },
+                      [&](int k) { return (const double*)Xt(s + k, s + q); }, ld, ldx, sA, sB, acc);
+        double* out = Xt(s + q, s + b + p);
+        acc_for_each(acc, [&](int r, int c, double v) { out[(size_t)r * ldx + c] = v; });
+    } else {
+        // X(s+b+p, s+q) = -sum_{k=0}^{p} X(s+b+p, s+b+k) T1(k, q)
+        chain_product(0, p, [&](int k) { return (const double*)Xt(s + b + p, s + b + k); },
+                      [&](int k) { return (const double*)Xt(s + q, s + b + k); }, ldx, ldx, sA, sB, acc);
+        double* out = Xt(s + b + p, s + q);
+        acc_for_each(acc, [&](int r, int c, double v) { out[(size_t)r * ldx + c] = -v; });
     }
-    // X_ij = -W_i * S
-    acc_for_each(acc, [&](int r, int c, double v) { sB[r * SP + c] = v; });  // s[k=r][c] = S[r][c]
-    stage<true>(sA, work + (size_t)i * T64 * T64, T64);                        // op(A) = W_i
-    __syncthreads();
-    v4d out[2][2];
-    acc_zero(out);
-    mma64(sA, sB, out);
-    double* Xij = X + (size_t)i * T64 * ldx + (size_t)j * T64;
-    acc_for_each(out, [&](int r, int c, double v) { Xij[(size_t)r * ldx + c] = -v; });
 }
 
 // Y = scale * X^T X, lower tiles computed, both triangles written:
@@ -870,10 +903,14 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
     int rc = pbx_potrf(A, np, ld, work, info_dev, s);
     if (rc) return rc;
     const int T = np / T64;
-    PBX_HIP(hipMemsetAsync(X, 0, sizeof(double) * (size_t)np * ld, s));
+    // every lower tile of X is written below before lauum reads it (the
+    // upper tiles hold the recursion's partial products and are not read)
     hipLaunchKernelGGL(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
-    for (int i = 1; i < T; ++i)
-        hipLaunchKernelGGL(trtri_row_kernel, dim3(i), dim3(256), 0, s, A, ld, X, ld, i, work);
+    for (int b = 1; b < T; b *= 2) {
+        const int npairs = (T - b + 2 * b - 1) / (2 * b);
+        hipLaunchKernelGGL(trtri_level_kernel<0>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+        hipLaunchKernelGGL(trtri_level_kernel<1>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+    }
     hipLaunchKernelGGL(lauum_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
