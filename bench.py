@@ -64,7 +64,7 @@ def main():
     ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-6; 0 = library default)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
-    ap.add_argument("--requests", type=int, default=48, help="c5: requests taken from the head of the stream")
+    ap.add_argument("--requests", type=int, default=64, help="c5: requests taken from the head of the stream")
     ap.add_argument("--slots", type=int, default=16, help="c5: concurrent simulations")
     ap.add_argument("--request-steps", type=int, default=200, help="c5: steps per request")
     args = ap.parse_args()
@@ -178,8 +178,9 @@ def serve_bench(args):
     res = Server(slots=args.slots, hbm_budget=budget).run(reqs, realtime=True)
     served = summary(res)
     print("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served), file=sys.stderr)
-    bad = [r.status for r in res if r.status != "ok"]
-    assert not bad, bad[:3]
+    failed = [(r.rid, r.N, r.mode, r.status) for r in res if r.status != "ok"]
+    if failed:
+        print("c5 failed requests: %s" % failed, file=sys.stderr)
     out = {
         "metric": "served sims/s + p99 latency (request stream)",
         "value": served["sims_per_s"],
@@ -194,11 +195,13 @@ def serve_bench(args):
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "request stream: first %d requests, N log-uniform 32..3000, phi 0.02..0.30, "
-                               "60%% rpy_smith / 25%% rpy (open box) / 15%% rpy_smith_lub, ermak dt=1ps, "
+                               "60%% rpy_smith / 25%% rpy (open box) ermak / 15%% rpy_smith_lub midpoint, dt=1ps, "
                                "%d steps each, flux x=0, Poisson arrivals at 1.5x the burst throughput"
                                % (args.requests, args.request_steps),
                    "requests": args.requests, "slots": args.slots, "hbm_budget_bytes": budget,
                    "arrival_rate_per_s": rate, "gpu_max_hw_queues": os.environ["GPU_MAX_HW_QUEUES"]},
+        "completed": served["completed"],
+        "failed": served["failed"],
         "p50_s": served["p50_s"],
         "p99_s": served["p99_s"],
         "one_at_a_time_sims_per_s": base["sims_per_s"],

@@ -25,8 +25,8 @@ The request stream follows the served benchmark: request k has seed k;
 N = round(exp(U(ln 32, ln 3000))), volume fraction U(0.02, 0.30), radius
 10 A; 60 % periodic RPY with Ewald sums (rpy_smith, alpha = sqrt(pi),
 m = n = 2), 25 % free-space RPY, 15 % Ewald RPY with Jeffrey-Onishi
-lubrication (cutoff 0.5, Cichocki projection); Ermak, dt = 1 ps, flux
-through the plane x = 0; Poisson arrivals.  Three choices keep every request
+lubrication (cutoff 0.5, Cichocki projection); dt = 1 ps, flux through the
+plane x = 0; Poisson arrivals.  Three choices keep every request
 runnable, as the reference itself would not be:
   - free-space RPY runs in an open box (box_length far beyond any
     displacement: no images, no wrap).  With minimum-image pair vectors in the
@@ -37,13 +37,17 @@ runnable, as the reference itself would not be:
     leaves near-contact pairs, and at these densities almost no move of all
     beads is overlap-free (the same reason as for the N = 4000 workload).
     RPY and its Ewald sum stay positive definite for overlapping beads;
-  - with lubrication they are on, as in the N = 500 lubrication workload
-    (overlap_treshold 0.1 A, placement with a 0.1 A minimum gap): the
-    Jeffrey-Onishi series is undefined for overlapping pairs (an overlapped
-    pair turns D into NaN, in the reference as here), and lubrication slows
-    the approach of near pairs, so rejections stay rare.  max_move_attempts
-    is 10000, so a request that cannot move fails instead of stalling a
-    slot.
+  - lubrication requests use the midpoint scheme of the N = 500
+    lubrication workload (whose Ewald-plus-JO definition "lubrication" takes
+    here) with overlap checks on (placement with a 0.1 A minimum gap,
+    overlap_treshold 0, max_move_attempts 10000 so a request that cannot
+    move fails instead of stalling a slot).  The Jeffrey-Onishi series is
+    undefined for overlapping pairs (an overlapped pair turns D into NaN, in
+    the reference as here), and with Ermak steps, which lack the divergence
+    term, lubricated pairs drift into contact: measured on requests 2, 19 and
+    23 of the stream, rejections grow to hundreds of attempts per step and
+    request 19 exhausts 10000 attempts, while midpoint steps need one
+    attempt each.
 """
 import math
 import threading
@@ -111,8 +115,8 @@ def make_request(p, steps=200, dt=1.0, a=10.0):
     if p["mode"] in ("rpy_smith", "rpy_smith_lub"):
         inp.update(ewald_alpha=math.sqrt(math.pi), ewald_real=2, ewald_imag=2)
     if lub:
-        inp.update(lubrication_cutoff=0.5, cichocki_correction=True, check_overlaps=True, overlap_treshold=0.1,
-                   max_move_attempts=10000)
+        inp.update(lubrication_cutoff=0.5, cichocki_correction=True, check_overlaps=True, overlap_treshold=0.0,
+                   max_move_attempts=10000, propagation_scheme="midpoint", m_midpoint=100)
     if p["mode"] == "rpy":
         inp["box_length"] = OPEN_BOX
     return Request(p["rid"], xyz, radii, inp, dt, steps, p["arrival"])
@@ -153,6 +157,12 @@ def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attem
         S = max(2, (T + 11) // 12)  # column slab width of the batched attempt products
         batch = 8 * (33 * n + N) + 4 * 32 + (8 * (-(-T // S)) * np_ * 32 if mode != "nohi" else 0)
     return 8 * np_ * np_ * mats + works * work + ring + vecs + batch + (1 << 20)
+
+
+def request_footprint(req):
+    """footprint_bytes for a request's own keywords."""
+    return footprint_bytes(req.N, req.mode, req.inp.get("propagation_scheme", "ermak"),
+                           bool(req.inp.get("check_overlaps", True)))
 
 
 class Result:
@@ -205,8 +215,7 @@ class Server:
         status 'rejected: footprint'."""
         order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival if realtime else 0.0, i))
         results = [Result(r.rid, r.N, r.mode) for r in requests]
-        fps = [footprint_bytes(r.N, r.mode, r.inp.get("propagation_scheme", "ermak"),
-                               bool(r.inp.get("check_overlaps", True))) for r in requests]
+        fps = [request_footprint(r) for r in requests]
         cv = threading.Condition()
         state = {"free": self.slots, "mem": 0.0, "done": 0}
         work = []

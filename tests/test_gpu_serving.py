@@ -4,7 +4,7 @@ request's result is the same alone and beside other simulations."""
 import numpy as np
 import pytest
 
-from pybrown_amd.serving import Server, footprint_bytes, make_request, run_one, stream_params, summary
+from pybrown_amd.serving import Server, make_request, request_footprint, run_one, stream_params, summary
 
 pytestmark = pytest.mark.gpu
 
@@ -48,6 +48,20 @@ def test_run_equals_stepwise_propagation(mode):
         b.close()
 
 
+def test_run_reports_a_failing_factorisation():
+    # free-space RPY with minimum-image vectors in a dense periodic box is not
+    # positive definite: the first step's factorisation fails
+    req = _request("rpy", 300, steps=5)
+    req.inp["box_length"] = 150.0
+    req.xyz = req.xyz - 150.0 * np.round(req.xyz / 150.0)
+    box = _box(req)
+    try:
+        with pytest.raises(np.linalg.LinAlgError, match="not positive definite"):
+            box.run(req.dt, req.steps)
+    finally:
+        box.close()
+
+
 @pytest.mark.parametrize("mode,N", [("rpy_smith", 40), ("rpy_smith", 700), ("rpy", 333), ("rpy_smith_lub", 250)])
 def test_footprint_bounds_the_allocation(mode, N):
     req = _request(mode, N, steps=1)
@@ -56,7 +70,7 @@ def test_footprint_bounds_the_allocation(mode, N):
         held = box.device_bytes
     finally:
         box.close()
-    est = footprint_bytes(N, mode, check_overlaps=req.inp["check_overlaps"])
+    est = request_footprint(req)
     print("%s N=%d held %d estimate %d" % (mode, N, held, est))
     assert held <= est <= held * 1.05 + (2 << 20)
 

@@ -49,6 +49,7 @@ def test_make_request_boxes():
             assert abs(40 * 4 / 3 * np.pi * 1000.0 / L ** 3 - p["phi"]) < 1e-9
         assert ("ewald_alpha" in r.inp) == (mode != "rpy")
         assert ("lubrication_cutoff" in r.inp) == (mode == "rpy_smith_lub")
+        assert r.inp.get("propagation_scheme", "ermak") == ("midpoint" if mode == "rpy_smith_lub" else "ermak")
         if mode == "rpy_smith_lub":
             from oracle.orc import Oracle
             P = Oracle().pointer_pbc_matrix(r.xyz, r.inp["box_length"])
