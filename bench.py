@@ -62,6 +62,8 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-6; 0 = library default)")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="noise factor: FP64, or FP32 trailing updates with FP64 diagonal tiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--requests", type=int, default=64, help="c5: requests taken from the head of the stream")
@@ -102,7 +104,7 @@ def main():
     xyz, a, inp, dt, _ = workload(args.workload, seed=0)
     N = len(a)
     n = 3 * N
-    box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp, xi_pool_attempts=64)
+    box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp, xi_pool_attempts=64, precision=args.precision)
     for _ in range(args.warmup):
         box.propagate(dt)
     torch.cuda.synchronize()
@@ -122,6 +124,10 @@ def main():
     print("stage ms/step: assembly %.3f  cholesky %.3f  propagation %.3f  (over %d steps; wall %.3f)"
           % (stage[0], stage[1], stage[2], nt, ms), file=sys.stderr)
 
+    accuracy = None
+    if args.precision == "mixed":
+        accuracy = mixed_accuracy(box, xyz, a, inp, dt)
+        print("mixed factor: %s" % accuracy, file=sys.stderr)
     out = {
         "metric": METRIC[args.workload],
         "value": 1e3 / ms,
@@ -133,15 +139,43 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 + f32 (trailing Cholesky updates)",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD_TEXT[args.workload], "N": N, "box_length": round(inp["box_length"], 4)},
+        "config": {"workload": WORKLOAD_TEXT[args.workload], "N": N, "box_length": round(inp["box_length"], 4),
+                   "precision": args.precision},
         "stage_ms": {"assembly": stage[0], "cholesky": stage[1], "propagation": stage[2]},
-        "roofline": roofline(args.workload, N, stage),
+        "roofline": roofline(args.workload, N, stage, args.precision),
     }
+    if accuracy is not None:
+        out["accuracy"] = accuracy
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
     print(json.dumps(out))
+
+
+def mixed_accuracy(box, xyz, a, inp, dt, steps=3):
+    """Errors of the mixed-precision engine against an FP64 twin: the
+    residual ||B B^T - D||_F / ||D||_F of both factors (on the device) and the
+    relative error of one step's displacement from identical positions and
+    noise, the largest over `steps` steps."""
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    ref = Box([Bead(p, r) for p, r in zip(box._pos.copy(), a)], dict(inp, seed=inp["seed"] + 1))
+    mix = Box([Bead(p, r) for p, r in zip(box._pos.copy(), a)], dict(inp, seed=inp["seed"] + 1), precision="mixed")
+    try:
+        errs = []
+        for _ in range(steps):
+            start = ref._pos.copy()
+            mix._pos[:] = start
+            ref.propagate(dt)
+            mix.propagate(dt)
+            d_ref = ref._pos - start
+            errs.append(float(np.linalg.norm(mix._pos - start - d_ref) / np.linalg.norm(d_ref)))
+        return {"factor_residual": mix.factor_residual(), "fp64_factor_residual": ref.factor_residual(),
+                "step_displacement_rel_error": max(errs), "steps_compared": steps}
+    finally:
+        ref.close()
+        mix.close()
 
 
 def serve_bench(args):
@@ -222,6 +256,7 @@ WORKLOAD_TEXT = {
 }
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 (vector and matrix)
 
 
 def smith_flops(N, m=2, nrec=2):
@@ -250,17 +285,20 @@ def measured_traffic(kernel_key):
     return None if entry is None else entry.get("hbm_bytes")
 
 
-def roofline(name, N, stage):
-    """Roofline entry for the dominant stage of the step (by measured time)."""
+def roofline(name, N, stage, precision="fp64"):
+    """Roofline entry for the dominant stage of the step (by measured time).
+    The mixed factorisation's bulk runs on the FP32 matrix cores, so its
+    fraction is against the FP32 peak."""
     n = 3 * N
     k = int(np.argmax(stage))
     if k == 1:
         from pybrown_amd._lib import lib
         flops = n ** 3 / 3.0
         ach = flops / (stage[1] * 1e-3) / 1e12
-        return {"kernel": "cholesky (potrf, n=%d)" % n, "bound": "mfma", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic("cholesky_%s_n%d_v%d" % (name, n, lib.pbx_get_potrf_variant())),
+        peak = FP64_PEAK_TFLOPS if precision == "fp64" else FP32_PEAK_TFLOPS
+        key = "cholesky_%s_n%d_v%d" % (name, n, lib.pbx_get_potrf_variant()) + ("" if precision == "fp64" else "_mixed")
+        return {"kernel": "cholesky (potrf, n=%d, %s)" % (n, precision), "bound": "mfma", "achieved": ach,
+                "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": measured_traffic(key),
                 "algorithmic": "n^3/3 = %.3e flop per factorisation" % flops}
     if k == 0:
         flops = smith_flops(N) if name in ("c2", "c4", "c3") else 30.0 * N * (N - 1) / 2

@@ -64,6 +64,11 @@ int check_overlaps(double* positions, double* as, int number_of_beads, double bo
  * to zero.  Returns 0, or k > 0 when the pivot at column k-1 is not
  * positive (numpy raises LinAlgError there), or a negative error. */
 int pbx_cholesky(const double* A, int n, double* L);
+/* As pbx_cholesky, with the trailing updates on FP32 matrix cores (operands
+ * rounded to FP32, FP32 accumulation added into the FP64 matrix) and the
+ * diagonal tiles factored in FP64.  No reference counterpart: the mixed-
+ * precision noise factor measured against the FP64 one. */
+int pbx_cholesky_mixed(const double* A, int n, double* L);
 
 /* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
@@ -108,7 +113,9 @@ typedef struct pbx_config {
     int n_labels;               /* distinct bead labels (flux counters)     */
     long long max_move_attempts;
     int xi_pool_attempts;       /* attempts (3N draws) per noise chunk (0 = 64) */
-    int reserved;
+    int factor_precision;       /* noise factor: 0 = FP64 (default), 1 = mixed:
+                                   trailing updates on FP32 matrix cores,
+                                   diagonal tiles and solves in FP64        */
     double box_length, T, viscosity;
     double ewald_alpha;
     double lubrication_cutoff;
@@ -174,6 +181,10 @@ int pbx_sim_run(pbx_sim* sim, double dt, long long j0, long long steps, int diff
 /* Device memory held by the simulation, in bytes (what admission control
  * counts against an HBM budget). */
 long long pbx_sim_device_bytes(pbx_sim* sim);
+/* ||D - B B^T||_F / ||D||_F for the simulation's current diffusion matrix
+ * and noise factor, computed on the device (FP64).  No reference
+ * counterpart: the accuracy measure of the noise factor. */
+int pbx_sim_factor_residual(pbx_sim* sim, double* rel);
 
 int pbx_sim_get_positions(pbx_sim* sim, double* xyz);
 int pbx_sim_set_positions(pbx_sim* sim, const double* xyz);

@@ -37,7 +37,7 @@ class Config(ctypes.Structure):
     _fields_ = [
         ("hydrodynamics", _i), ("scheme", _i), ("ewald_real", _i), ("ewald_imag", _i),
         ("cichocki_correction", _i), ("check_overlaps", _i), ("measure_flux", _i), ("n_labels", _i),
-        ("max_move_attempts", ctypes.c_longlong), ("xi_pool_attempts", _i), ("reserved", _i),
+        ("max_move_attempts", ctypes.c_longlong), ("xi_pool_attempts", _i), ("factor_precision", _i),
         ("box_length", _d), ("T", _d), ("viscosity", _d), ("ewald_alpha", _d),
         ("lubrication_cutoff", _d), ("m_midpoint", _d), ("overlap_treshold", _d),
         ("flux_normal", _d * 3), ("flux_plane_point", _d * 3), ("external_force", _d * 3),
@@ -64,6 +64,7 @@ _sig("pointer_immobile_pbc_matrix", None, [_dp, _dp, _i, _i, _d, _dp])
 _sig("check_overlaps", _i, [_dp, _dp, _i, _d, _d, _ip])
 # dense
 _sig("pbx_cholesky", _i, [_dp, _i, _dp])
+_sig("pbx_cholesky_mixed", _i, [_dp, _i, _dp])
 _sig("pbx_spd_inverse_host", _i, [_dp, _i, _d, _dp])
 _sig("pbx_gemm_nt", _i, [_dp, _dp, _dp, _i, _i, _i])
 _sig("pbx_set_potrf_variant", _i, [_i])
@@ -82,6 +83,7 @@ _sig("pbx_sim_propagate", _i, [ctypes.c_void_p, _d, _i, _i, _i])
 _sig("pbx_sim_propagate_out", _i, [ctypes.c_void_p, _d, _i, _i, _i, _dp, _lp])
 _sig("pbx_sim_run", _i, [ctypes.c_void_p, _d, ctypes.c_longlong, ctypes.c_longlong, _i, _i, _i, _lp, _dp, _i])
 _sig("pbx_sim_device_bytes", ctypes.c_longlong, [ctypes.c_void_p])
+_sig("pbx_sim_factor_residual", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_get_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_set_positions", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_get_flux", _i, [ctypes.c_void_p, _lp])
@@ -100,7 +102,8 @@ _sig("pbx_version", ctypes.c_char_p, [])
 
 EXPORTED = [
     "RPY_M_matrix", "RPY_Smith_M_matrix", "JO_2B_R_matrix", "JO_R_lubrication_correction_matrix",
-    "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_spd_inverse_host", "pbx_gemm_nt",
+    "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_cholesky_mixed",
+    "pbx_spd_inverse_host", "pbx_gemm_nt",
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
@@ -109,6 +112,7 @@ EXPORTED = [
     "pbx_sim_propagate_out",
     "pbx_sim_run",
     "pbx_sim_device_bytes",
+    "pbx_sim_factor_residual",
     "pbx_sim_get_positions", "pbx_sim_set_positions", "pbx_sim_get_flux", "pbx_sim_get_matrix",
     "pbx_sim_get_diag", "pbx_sim_draw_count", "pbx_sim_last_attempts", "pbx_sim_sync",
     "pbx_sim_set_timing", "pbx_sim_get_timing",

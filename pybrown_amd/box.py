@@ -51,7 +51,7 @@ _UNSUPPORTED = ("walls", "external_force_region", "measure_concentration", "reac
 class Box:
     """Simulation box (pyBrown/box.py:30-109)."""
 
-    def __init__(self, beads, input_data, xi_pool_attempts=64, noise="native", attempt_batch=0):
+    def __init__(self, beads, input_data, xi_pool_attempts=64, noise="native", attempt_batch=0, precision="fp64"):
         self.inp = dict(DEFAULTS)
         self.inp.update(input_data)
         inp = self.inp
@@ -69,7 +69,10 @@ class Box:
 
         if any(b.bonded_with or b.angled_with for b in beads):
             raise NotImplementedError("bonds and angles (force fields) are outside the hydrodynamic step")
-        self._init_args = (xi_pool_attempts, noise, attempt_batch)
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self._init_args = (xi_pool_attempts, noise, attempt_batch, precision)
         self.beads = beads
         self.box_length = inp["box_length"]
         # box.py:621-629
@@ -136,6 +139,7 @@ class Box:
         cfg.n_labels = max(1, len(self.mobile_labels))
         cfg.max_move_attempts = int(inp["max_move_attempts"])
         cfg.xi_pool_attempts = int(xi_pool_attempts)
+        cfg.factor_precision = 1 if precision == "mixed" else 0
         cfg.box_length = float(self.box_length)
         cfg.T = float(self.T)
         cfg.viscosity = float(self.viscosity)
@@ -232,6 +236,13 @@ class Box:
             self.net_flux = {lab: int(flux[-1, k]) for k, lab in enumerate(self.mobile_labels)}
         return flux, frames
 
+    def factor_residual(self):
+        """||D - B B^T||_F / ||D||_F of the current matrices (on the device)."""
+        import ctypes
+        v = ctypes.c_double(0.0)
+        _lib.check_rc(lib.pbx_sim_factor_residual(self._sim, ctypes.byref(v)), "factor_residual")
+        return v.value
+
     @property
     def device_bytes(self):
         return int(lib.pbx_sim_device_bytes(self._sim))
@@ -288,8 +299,10 @@ class Box:
                 "snapshot": self.snapshot()}
 
     def __setstate__(self, st):
-        xi, noise, batch = st["init"]
-        self.__init__(st["beads"], st["input_data"], xi_pool_attempts=xi, noise=noise, attempt_batch=batch)
+        xi, noise, batch = st["init"][:3]
+        precision = st["init"][3] if len(st["init"]) > 3 else "fp64"
+        self.__init__(st["beads"], st["input_data"], xi_pool_attempts=xi, noise=noise, attempt_batch=batch,
+                      precision=precision)
         self.pseudorandom_number_generator = st["rng"]
         self.restore(st["snapshot"])
         if st["net_flux"] is not None:

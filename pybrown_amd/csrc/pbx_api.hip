@@ -244,7 +244,7 @@ extern "C" int check_overlaps(double* positions, double* as, int number_of_beads
 // Lower Cholesky of the n x n row-major SPD matrix A (box.py:554).  L gets
 // the factor with a zero upper triangle.  Returns 0, k>0 (non-positive pivot
 // at column k-1) or a negative error.
-extern "C" int pbx_cholesky(const double* A, int n, double* L) {
+static int cholesky_host(const double* A, int n, double* L, int mixed) {
     pbx_clear_error();
     if (n <= 0) return PBX_ERR_ARG;
     int np = (int)pbx_round_up(n, 64), ld = np;
@@ -256,7 +256,7 @@ extern "C" int pbx_cholesky(const double* A, int n, double* L) {
     PBX_HIP(hipMemcpy2D(dA.p, sizeof(double) * ld, A, sizeof(double) * n, sizeof(double) * n, n,
                         hipMemcpyHostToDevice));
     TRY(pbx_fill_pad(dA.as<double>(), n, np, ld, 0));
-    TRY(pbx_potrf(dA.as<double>(), np, ld, dW.as<double>(), dI.as<int>(), 0));
+    TRY(pbx_potrf(dA.as<double>(), np, ld, dW.as<double>(), dI.as<int>(), 0, mixed));
     int info = 0;
     PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
     PBX_HIP(hipMemcpy2D(L, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
@@ -270,6 +270,13 @@ extern "C" int pbx_cholesky(const double* A, int n, double* L) {
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
     return info;
 }
+
+extern "C" int pbx_cholesky(const double* A, int n, double* L) { return cholesky_host(A, n, L, 0); }
+
+// The same factor with the trailing updates on FP32 matrix cores (operands
+// rounded to FP32, FP32 accumulation added into the FP64 matrix); diagonal
+// tiles factored and inverted in FP64.
+extern "C" int pbx_cholesky_mixed(const double* A, int n, double* L) { return cholesky_host(A, n, L, 1); }
 
 // X = scale * inv(A) for SPD A (box.py:514,524,537 np.linalg.inv call sites)
 extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double* X) {
@@ -472,7 +479,7 @@ int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
 // box.py:546-556 _decompose_D_matrix
 int factor(PbxSim* s) {
     TRY(pbx_copy_lower(s->D, s->B, s->np, s->ld, s->stream));
-    TRY(pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream));
+    TRY(pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision));
     return PBX_OK;
 }
 
@@ -735,6 +742,10 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     if (cfg->hydrodynamics < PBX_NOHI || cfg->hydrodynamics > PBX_RPY_SMITH_LUB || cfg->scheme < 0 ||
         cfg->scheme > 1) {
         pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: unknown hydrodynamics/scheme");
+        return PBX_ERR_ARG;
+    }
+    if (cfg->factor_precision != 0 && cfg->factor_precision != 1) {
+        pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: factor_precision must be 0 (FP64) or 1 (mixed)");
         return PBX_ERR_ARG;
     }
     PbxSim* s = new PbxSim();
@@ -1209,6 +1220,18 @@ extern "C" int pbx_sim_run(pbx_sim* s, double dt, long long j0, long long steps,
         }
     }
     return PBX_OK;
+}
+
+// ||D - B B^T||_F / ||D||_F of the current diffusion matrix and noise factor.
+extern "C" int pbx_sim_factor_residual(pbx_sim* s, double* rel) {
+    pbx_clear_error();
+    if (!s || !rel || s->cfg.hydrodynamics == PBX_NOHI || !s->have_D || !s->have_B) {
+        pbx_set_error(PBX_ERR_STATE, "factor_residual: needs a dense D and its factor");
+        return PBX_ERR_STATE;
+    }
+    DevBuf acc;
+    TRY(acc.alloc(2 * sizeof(double)));
+    return pbx_factor_residual(s->D, s->B, s->n, s->np, s->ld, acc.as<double>(), rel, s->stream);
 }
 
 extern "C" long long pbx_sim_device_bytes(pbx_sim* s) { return s ? (long long)s->device_bytes : 0; }

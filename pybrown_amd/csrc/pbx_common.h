@@ -105,7 +105,12 @@ int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a
 int pbx_launch_jo_2b(double ai, double aj, double rx, double ry, double rz, double* out18, hipStream_t s);
 
 // dense linear algebra (pbx_dense.hip); np must be a multiple of 64
-int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s);
+// mixed != 0: the trailing updates on FP32 matrix cores (operands rounded to
+// FP32, FP32 accumulation, added into the FP64 matrix); diagonal tiles and
+// solves in FP64.
+int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, int mixed = 0);
+int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld, double* acc, double* rel,
+                        hipStream_t s);
 int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
                     double scale);
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);

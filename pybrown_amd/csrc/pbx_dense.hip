@@ -350,6 +350,86 @@ __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int
     }
 }
 
+// FP32 bulk: S += sum_m L_im L_jm^T with the operands rounded to FP32 in LDS
+// (the FP64 sA/sB buffers hold the FP32 tiles).
+__device__ __forceinline__ void panel_update32(const double* A, int ld, int i, int j, int m0, int P, float* sA,
+                                               float* sB, v4f S[2][2]) {
+    const double* Ai = A + (size_t)i * T64 * ld;
+    const double* Aj = A + (size_t)j * T64 * ld;
+    for (int m = m0; m < m0 + P; ++m) {
+        if (m > m0) __syncthreads();
+        stage_f32t(sA, Ai + (size_t)m * T64, ld);
+        if (i != j) stage_f32t(sB, Aj + (size_t)m * T64, ld);
+        __syncthreads();
+        mma64_f32(sA, i == j ? sA : sB, S);
+    }
+}
+
+// Tile (i, j) less panels m0 .. m0+P-1, FP64 or with the FP32 bulk (MIXED),
+// at this thread's FP64 accumulator positions (acc_row, acc_col), so stores
+// and later loads of the tile stay with the thread that owns each value.
+// The FP32 accumulators hold other rows (acc32_row) and are re-laid through
+// LDS.  Ends block-synchronised when the FP32 path ran; otherwise other waves
+// may still read sA/sB.
+struct Updated {
+    double v[16];
+    __device__ __forceinline__ int row(int q) const { return acc_row(q >> 3, q & 3); }
+    __device__ __forceinline__ int col(int q) const { return acc_col((q >> 2) & 1); }
+};
+
+template <bool MIXED>
+__device__ __forceinline__ void update_tile(const double* A, int ld, int i, int j, int m0, int P, double* sA,
+                                            double* sB, Updated& u) {
+    const double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
+    if (MIXED && P > 0) {
+        double cv[16];
+        acc32_load(Aij, ld, cv);
+        v4f S[2][2];
+        acc32_zero(S);
+        panel_update32(A, ld, i, j, m0, P, (float*)sA, (float*)sB, S);
+        __syncthreads();
+        int q = 0;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v, ++q)
+                    sA[acc32_row(x, v) * SP + acc_col(y)] = cv[q] - (double)S[x][y][v];
+        __syncthreads();
+#pragma unroll
+        for (int q2 = 0; q2 < 16; ++q2) u.v[q2] = sA[u.row(q2) * SP + u.col(q2)];
+        __syncthreads();
+        return;
+    }
+    acc_load(Aij, ld, u.v);
+    v4d S[2][2];
+    acc_zero(S);
+    if (P > 0) panel_update(A, ld, i, j, m0, P, sA, sB, S);
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v, ++q) u.v[q] -= S[x][y][v];
+}
+
+// X = the updated tile (lower part only when `lower`)
+__device__ __forceinline__ void store_updated(double* X, int ld, const Updated& u, bool lower) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = u.row(q), c = u.col(q);
+        if (!lower || c <= r) X[(size_t)r * ld + c] = u.v[q];
+    }
+}
+
+// s[c][r] = the updated tile (op(A) layout)
+__device__ __forceinline__ void put_updated_t(double* s, const Updated& u) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s[u.col(q) * SP + u.row(q)] = u.v[q];
+}
+
 // s[c][r] = cv - S at this thread's accumulator positions (op(A) layout)
 __device__ __forceinline__ void put_diff_t(double* s, const double cv[16], v4d S[2][2]) {
     int q = 0;
@@ -413,7 +493,7 @@ __device__ __forceinline__ void store_l(const double* sL, double* Akk, int ld) {
     }
 }
 
-template <int V, int OCC>
+template <int V, int OCC, bool MIXED>
 __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
                                                         int upd_target, double* work, double* scr, int* info,
                                                         int* la) {
@@ -472,16 +552,12 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     if (b <= nh) {
         // tiles (c0+1, c0) (to the scratch tile) and (c0+1, c0+1)
         const int i = c0 + 1, j = (b == 1) ? c0 : c0 + 1;
-        double* Aij = tile(i, j);
-        double cv[16];
-        acc_load(Aij, ld, cv);
-        v4d S[2][2];
-        acc_zero(S);
-        if (P > 0) panel_update(A, ld, i, j, c0 - P, P, sA, sB, S);
+        Updated u;
+        update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
         if (b == 1)
-            store_diff(scr, T64, cv, S, false);
+            store_updated(scr, T64, u, false);
         else
-            store_diff(Aij, ld, cv, S, true);
+            store_updated(tile(i, j), ld, u, true);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
         if (threadIdx.x == 0) __hip_atomic_fetch_add(la + LA_UPD, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
@@ -491,15 +567,10 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         const int i = c0 + nc + (b - 1 - nh);
         double* A0 = tile(i, c0);
         if (nc == 1) {
-            double cv[16];
-            acc_load(A0, ld, cv);
-            v4d S[2][2];
-            acc_zero(S);
-            if (P > 0) {
-                panel_update(A, ld, i, c0, c0 - P, P, sA, sB, S);
-                __syncthreads();
-            }
-            put_diff_t(sA, cv, S);
+            Updated u;
+            update_tile<MIXED>(A, ld, i, c0, c0 - P, P, sA, sB, u);
+            if (P > 0) __syncthreads();
+            put_updated_t(sA, u);
             if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
             stage<true>(sB, W0, T64);
             __syncthreads();
@@ -513,37 +584,26 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         double* Aii = tile(i, i);
         const bool diag = (i == c0 + 2);  // also finishes tile (i, i) for the next launch
         if (diag && P > 0) {
-            double cv[16];
-            acc_load(Aii, ld, cv);
-            v4d S[2][2];
-            acc_zero(S);
-            panel_update(A, ld, i, i, c0 - P, P, sA, sB, S);
-            store_diff(Aii, ld, cv, S, true);
+            Updated u;
+            update_tile<MIXED>(A, ld, i, i, c0 - P, P, sA, sB, u);
+            store_updated(Aii, ld, u, true);
             __syncthreads();
         }
         // tile (i, c0+1) through the panels, back in place (each thread later
         // reads back the values it wrote)
         if (P > 0) {
-            double cv[16];
-            acc_load(A1, ld, cv);
-            v4d S[2][2];
-            acc_zero(S);
-            panel_update(A, ld, i, c0 + 1, c0 - P, P, sA, sB, S);
-            store_diff(A1, ld, cv, S, false);
+            Updated u;
+            update_tile<MIXED>(A, ld, i, c0 + 1, c0 - P, P, sA, sB, u);
+            store_updated(A1, ld, u, false);
             __syncthreads();
         }
         // tile (i, c0): update, then L_{i,c0} = C W_{c0}^T
         v4d L0[2][2];
         {
-            double cv[16];
-            acc_load(A0, ld, cv);
-            v4d S[2][2];
-            acc_zero(S);
-            if (P > 0) {
-                panel_update(A, ld, i, c0, c0 - P, P, sA, sB, S);
-                __syncthreads();
-            }
-            put_diff_t(sA, cv, S);
+            Updated u;
+            update_tile<MIXED>(A, ld, i, c0, c0 - P, P, sA, sB, u);
+            if (P > 0) __syncthreads();
+            put_updated_t(sA, u);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
         stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
@@ -593,13 +653,9 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     int ii, jj;
     tri_decode(b - 1 - nh - nw + (nc == 2 ? 1 : 0), ii, jj);
     const int i = c0 + nc + ii, j = c0 + nc + jj;
-    double* Aij = tile(i, j);
-    double cv[16];
-    acc_load(Aij, ld, cv);
-    v4d S[2][2];
-    acc_zero(S);
-    panel_update(A, ld, i, j, c0 - P, P, sA, sB, S);
-    store_diff(Aij, ld, cv, S, i == j);
+    Updated u;
+    update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
+    store_updated(tile(i, j), ld, u, i == j);
 }
 
 // ---------------------------------------------------------------------------
@@ -672,6 +728,64 @@ __global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int l
                       [&](int k) { return (const double*)Xt(s + q, s + b + k); }, ldx, ldx, sA, sB, acc);
         double* out = Xt(s + b + p, s + q);
         acc_for_each(acc, [&](int r, int c, double v) { out[(size_t)r * ldx + c] = -v; });
+    }
+}
+
+// Factorisation residual: acc[0] += ||D - B B^T||^2, acc[1] += ||D||^2 over
+// the leading n x n block (B lower; each strictly lower tile counts twice).
+// One workgroup per lower tile (i, j): E_ij = D_ij - sum_{k <= j} B_ik B_jk^T.
+// Diagonal tiles of B hold stale values above the diagonal (the factor is
+// written over the lower triangle only), so those are zeroed once staged.
+__device__ __forceinline__ void zero_upper_t(double* s) {
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (c > r) s[c * SP + r] = 0.0;  // op layout: s[col][row]
+    }
+}
+
+__global__ __launch_bounds__(256) void residual_kernel(const double* D, const double* B, int ld, int n,
+                                                       double* acc) {
+    __shared__ double sA[T64 * SP];
+    __shared__ double sB[T64 * SP];
+    __shared__ double red[2][4];
+    int i, j;
+    tri_decode(blockIdx.x, i, j);
+    v4d S[2][2];
+    acc_zero(S);
+    for (int k = 0; k <= j; ++k) {
+        if (k > 0) __syncthreads();
+        stage<true>(sA, B + (size_t)i * T64 * ld + (size_t)k * T64, ld);  // op(A) = B_ik
+        stage<true>(sB, B + (size_t)j * T64 * ld + (size_t)k * T64, ld);  // op(B)(kk, c) = B_jk[c][kk]
+        if (k == i || k == j) {
+            __syncthreads();
+            if (k == i) zero_upper_t(sA);
+            if (k == j) zero_upper_t(sB);
+        }
+        __syncthreads();
+        mma64(sA, sB, S);
+    }
+    const double* Dij = D + (size_t)i * T64 * ld + (size_t)j * T64;
+    double e2 = 0.0, d2 = 0.0;
+    acc_for_each(S, [&](int r, int c, double v) {
+        if (i * T64 + r < n && j * T64 + c < n) {
+            const double d = Dij[(size_t)r * ld + c];
+            e2 += (d - v) * (d - v);
+            d2 += d * d;
+        }
+    });
+    const double w = (i == j) ? 1.0 : 2.0;
+    for (int off = 32; off > 0; off >>= 1) {
+        e2 += __shfl_down(e2, off);
+        d2 += __shfl_down(d2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = e2;
+        red[1][threadIdx.x >> 6] = d2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(acc, w * (red[0][0] + red[0][1] + red[0][2] + red[0][3]));
+        atomicAdd(acc + 1, w * (red[1][0] + red[1][1] + red[1][2] + red[1][3]));
     }
 }
 
@@ -817,11 +931,51 @@ int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s
     return PBX_OK;
 }
 
+// Variant 6 launch sequence: launch (c0, P, nc) for c0 = 0, 2, 4, ...
+template <bool MIXED>
+static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s) {
+    const int T = np / T64;
+    int* la = (int*)(work + (size_t)np * T64);
+    // Up to 64 tile columns the trailing updates of a launch finish well
+    // within its diagonal chain, so each workgroup gets a compute unit of
+    // its own (dynamic LDS beyond half the CU's 160 KB): nothing shares the
+    // chain's CU.  Larger orders are throughput-bound and keep two
+    // workgroups per CU.
+    const bool own_cu = T <= g_own_cu_tiles;
+    const size_t dyn_lds = own_cu ? 24576 : 0;
+    hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
+    int upd_target = 0;
+    for (int c0 = 0, P = 0; c0 < T;) {
+        const int nc = T - c0 < 2 ? T - c0 : 2;
+        const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
+        const int nw = T - c0 - nc;
+        const int ntr = (P > 0 && nw > 0) ? nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0) : 0;
+        upd_target += nh;
+        if (own_cu)
+            hipLaunchKernelGGL((potrf2_kernel<5, 1, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), dyn_lds, s, A, ld, T,
+                               c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+        else
+            hipLaunchKernelGGL((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
+                               P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+        P = nc;
+        c0 += nc;
+    }
+    hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
+}
+
+static void potrf_two_col(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, bool mixed) {
+    if (mixed)
+        potrf_two_col_launch<true>(A, np, ld, work, info_dev, s);
+    else
+        potrf_two_col_launch<false>(A, np, ld, work, info_dev, s);
+}
+
 // In-place lower Cholesky of the np x np matrix A; inverses of the diagonal
 // factor tiles are left in work (pbx_dense_work_doubles(np) doubles).
 // *info_dev must be zero on entry; it receives column+1 of the first
-// non-positive pivot.
-int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s) {
+// non-positive pivot.  mixed: FP32 trailing updates (always the two-column
+// launches).
+int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, int mixed) {
     if (np % T64 != 0 || np <= 0) {
         pbx_set_error(PBX_ERR_ARG, "potrf: order %d is not a positive multiple of 64", np);
         return PBX_ERR_ARG;
@@ -837,33 +991,8 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         TRY_RC(pbx_potrf_persist(A, np, ld, work, info_dev, state, device_cus(), s));
         return PBX_OK;
     }
-    if (g_potrf_variant == 6) {
-        int* la = (int*)(work + (size_t)np * T64);
-        // Up to 64 tile columns the trailing updates of a launch finish well
-        // within its diagonal chain, so each workgroup gets a compute unit of
-        // its own (dynamic LDS beyond half the CU's 160 KB): nothing shares
-        // the chain's CU.  Larger orders are throughput-bound and keep two
-        // workgroups per CU.
-        const bool own_cu = T <= g_own_cu_tiles;
-        const size_t dyn_lds = own_cu ? 24576 : 0;
-        hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
-        int upd_target = 0;
-        for (int c0 = 0, P = 0; c0 < T;) {
-            const int nc = T - c0 < 2 ? T - c0 : 2;
-            const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
-            const int nw = T - c0 - nc;
-            const int ntr = (P > 0 && nw > 0) ? nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0) : 0;
-            upd_target += nh;
-            if (own_cu)
-                hipLaunchKernelGGL((potrf2_kernel<5, 1>), dim3(1 + nh + nw + ntr), dim3(256), dyn_lds, s, A, ld, T, c0,
-                                   P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
-            else
-                hipLaunchKernelGGL((potrf2_kernel<5, 2>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0, P,
-                                   nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
-            P = nc;
-            c0 += nc;
-        }
-        hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
+    if (g_potrf_variant == 6 || mixed) {
+        potrf_two_col(A, np, ld, work, info_dev, s, mixed != 0);
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
@@ -913,6 +1042,21 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
     }
     hipLaunchKernelGGL(lauum_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
     PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// ||D - B B^T||_F / ||D||_F over the leading n x n block (acc: 2 doubles of
+// device scratch, cleared here).
+int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld, double* acc, double* rel,
+                        hipStream_t s) {
+    const int T = np / T64;
+    PBX_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), s));
+    hipLaunchKernelGGL(residual_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, ld, n, acc);
+    PBX_HIP(hipGetLastError());
+    double h[2];
+    PBX_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+    PBX_HIP(hipStreamSynchronize(s));
+    *rel = h[1] > 0 ? sqrt(h[0] / h[1]) : 0.0;
     return PBX_OK;
 }
 

@@ -115,6 +115,71 @@ __device__ __forceinline__ void acc_load(const double* C, int ld, double out[16]
             for (int v = 0; v < 4; ++v) out[q++] = C[(size_t)acc_row(x, v) * ld + acc_col(y)];
 }
 
+// ---------------------------------------------------------------------------
+// FP32 tile products (v_mfma_f32_16x16x4_f32) for the mixed-precision bulk
+// updates.  Operands are rounded to FP32 while staged; products accumulate in
+// FP32.  Operand lanes are as for the FP64 instruction, but the FP32
+// accumulator does not interleave rows: register v of lane group g holds
+// D[4g + v][l%16] (checked against the FP64 layout on the device).
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int SPF = 65;  // FP32 LDS row stride
+
+__device__ __forceinline__ v4f mfma32(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// s[col * SPF + row] = (float)X[row][col] (op-layout staging of an FP64 tile)
+__device__ __forceinline__ void stage_f32t(float* s, const double* X, int ld) {
+    v2d v[8];
+    stage_fetch(v, X, ld);
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = threadIdx.x + it * 256;
+        const int row = e >> 5, col = (e & 31) * 2;
+        s[col * SPF + row] = (float)v[it].x;
+        s[(col + 1) * SPF + row] = (float)v[it].y;
+    }
+}
+
+__device__ __forceinline__ void mma64_f32(const float* sA, const float* sB, v4f acc[2][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+    for (int k = 0; k < T64; k += 4) {
+        const float* pa = sA + (k + lk) * SPF;
+        const float* pb = sB + (k + lk) * SPF;
+        const float a0 = pa[r0 + li], a1 = pa[r0 + 16 + li];
+        const float b0 = pb[c0 + li], b1 = pb[c0 + 16 + li];
+        acc[0][0] = mfma32(a0, b0, acc[0][0]);
+        acc[0][1] = mfma32(a0, b1, acc[0][1]);
+        acc[1][0] = mfma32(a1, b0, acc[1][0]);
+        acc[1][1] = mfma32(a1, b1, acc[1][1]);
+    }
+}
+
+__device__ __forceinline__ void acc32_zero(v4f acc[2][2]) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (v4f){0.f, 0.f, 0.f, 0.f};
+}
+
+// Tile row of FP32 accumulator element (x, v) of this thread (columns as acc_col).
+__device__ __forceinline__ int acc32_row(int x, int v) {
+    return ((threadIdx.x >> 7) * 32) + 16 * x + 4 * ((threadIdx.x & 63) >> 4) + v;
+}
+
+__device__ __forceinline__ void acc32_load(const double* C, int ld, double out[16]) {
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[q++] = C[(size_t)acc32_row(x, v) * ld + acc_col(y)];
+}
+
 __device__ __forceinline__ double bcast(double x, int lane) {
     int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
     int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);

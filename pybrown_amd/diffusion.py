@@ -78,12 +78,17 @@ def JO_R_lubrication_correction_matrix(beads, pointers, lubrication_cutoff, cich
     return C.reshape(3 * N, 3 * N)
 
 
-def cholesky(D):
-    """np.linalg.cholesky (box.py:554) on the FP64 matrix cores."""
+def cholesky(D, precision="fp64"):
+    """np.linalg.cholesky (box.py:554) on the FP64 matrix cores; with
+    precision="mixed" the trailing updates run on the FP32 matrix cores
+    (operands rounded to FP32) and the diagonal tiles stay FP64."""
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("precision must be 'fp64' or 'mixed'")
     D = np.ascontiguousarray(D, dtype=np.float64)
     n = D.shape[0]
     L = np.zeros_like(D)
-    rc = lib.pbx_cholesky(ptr(D), n, ptr(L))
+    f = lib.pbx_cholesky if precision == "fp64" else lib.pbx_cholesky_mixed
+    rc = f(ptr(D), n, ptr(L))
     _lib.check_rc(rc, "cholesky")
     return L
 
