@@ -158,6 +158,35 @@ int pbx_sim_restore(pbx_sim* sim, const void* buf, long long bytes);
 /* count values of RandomState(seed).normal(0, 1) after skipping `skip`. */
 int pbx_rng_normals(unsigned int seed, long long skip, double* out, long long count);
 
+/* The step's force field (box.py:404-461, 685-723; interactions.py:483-589),
+ * recomputed from the positions at the start of every propagate and used in
+ * the deterministic terms D F dt / kBT (and the midpoint's (D_mid - D0) F):
+ *   F = unit * (external_force, zeroed on beads outside an enabled axis range
+ *               + Lennard-Jones over all bead pairs + harmonic bonds + angles),
+ *   E = sum of the interaction energies (input energy unit; pbx_sim_get_energy).
+ * Bonds and angles list bead indices (0-based, the order of the reference's
+ * loops); angles in degrees.  Set once, after pbx_sim_create. */
+typedef struct pbx_forces {
+    int lennard_jones_6, lennard_jones_12;
+    double lennard_jones_alpha;
+    const double* hard_core_radius;  /* N (sigma = sum of the pair's), or NULL */
+    const double* epsilon_LJ;        /* N (epsilon = sqrt of the product), or NULL */
+    int n_bonds;
+    const int* bond_beads;           /* 2 per bond: i (the bead listing it), j */
+    const double* bond_params;       /* 2 per bond: equilibrium distance, force constant */
+    int n_angles;
+    const int* angle_beads;          /* 3 per angle: i, j, k */
+    const double* angle_params;      /* 2 per angle: equilibrium angle (deg), force constant */
+    int region_on[3];                /* external_force_region: axis x, y, z enabled */
+    double region[6];                /* per axis: lower, upper bound (inclusive) */
+    double unit;                     /* energy_unit factor on F: 1 (J), 6.9477e-21 (kcal/mol),
+                                        1.602176634e-19 (eV) */
+} pbx_forces;
+int pbx_sim_set_forces(pbx_sim* sim, const pbx_forces* forces);
+int pbx_sim_get_energy(pbx_sim* sim, double* E);
+/* The force vector F (3N) the last step used (Box.F). */
+int pbx_sim_get_force(pbx_sim* sim, double* F);
+
 /* Box.propagate (box.py:150-198).  Returns 0; k > 0 for a non-SPD matrix
  * (pivot k-1); -4 when max_move_attempts is exhausted; other negatives for
  * argument/HIP errors. */

@@ -21,6 +21,7 @@ import math
 
 import numpy as np
 
+from oracle import forces
 from oracle.orc import Oracle
 
 BOLTZMANN = 1.380649e-23  # scipy.constants.Boltzmann (exact SI value)
@@ -35,6 +36,7 @@ DEFAULTS = {
     "divergence_term": False, "check_overlaps": True,
     "external_force": [0.0, 0.0, 0.0], "cichocki_correction": True,
     "overlap_treshold": 0.0, "max_move_attempts": 1000000,
+    "lennard_jones_6": False, "lennard_jones_12": False, "lennard_jones_alpha": 4.0, "energy_unit": "joule",
 }
 
 
@@ -43,7 +45,7 @@ class OracleBox:
 
     xyz: (N, 3) positions; radii: (N,); labels: list of N strings."""
 
-    def __init__(self, xyz, radii, labels, inp, lib="orc"):
+    def __init__(self, xyz, radii, labels, inp, lib="orc", hard_core=None, epsilon_LJ=None, bonds=(), angles=()):
         self.inp = dict(DEFAULTS)
         self.inp.update(inp)
         self.c = Oracle(lib)
@@ -92,6 +94,11 @@ class OracleBox:
             self.flux_plane_point = np.array(self.inp["measure_flux"]["plane_point"], float)
             self.net_flux = {lab: 0 for lab in self.mobile_labels}
         self.attempt_log = []
+        # force field (oracle/forces.py restates box.py:404-461 and interactions.py)
+        self.hard_core = np.array(hard_core if hard_core is not None else self.a, dtype=np.float64)
+        self.epsilon_LJ = np.array(epsilon_LJ if epsilon_LJ is not None else np.zeros(self.N), dtype=np.float64)
+        self.bonds, self.angles = list(bonds), list(angles)
+        self.E = 0.0
 
     # -- geometry ---------------------------------------------------------
     def _pointers(self):
@@ -248,7 +255,11 @@ class OracleBox:
                 self._compute_Dtot_matrix()
             if cholesky:
                 self._decompose_D_matrix()
-        self.F = np.tile(self.Fex, self.N).astype(np.float64)
+        self.F, self.E = forces.forces_and_energy(
+            self.r, self.box_length, Fex=self.Fex, region=self.inp.get("external_force_region"),
+            lj6=bool(self.inp["lennard_jones_6"]), lj12=bool(self.inp["lennard_jones_12"]),
+            alpha=self.inp["lennard_jones_alpha"], hc=self.hard_core, eps=self.epsilon_LJ, bonds=self.bonds,
+            angles=self.angles, unit=self.inp["energy_unit"])
         if self.propagation_scheme == "ermak":
             self._ermak_step(dt)
         elif self.propagation_scheme == "midpoint":

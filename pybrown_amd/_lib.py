@@ -44,6 +44,17 @@ class Config(ctypes.Structure):
     ]
 
 
+class Forces(ctypes.Structure):
+    """Mirror of pbx_forces (include/pybrown_amd.h)."""
+    _fields_ = [
+        ("lennard_jones_6", _i), ("lennard_jones_12", _i), ("lennard_jones_alpha", _d),
+        ("hard_core_radius", _dp), ("epsilon_LJ", _dp),
+        ("n_bonds", _i), ("bond_beads", _ip), ("bond_params", _dp),
+        ("n_angles", _i), ("angle_beads", _ip), ("angle_params", _dp),
+        ("region_on", _i * 3), ("region", _d * 6), ("unit", _d),
+    ]
+
+
 XI_FILL = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, _dp, ctypes.c_longlong)
 
 
@@ -79,6 +90,9 @@ _sig("pbx_sim_snapshot_size", ctypes.c_longlong, [ctypes.c_void_p])
 _sig("pbx_sim_snapshot", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
 _sig("pbx_sim_restore", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
 _sig("pbx_rng_normals", _i, [ctypes.c_uint, ctypes.c_longlong, _dp, ctypes.c_longlong])
+_sig("pbx_sim_set_forces", _i, [ctypes.c_void_p, ctypes.POINTER(Forces)])
+_sig("pbx_sim_get_energy", _i, [ctypes.c_void_p, _dp])
+_sig("pbx_sim_get_force", _i, [ctypes.c_void_p, _dp])
 _sig("pbx_sim_propagate", _i, [ctypes.c_void_p, _d, _i, _i, _i])
 _sig("pbx_sim_propagate_out", _i, [ctypes.c_void_p, _d, _i, _i, _i, _dp, _lp])
 _sig("pbx_sim_run", _i, [ctypes.c_void_p, _d, ctypes.c_longlong, ctypes.c_longlong, _i, _i, _i, _lp, _dp, _i])
@@ -108,7 +122,7 @@ EXPORTED = [
     "pbx_get_potrf_variant",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
     "pbx_sim_set_attempt_batch", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
-    "pbx_sim_restore", "pbx_sim_propagate",
+    "pbx_sim_restore", "pbx_sim_set_forces", "pbx_sim_get_energy", "pbx_sim_get_force", "pbx_sim_propagate",
     "pbx_sim_propagate_out",
     "pbx_sim_run",
     "pbx_sim_device_bytes",

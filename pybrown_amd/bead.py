@@ -123,3 +123,23 @@ def check_overlaps(beads, box_length, overlap_treshold, connection_matrix):
     if r < 0:
         raise RuntimeError("check_overlaps: %s" % _lib.last_error())
     return r == 1
+
+
+def bonded_lists(beads):
+    """Harmonic bonds (i, j, dist_eq, k) and angles (i, j, k, angle_eq, k_angle)
+    as bead indices, in the order of the reference's loops
+    (interactions.py:189-205, 295-316); bead ids resolve to the first bead
+    carrying them (bead.py:447-453)."""
+    first = {}
+    for n, b in enumerate(beads):
+        first.setdefault(b.bead_id, n)
+    bonds, angles = [], []
+    for i, b in enumerate(beads):
+        for other in b.bonded_with:
+            deq, k = b.bonded_how[other]
+            bonds.append((i, first[other], float(deq), float(k)))
+    for i, b in enumerate(beads):
+        for id1, id2 in b.angled_with:
+            aeq, k = b.angled_how[(id1, id2)]
+            angles.append((i, first[id1], first[id2], float(aeq), float(k)))
+    return bonds, angles

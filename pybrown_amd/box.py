@@ -25,6 +25,7 @@ import numpy as np
 
 from pybrown_amd import _lib
 from pybrown_amd._lib import lib, ptr
+from pybrown_amd.bead import bonded_lists
 from pybrown_amd.plane import Plane
 
 BOLTZMANN = 1.380649e-23  # scipy.constants.Boltzmann
@@ -45,7 +46,11 @@ DEFAULTS = {
     "overlap_treshold": 0.0, "max_move_attempts": 1000000,
 }
 
-_UNSUPPORTED = ("walls", "external_force_region", "measure_concentration", "reactions")
+_UNSUPPORTED = ("walls", "measure_concentration", "reactions")
+
+# interactions.py:823-841: factors the reference applies to F for non-SI
+# energy units (box.py:414-428)
+UNIT_FACTORS = {"joule": 1.0, "kcal/mol": 10 ** (-21) * 6.9477, "eV": 10 ** (-19) * 1.602176634}
 
 
 class Box:
@@ -60,15 +65,15 @@ class Box:
                 raise NotImplementedError("keyword %r is outside the hydrodynamic step" % k)
         if inp["divergence_term"]:
             raise NotImplementedError("divergence_term is outside the hydrodynamic step")
-        if inp["lennard_jones_6"] or inp["lennard_jones_12"] or inp["custom_interactions"]:
-            raise NotImplementedError("force fields are outside the hydrodynamic step")
+        if inp["custom_interactions"]:
+            raise NotImplementedError("custom interactions are Python functions: not run on the device")
+        if inp["energy_unit"] not in UNIT_FACTORS:
+            raise ValueError("unknown energy_unit %r" % inp["energy_unit"])
         if inp["hydrodynamics"] not in _lib.HYDRO_CODES:
             raise ValueError("unknown hydrodynamics %r" % inp["hydrodynamics"])
         if inp["propagation_scheme"] not in _lib.SCHEME_CODES:
             raise ValueError("unknown propagation_scheme %r" % inp["propagation_scheme"])
 
-        if any(b.bonded_with or b.angled_with for b in beads):
-            raise NotImplementedError("bonds and angles (force fields) are outside the hydrodynamic step")
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
@@ -126,7 +131,6 @@ class Box:
         self.end_simulation = False
         self.E = 0.0
         self.Fex = np.array(inp["external_force"], dtype=np.float64)
-        self.F = np.tile(self.Fex, len(self.beads)).astype(np.float64)
 
         cfg = _lib.Config()
         cfg.hydrodynamics = _lib.HYDRO_CODES[self.hydrodynamics]
@@ -175,10 +179,47 @@ class Box:
             _lib.check_rc(lib.pbx_sim_set_noise_source(self._sim, self._fill_cb, None), "set_noise_source")
             self.noise = "callback"
         _lib.check_rc(lib.pbx_sim_set_attempt_batch(self._sim, int(attempt_batch)), "set_attempt_batch")
+        self._set_forces()
         self._synced = self._pos.copy()
         self._counts = np.zeros(max(1, len(self.mobile_labels)), dtype=np.int64)
         self._pos_ptr = ptr(self._pos.reshape(-1))
         self._flux_ptr = self._counts.ctypes.data_as(_lib._lp)
+
+    # -- force field (box.py:404-461, 685-723; interactions.py:483-589) ----
+    def _set_forces(self):
+        import ctypes
+        inp = self.inp
+        f = _lib.Forces()
+        f.lennard_jones_6 = 1 if inp["lennard_jones_6"] else 0
+        f.lennard_jones_12 = 1 if inp["lennard_jones_12"] else 0
+        f.lennard_jones_alpha = float(inp["lennard_jones_alpha"])
+        self._hc = np.ascontiguousarray([b.hard_core_radius for b in self.beads], dtype=np.float64)
+        self._eps = np.ascontiguousarray([b.epsilon_LJ for b in self.beads], dtype=np.float64)
+        f.hard_core_radius = ptr(self._hc)
+        f.epsilon_LJ = ptr(self._eps)
+        bonds, angles = bonded_lists(self.beads)
+        self.bonds, self.angles = bonds, angles
+        self._bond_beads = np.ascontiguousarray([[q[0], q[1]] for q in bonds] or [[0, 0]], dtype=np.int32)
+        self._bond_params = np.ascontiguousarray([[q[2], q[3]] for q in bonds] or [[0.0, 0.0]], dtype=np.float64)
+        self._angle_beads = np.ascontiguousarray([[q[0], q[1], q[2]] for q in angles] or [[0, 0, 0]], dtype=np.int32)
+        self._angle_params = np.ascontiguousarray([[q[3], q[4]] for q in angles] or [[0.0, 0.0]], dtype=np.float64)
+        f.n_bonds, f.n_angles = len(bonds), len(angles)
+        f.bond_beads = _lib.iptr(self._bond_beads)
+        f.bond_params = ptr(self._bond_params)
+        f.angle_beads = _lib.iptr(self._angle_beads)
+        f.angle_params = ptr(self._angle_params)
+        region = inp.get("external_force_region") or {}
+        for c, ax in enumerate("xyz"):
+            if ax in region:
+                f.region_on[c] = 1
+                f.region[2 * c], f.region[2 * c + 1] = float(region[ax][0]), float(region[ax][1])
+        f.unit = UNIT_FACTORS[inp["energy_unit"]]
+        _lib.check_rc(lib.pbx_sim_set_forces(self._sim, ctypes.byref(f)), "set_forces")
+        self._E = np.zeros(1)
+
+    def _read_energy(self):
+        lib.pbx_sim_get_energy(self._sim, ptr(self._E))
+        self.E = float(self._E[0])
 
     # -- plumbing ---------------------------------------------------------
     def _bind_views(self):
@@ -210,6 +251,7 @@ class Box:
                                        int(bool(cholesky)), self._pos_ptr, self._flux_ptr)
         _lib.check_rc(rc, "propagate")
         self._synced[:] = self._pos
+        self._read_energy()
         if self.is_flux:
             self.net_flux = {lab: int(self._counts[k]) for k, lab in enumerate(self.mobile_labels)}
 
@@ -232,9 +274,17 @@ class Box:
         _lib.check_rc(rc, "run")
         _lib.check_rc(lib.pbx_sim_get_positions(self._sim, ptr(self._pos.reshape(-1))), "get_positions")
         self._synced[:] = self._pos
+        self._read_energy()
         if self.is_flux and steps:
             self.net_flux = {lab: int(flux[-1, k]) for k, lab in enumerate(self.mobile_labels)}
         return flux, frames
+
+    @property
+    def F(self):
+        """The force vector (3N) the last step used (box.py:404-428)."""
+        out = np.zeros(3 * self.N)
+        _lib.check_rc(lib.pbx_sim_get_force(self._sim, ptr(out)), "get_force")
+        return out
 
     def factor_residual(self):
         """||D - B B^T||_F / ||D||_F of the current matrices (on the device)."""

@@ -371,6 +371,12 @@ struct PbxSim {
     int have_D = 0, have_B = 0, have_Rff = 0;
     std::vector<void*> owned;
     size_t device_bytes = 0;         // sum of the allocations in `owned`
+    // force field (pbx_sim_set_forces): recomputed at the start of every step
+    int forces_set = 0, forces_on = 0;
+    PbxForceArgs fa{};
+    double* e_part = nullptr;        // per-bead energies
+    double* E_dev = nullptr;
+    double* E_host = nullptr;        // pinned: the last step's energy
     // optional stage timing with HIP events on the simulation stream
     int timing = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -883,6 +889,7 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     if (s->flag_host) (void)hipHostFree(s->flag_host);
     if (s->pos_host) (void)hipHostFree(s->pos_host);
     if (s->flux_host) (void)hipHostFree(s->flux_host);
+    if (s->E_host) (void)hipHostFree(s->E_host);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return PBX_OK;
@@ -1146,6 +1153,8 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         }
         TRY(post_info(s));
     }
+    // box.py:190: forces from the positions at the start of the step
+    if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
     if (cf.scheme == PBX_ERMAK)
         TRY(ermak(s, dt, c));
     else
@@ -1164,6 +1173,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         s->timed_steps += 1;
     }
     PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    if (s->forces_on) PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
     if (cf.measure_flux) {
         int nl = cf.n_labels > 0 ? cf.n_labels : 1;
         PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
@@ -1219,6 +1229,124 @@ extern "C" int pbx_sim_run(pbx_sim* s, double dt, long long j0, long long steps,
                 memset(flux_out + t * nl, 0, sizeof(long long) * nl);
         }
     }
+    return PBX_OK;
+}
+
+// box.py:404-461, 685-723 and interactions.py:483-589: the force field the
+// step recomputes from the positions at the start of every propagate.
+extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
+    pbx_clear_error();
+    if (!s || !f || !(f->unit > 0.0) || f->n_bonds < 0 || f->n_angles < 0 ||
+        (f->n_bonds && (!f->bond_beads || !f->bond_params)) || (f->n_angles && (!f->angle_beads || !f->angle_params))) {
+        pbx_set_error(PBX_ERR_ARG, "set_forces: bad arguments");
+        return PBX_ERR_ARG;
+    }
+    if (s->forces_set) {
+        pbx_set_error(PBX_ERR_STATE, "set_forces: the force field is set once per simulation");
+        return PBX_ERR_STATE;
+    }
+    const int N = s->N;
+    for (int q = 0; q < 2 * f->n_bonds; ++q)
+        if (f->bond_beads[q] < 0 || f->bond_beads[q] >= N) {
+            pbx_set_error(PBX_ERR_ARG, "set_forces: bond bead index out of range");
+            return PBX_ERR_ARG;
+        }
+    for (int q = 0; q < 3 * f->n_angles; ++q)
+        if (f->angle_beads[q] < 0 || f->angle_beads[q] >= N) {
+            pbx_set_error(PBX_ERR_ARG, "set_forces: angle bead index out of range");
+            return PBX_ERR_ARG;
+        }
+    PbxForceArgs& a = s->fa;
+    a.N = N;
+    a.L = s->cfg.box_length;
+    a.unit = f->unit;
+    a.lj = (f->lennard_jones_6 && f->lennard_jones_12) ? 3 : f->lennard_jones_6 ? 1 : f->lennard_jones_12 ? 2 : 0;
+    a.alpha = f->lennard_jones_alpha;
+    int region = 0;
+    for (int c = 0; c < 3; ++c) {
+        a.fex[c] = s->cfg.external_force[c];
+        a.reg_on[c] = f->region_on[c] ? 1 : 0;
+        a.reg_lo[c] = f->region[2 * c];
+        a.reg_hi[c] = f->region[2 * c + 1];
+        region |= a.reg_on[c];
+    }
+    s->forces_set = 1;
+    s->forces_on = a.lj || f->n_bonds || f->n_angles || region;
+    if (!s->forces_on) {
+        // constant F = unit * Fex on every bead (box.py:459-460, 414-428)
+        std::vector<double> F(s->np, 0.0);
+        s->F_nonzero = 0;
+        for (int i = 0; i < N; ++i)
+            for (int c = 0; c < 3; ++c) {
+                F[3 * i + c] = f->unit == 1.0 ? a.fex[c] : f->unit * a.fex[c];
+                if (F[3 * i + c] != 0.0) s->F_nonzero = 1;
+            }
+        PBX_HIP(hipMemcpy(s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice));
+        return PBX_OK;
+    }
+    s->F_nonzero = 1;
+    auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
+        TRY(sim_alloc(s, dst, bytes ? bytes : 8));
+        if (bytes) PBX_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return PBX_OK;
+    };
+    if (a.lj) {
+        std::vector<double> hc(N), ep(N);
+        for (int i = 0; i < N; ++i) {
+            hc[i] = f->hard_core_radius ? f->hard_core_radius[i] : 0.0;
+            ep[i] = f->epsilon_LJ ? f->epsilon_LJ[i] : 0.0;
+        }
+        TRY(upload((void**)&a.hc, hc.data(), sizeof(double) * N));
+        TRY(upload((void**)&a.eps, ep.data(), sizeof(double) * N));
+    }
+    // per-bead incidence lists (counting sort by bead, entries in list order)
+    std::vector<int> bptr(N + 1, 0), aptr(N + 1, 0);
+    for (int q = 0; q < f->n_bonds; ++q) {
+        ++bptr[f->bond_beads[2 * q] + 1];
+        ++bptr[f->bond_beads[2 * q + 1] + 1];
+    }
+    for (int q = 0; q < f->n_angles; ++q)
+        for (int r = 0; r < 3; ++r) ++aptr[f->angle_beads[3 * q + r] + 1];
+    for (int i = 0; i < N; ++i) {
+        bptr[i + 1] += bptr[i];
+        aptr[i + 1] += aptr[i];
+    }
+    std::vector<int> bent(2 * f->n_bonds), aent(3 * f->n_angles), bfill(bptr.begin(), bptr.end() - 1),
+        afill(aptr.begin(), aptr.end() - 1);
+    for (int q = 0; q < f->n_bonds; ++q)
+        for (int r = 0; r < 2; ++r) bent[bfill[f->bond_beads[2 * q + r]]++] = 2 * q + r;
+    for (int q = 0; q < f->n_angles; ++q)
+        for (int r = 0; r < 3; ++r) aent[afill[f->angle_beads[3 * q + r]]++] = 4 * q + r;
+    TRY(upload((void**)&a.bond_ptr, bptr.data(), sizeof(int) * (N + 1)));
+    TRY(upload((void**)&a.bond_ent, bent.data(), sizeof(int) * bent.size()));
+    TRY(upload((void**)&a.bond_ij, f->bond_beads, sizeof(int) * 2 * f->n_bonds));
+    TRY(upload((void**)&a.bond_par, f->bond_params, sizeof(double) * 2 * f->n_bonds));
+    TRY(upload((void**)&a.ang_ptr, aptr.data(), sizeof(int) * (N + 1)));
+    TRY(upload((void**)&a.ang_ent, aent.data(), sizeof(int) * aent.size()));
+    TRY(upload((void**)&a.ang_ijk, f->angle_beads, sizeof(int) * 3 * f->n_angles));
+    TRY(upload((void**)&a.ang_par, f->angle_params, sizeof(double) * 2 * f->n_angles));
+    TRY(sim_alloc(s, (void**)&s->e_part, sizeof(double) * N));
+    TRY(sim_alloc(s, (void**)&s->E_dev, sizeof(double)));
+    if (!s->E_host) {
+        PBX_HIP(hipHostMalloc((void**)&s->E_host, sizeof(double)));
+        *s->E_host = 0.0;
+    }
+    return PBX_OK;
+}
+
+// The force vector of the last step (3N, box.py:404-428).
+extern "C" int pbx_sim_get_force(pbx_sim* s, double* F) {
+    if (!s || !F) return PBX_ERR_ARG;
+    PBX_HIP(hipMemcpyAsync(F, s->F, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+// The energy of the last step's force field (box.py:406-412; 0 without
+// interactions), in the input's energy unit.
+extern "C" int pbx_sim_get_energy(pbx_sim* s, double* E) {
+    if (!s || !E) return PBX_ERR_ARG;
+    *E = (s->forces_on && s->E_host) ? *s->E_host : 0.0;
     return PBX_OK;
 }
 

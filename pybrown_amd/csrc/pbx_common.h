@@ -109,6 +109,31 @@ int pbx_launch_jo_2b(double ai, double aj, double rx, double ry, double rz, doub
 // FP32, FP32 accumulation, added into the FP64 matrix); diagonal tiles and
 // solves in FP64.
 int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, int mixed = 0);
+// Force field of the step (pbx_forces.hip): per-bead incidence lists hold
+// entries 2*bond+role (role 0: first bead) and 4*angle+role (roles 0..2).
+struct PbxForceArgs {
+    int N;
+    double L;
+    double fex[3];
+    int reg_on[3];
+    double reg_lo[3], reg_hi[3];
+    int lj;  // 0 none, 1 LJ-6, 2 LJ-12, 3 LJ-6-12
+    double alpha;
+    const double* hc;
+    const double* eps;
+    const int* bond_ptr;
+    const int* bond_ent;
+    const int* bond_ij;
+    const double* bond_par;
+    const int* ang_ptr;
+    const int* ang_ent;
+    const int* ang_ijk;
+    const double* ang_par;
+    double unit;
+};
+int pbx_launch_forces(const PbxForceArgs& a, const double* pos, double* F, double* e_part, double* E,
+                      hipStream_t s);
+
 int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld, double* acc, double* rel,
                         hipStream_t s);
 int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
