@@ -124,6 +124,9 @@ typedef struct pbx_config {
     double flux_normal[3];
     double flux_plane_point[3];
     double external_force[3];
+    int divergence_term;        /* divergence_term: the finite-difference
+                                   divergence of D in lubrication Ermak steps
+                                   (box.py:211-215, 560-610)                */
 } pbx_config;
 
 typedef struct PbxSim pbx_sim;

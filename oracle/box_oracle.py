@@ -176,8 +176,35 @@ class OracleBox:
         self.xi = self.rng.normal(0.0, 1.0, 3 * self.N)
         self.draw_count += 3 * self.N
 
-    def _ermak_step(self, dt):
+    def _compute_divergence(self, build_Dff, build_Dnf):
+        """box.py:560-610: forward differences of D, one coordinate at a time."""
+        eps = 0.0000001
+        lub = self.hydrodynamics in ("rpy_lub", "rpy_smith_lub")
+        D0 = np.copy(self.D)
+        if lub:
+            Mff0, Rff0, R0 = np.copy(self.Mff), np.copy(self.Rff), np.copy(self.R)
+        div = -np.sum(self.D, axis=1)
+        for i in range(self.N):
+            for j, vector in enumerate(([eps, 0.0, 0.0], [0.0, eps, 0.0], [0.0, 0.0, eps])):
+                vector = np.array(vector)
+                self.r[i] += vector
+                if build_Dff:
+                    self._compute_Dff_matrix()
+                if lub and build_Dnf:
+                    self._compute_Dtot_matrix()
+                div += self.D[:][3 * i + j]
+                self.r[i] += -vector
+        self.D = D0
+        if lub:
+            self.Mff, self.Rff, self.R = Mff0, Rff0, R0
+        div /= eps
+        return div
+
+    def _ermak_step(self, dt, build_Dff=True, build_Dnf=True):
         self._deterministic_step(dt)
+        # box.py:211-215
+        if self.inp.get("divergence_term") and self.hydrodynamics in ("rpy_lub", "rpy_smith_lub"):
+            self._translate(self._compute_divergence(build_Dff, build_Dnf) * dt)
         attempts = 0
         while True:
             attempts += 1
@@ -261,7 +288,7 @@ class OracleBox:
             alpha=self.inp["lennard_jones_alpha"], hc=self.hard_core, eps=self.epsilon_LJ, bonds=self.bonds,
             angles=self.angles, unit=self.inp["energy_unit"])
         if self.propagation_scheme == "ermak":
-            self._ermak_step(dt)
+            self._ermak_step(dt, build_Dff, build_Dnf)
         elif self.propagation_scheme == "midpoint":
             self._midpoint_step(dt, build_Dff, build_Dnf, cholesky)
         else:

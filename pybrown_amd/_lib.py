@@ -41,6 +41,7 @@ class Config(ctypes.Structure):
         ("box_length", _d), ("T", _d), ("viscosity", _d), ("ewald_alpha", _d),
         ("lubrication_cutoff", _d), ("m_midpoint", _d), ("overlap_treshold", _d),
         ("flux_normal", _d * 3), ("flux_plane_point", _d * 3), ("external_force", _d * 3),
+        ("divergence_term", _i),
     ]
 
 

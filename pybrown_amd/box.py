@@ -63,8 +63,6 @@ class Box:
         for k in _UNSUPPORTED:
             if k in inp:
                 raise NotImplementedError("keyword %r is outside the hydrodynamic step" % k)
-        if inp["divergence_term"]:
-            raise NotImplementedError("divergence_term is outside the hydrodynamic step")
         if inp["custom_interactions"]:
             raise NotImplementedError("custom interactions are Python functions: not run on the device")
         if inp["energy_unit"] not in UNIT_FACTORS:
@@ -144,6 +142,7 @@ class Box:
         cfg.max_move_attempts = int(inp["max_move_attempts"])
         cfg.xi_pool_attempts = int(xi_pool_attempts)
         cfg.factor_precision = 1 if precision == "mixed" else 0
+        cfg.divergence_term = 1 if inp["divergence_term"] else 0
         cfg.box_length = float(self.box_length)
         cfg.T = float(self.T)
         cfg.viscosity = float(self.viscosity)

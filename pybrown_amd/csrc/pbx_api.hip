@@ -36,6 +36,10 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
 int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
                          const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s);
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s);
+int pbx_launch_div_shift(double* pos, int idx, double delta, hipStream_t s);
+int pbx_launch_div_init(const double* D, int n, int ld, double* div, hipStream_t s);
+int pbx_launch_div_add_row(const double* D, int row, int n, int ld, double* div, hipStream_t s);
+int pbx_launch_div_finish(const double* div, int n, double eps, double dt, double* out, hipStream_t s);
 int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const double* x, const double* z, double alpha,
                      double* y, hipStream_t s);
 int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
@@ -375,6 +379,10 @@ struct PbxSim {
     int forces_set = 0, forces_on = 0;
     PbxForceArgs fa{};
     double* e_part = nullptr;        // per-bead energies
+    // divergence term (box.py:560-610): perturbed D / Rff and the accumulator
+    double* Ddiv = nullptr;
+    double* RffDiv = nullptr;
+    double* divv = nullptr;
     double* E_dev = nullptr;
     double* E_host = nullptr;        // pinned: the last step's energy
     // optional stage timing with HIP events on the simulation stream
@@ -621,12 +629,38 @@ int undo(PbxSim* s, StepCtx& c) {
     return pbx_launch_translate(s->disp, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
 }
 
+// box.py:560-610 (applied in box.py:211-215): with lubrication, the
+// divergence of D by forward differences: each coordinate in turn moved by
+// eps = 1e-7, D rebuilt there (the far-field part only when this step builds
+// it), its row for that coordinate summed, the coordinate moved back (so it
+// rounds as (r + eps) - eps, like the reference), and the beads translated by
+// (sum - row sums of D) / eps * dt.  3N rebuilds of D, as in the reference.
+int divergence_step(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
+    const double eps = 0.0000001;
+    if (!s->divv) {
+        TRY(sim_matrix(s, &s->Ddiv));
+        TRY(sim_matrix(s, &s->RffDiv));
+        TRY(sim_alloc(s, (void**)&s->divv, sizeof(double) * s->np));
+    }
+    TRY(pbx_launch_div_init(s->D, s->n, s->ld, s->divv, s->stream));
+    for (int k = 0; k < s->n; ++k) {
+        TRY(pbx_launch_div_shift(s->pos, k, eps, s->stream));
+        if (bDff) TRY(build_Dff(s, nullptr, s->RffDiv));
+        if (bDnf) TRY(build_Dtot(s, bDff ? s->RffDiv : s->Rff, s->Ddiv));
+        TRY(pbx_launch_div_add_row(bDnf ? s->Ddiv : s->D, k, s->n, s->ld, s->divv, s->stream));
+        TRY(pbx_launch_div_shift(s->pos, k, -eps, s->stream));
+    }
+    TRY(pbx_launch_div_finish(s->divv, s->n, eps, dt, s->drift, s->stream));
+    return pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
+}
+
 // box.py:202-250.  With overlap checks the attempts run in batches of K:
 // one pass over B yields K candidate moves, one kernel flags those that
 // overlap, and the first clean one is applied; the host reads one index per
 // batch instead of one flag per attempt.  Results do not depend on K.
-int ermak(PbxSim* s, double dt, StepCtx& c) {
+int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     TRY(det(s, s->D, dt, 1.0, c));
+    if (s->cfg.divergence_term && is_lub(s)) TRY(divergence_step(s, dt, bDff, bDnf, c));
     if (!s->cfg.check_overlaps) {
         const double* xi;
         TRY(next_xi(s, &xi));
@@ -1156,7 +1190,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     // box.py:190: forces from the positions at the start of the step
     if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
     if (cf.scheme == PBX_ERMAK)
-        TRY(ermak(s, dt, c));
+        TRY(ermak(s, dt, build_Dff_flag, build_Dnf_flag, c));
     else
         TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
     TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));

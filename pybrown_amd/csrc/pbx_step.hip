@@ -486,6 +486,58 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
     return PBX_OK;
 }
 
+// ---- divergence term by finite differences (box.py:560-610) -------------
+
+__global__ void div_shift_kernel(double* pos, int idx, double delta) {
+    if (threadIdx.x == 0) pos[idx] += delta;
+}
+
+// div[r] = -sum_c D[r][c] (one wave per row)
+__global__ void div_init_kernel(const double* D, int n, int ld, double* div) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (r >= n) return;
+    double acc = 0.0;
+    for (int c = lane; c < n; c += 64) acc += D[(size_t)r * ld + c];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off);
+    if (lane == 0) div[r] = -acc;
+}
+
+// div[c] += D[row][c]
+__global__ void div_add_row_kernel(const double* D, int row, int n, int ld, double* div) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) div[c] += D[(size_t)row * ld + c];
+}
+
+// out[c] = (div[c] / eps) * dt
+__global__ void div_finish_kernel(const double* div, int n, double eps, double dt, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) out[c] = (div[c] / eps) * dt;
+}
+
+int pbx_launch_div_shift(double* pos, int idx, double delta, hipStream_t s) {
+    hipLaunchKernelGGL(div_shift_kernel, dim3(1), dim3(64), 0, s, pos, idx, delta);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_div_init(const double* D, int n, int ld, double* div, hipStream_t s) {
+    hipLaunchKernelGGL(div_init_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, n, ld, div);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_div_add_row(const double* D, int row, int n, int ld, double* div, hipStream_t s) {
+    hipLaunchKernelGGL(div_add_row_kernel, dim3((n + 255) / 256), dim3(256), 0, s, D, row, n, ld, div);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_div_finish(const double* div, int n, double eps, double dt, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(div_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, div, n, eps, dt, out);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
 int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
                          const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s) {
     FluxPlane f;
