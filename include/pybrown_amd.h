@@ -127,6 +127,8 @@ typedef struct pbx_config {
     int divergence_term;        /* divergence_term: the finite-difference
                                    divergence of D in lubrication Ermak steps
                                    (box.py:211-215, 560-610)                */
+    int n_overlap_exempt;       /* pairs the overlap test skips (bonded      */
+    const int* overlap_exempt;  /* beads: bead.py:401-413), 2 indices each   */
 } pbx_config;
 
 typedef struct PbxSim pbx_sim;

@@ -76,7 +76,10 @@ class OracleBox:
         self.propagation_scheme = self.inp["propagation_scheme"]
         self.m_midpoint = self.inp["m_midpoint"]
         self.overlaps = self.inp["check_overlaps"]
+        # bead.py:401-413: bonded pairs are not tested for overlaps
         self.mask = np.zeros((self.N, self.N), dtype=np.int32)
+        for bond in bonds:
+            self.mask[bond[0]][bond[1]] = self.mask[bond[1]][bond[0]] = 1
         if self.hydrodynamics == "nohi":
             self._compute_Dff_matrix()
             self._decompose_D_matrix()

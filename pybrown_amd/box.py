@@ -143,6 +143,10 @@ class Box:
         cfg.xi_pool_attempts = int(xi_pool_attempts)
         cfg.factor_precision = 1 if precision == "mixed" else 0
         cfg.divergence_term = 1 if inp["divergence_term"] else 0
+        # bead.py:401-413 connection matrix: bonded pairs skip the overlap test
+        self._exempt = _overlap_exemptions(self.beads)
+        cfg.n_overlap_exempt = len(self._exempt) // 2
+        cfg.overlap_exempt = _lib.iptr(self._exempt) if len(self._exempt) else None
         cfg.box_length = float(self.box_length)
         cfg.T = float(self.T)
         cfg.viscosity = float(self.viscosity)
@@ -370,6 +374,23 @@ class Box:
 
     def __str__(self):
         return "simulation box"
+
+
+def _overlap_exemptions(beads):
+    """Bead index pairs (flat int32) with a bond between them in the sense of
+    Bead.is_there_bond_between (bead.py:187-189): either bead's bonded_with
+    holds the other's bead_id."""
+    by_id = {}
+    for n, b in enumerate(beads):
+        if b.bead_id is not None:
+            by_id.setdefault(b.bead_id, []).append(n)
+    pairs = set()
+    for i, b in enumerate(beads):
+        for other in b.bonded_with:
+            for j in by_id.get(other, ()):
+                if j != i:
+                    pairs.add((min(i, j), max(i, j)))
+    return np.ascontiguousarray(sorted(pairs), dtype=np.int32).reshape(-1)
 
 
 def ctypes_void_p():

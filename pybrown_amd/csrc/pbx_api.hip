@@ -376,6 +376,7 @@ struct PbxSim {
     std::vector<void*> owned;
     size_t device_bytes = 0;         // sum of the allocations in `owned`
     // force field (pbx_sim_set_forces): recomputed at the start of every step
+    int* ovl_mask = nullptr;         // connection matrix of the overlap test (N x N), or null
     int forces_set = 0, forces_on = 0;
     PbxForceArgs fa{};
     double* e_part = nullptr;        // per-bead energies
@@ -597,7 +598,7 @@ int next_xi(PbxSim* s, const double** out) {
 
 int overlap_now(PbxSim* s, int* hit) {
     PBX_HIP(hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
-    TRY(pbx_launch_overlaps(s->pos, s->a, s->N, s->cfg.box_length, s->cfg.overlap_treshold, nullptr, s->flag,
+    TRY(pbx_launch_overlaps(s->pos, s->a, s->N, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask, s->flag,
                             s->stream));
     PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     PBX_HIP(hipStreamSynchronize(s->stream));
@@ -685,7 +686,8 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         TRY(noise_ensure(s, K));
         TRY(pbx_launch_attempt_batch(nohi ? nullptr : s->B, nohi ? s->Bd : nullptr, s->ld, s->np, s->N, s->pos,
                                      s->xi_dev, ring_pos(s), ring_attempts(s), (int)K, c.sq2dt, s->Pbuf, s->cand,
-                                     s->cflag, s->a, s->cfg.box_length, s->cfg.overlap_treshold, s->stream));
+                                     s->cflag, s->a, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask,
+                                     s->stream));
         TRY(pbx_launch_attempt_select(s->cand, s->N, (int)K, s->cflag, s->pos, c.pn, c.pp, c.flux_on, s->label,
                                       s->flux, s->flag, s->stream));
         PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
@@ -896,6 +898,22 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             if (cfg->scheme == PBX_MIDPOINT)
                 if ((rc = sim_matrix(s, &s->RffMid))) return fail(rc);
         }
+    }
+    if (cfg->n_overlap_exempt > 0) {
+        // bead.py:401-413 build_connection_matrix: pairs the overlap test skips
+        if (!cfg->overlap_exempt) return fail(PBX_ERR_ARG);
+        std::vector<int> m((size_t)N * N, 0);
+        for (int q = 0; q < cfg->n_overlap_exempt; ++q) {
+            const int i = cfg->overlap_exempt[2 * q], j = cfg->overlap_exempt[2 * q + 1];
+            if (i < 0 || i >= N || j < 0 || j >= N) {
+                pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: overlap exemption index out of range");
+                return fail(PBX_ERR_ARG);
+            }
+            m[(size_t)i * N + j] = m[(size_t)j * N + i] = 1;
+        }
+        if ((rc = sim_alloc(s, (void**)&s->ovl_mask, sizeof(int) * m.size()))) return fail(rc);
+        if (!hip_ok(hipMemcpy(s->ovl_mask, m.data(), sizeof(int) * m.size(), hipMemcpyHostToDevice), "mask"))
+            return fail(PBX_ERR_HIP);
     }
     if (!hip_ok(hipStreamSynchronize(s->stream), "create sync")) return fail(PBX_ERR_HIP);
     if (cfg->check_overlaps) {

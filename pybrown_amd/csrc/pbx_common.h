@@ -150,7 +150,8 @@ int pbx_attempt_slab_width(int np);
 size_t pbx_attempt_partial_doubles(int np);
 int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
                              const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
-                             double* cand, int* cflag, const double* a, double L, double thr, hipStream_t s);
+                             double* cand, int* cflag, const double* a, double L, double thr, const int* mask,
+                             hipStream_t s);
 int pbx_launch_attempt_select(const double* cand, int N, int K, const int* cflag, double* pos, const double* plane_n,
                               const double* plane_p, int flux_on, const int* label, long long* flux, int* sel,
                               hipStream_t s);

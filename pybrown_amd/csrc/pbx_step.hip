@@ -373,7 +373,8 @@ __global__ __launch_bounds__(128) void cand_kernel(const double* __restrict__ po
 }
 
 // Overlap flags of K candidates: lower 16 x 16 bead tiles, one pair per
-// thread; bit k of cflag[0] is set when candidate k overlaps.  A pair can overlap in some candidate only if
+// thread; bit k of cflag[0] is set when candidate k overlaps; pairs marked
+// in `mask` (N x N, may be null) are exempt, as in bead.c:180.  A pair can overlap in some candidate only if
 // its current minimum-image distance is at most S + thr + delta_i + delta_j
 // (the torus distance moves by at most the two displacements, and the
 // reference test flags only pairs within S + thr); other pairs skip the loop.
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __res
                                                              const double* __restrict__ pos,
                                                              const double* __restrict__ delta,
                                                              const double* __restrict__ a, int N, int K, double L,
-                                                             double thr, int* cflag) {
+                                                             double thr, const int* __restrict__ mask, int* cflag) {
     const int t = blockIdx.x;
     int bi = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
@@ -390,7 +391,8 @@ __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __res
     const int i = bi * 16 + (threadIdx.x >> 4), j = bj * 16 + (threadIdx.x & 15);
     bool near = false;
     double ai = 0.0, aj = 0.0;
-    if (i < N && j < i) {
+    // mask: the reference's connection matrix (bonded pairs are not tested)
+    if (i < N && j < i && !(mask && mask[(size_t)j + (size_t)N * i])) {
         ai = a[i];
         aj = a[j];
         double s2 = 0.0;
@@ -605,7 +607,8 @@ int pbx_attempt_kmax() { return KC_MAX; }
 
 int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
                              const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
-                             double* cand, int* cflag, const double* a, double L, double thr, hipStream_t s) {
+                             double* cand, int* cflag, const double* a, double L, double thr, const int* mask,
+                             hipStream_t s) {
     if (K < 1 || K > KC_MAX || ring_pos < 0 || ring_pos >= ring_att) {
         pbx_set_error(PBX_ERR_ARG, "attempt batch: K=%d ring_pos=%d ring_att=%d", K, ring_pos, ring_att);
         return PBX_ERR_ARG;
@@ -636,7 +639,7 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
     if (N > 1) {
         const int T16 = (N + 15) / 16;
         hipLaunchKernelGGL(overlaps_batch_kernel, dim3(T16 * (T16 + 1) / 2), dim3(256), 0, s, cand, pos, delta, a, N, K,
-                           L, thr, cflag);
+                           L, thr, mask, cflag);
         PBX_HIP(hipGetLastError());
     }
     return PBX_OK;

@@ -13,17 +13,19 @@ from oracle.box_oracle import OracleBox
 pytestmark = pytest.mark.gpu
 
 
-def _chain(N, spacing, seed, L=1.0e6):
+def _chain(N, spacing, seed, L=1.0e6, min_sep=None):
     """A random walk with steps of `spacing` (a bead-spring chain), bonds
-    between neighbours and angles over each three consecutive beads."""
+    between neighbours and angles over each three consecutive beads;
+    non-neighbours at least `min_sep` apart (default 0.9 spacing)."""
     from pybrown_amd.bead import Bead
     rng = np.random.RandomState(seed)
+    min_sep = 0.9 * spacing if min_sep is None else min_sep
     xyz = np.zeros((N, 3))
     for i in range(1, N):
         while True:
             v = rng.normal(size=3)
             p = xyz[i - 1] + spacing * v / np.linalg.norm(v)
-            if i < 2 or np.linalg.norm(p - xyz[:i - 1], axis=1).min() > 0.9 * spacing:
+            if i < 2 or np.linalg.norm(p - xyz[:i - 1], axis=1).min() > min_sep:
                 break
         xyz[i] = p
     xyz -= xyz.mean(axis=0)
@@ -85,6 +87,8 @@ def _trajectory(beads, xyz, inp, steps, dt, tol=1e-10):
         for j in range(steps):
             g.propagate(dt)
             o.propagate(dt)
+            if o.attempt_log:
+                assert g.last_move_attempts == o.attempt_log[-1], j
             pos_g = np.array([b.r for b in g.beads])
             err = np.abs(pos_g - o.r).max() / np.abs(o.r).max()
             worst = max(worst, err)
@@ -118,3 +122,17 @@ def test_midpoint_with_region_force_and_bonds():
            "energy_unit": "kcal/mol", "check_overlaps": False}
     worst, E = _trajectory(beads, xyz, inp, 10, 0.05)
     print("midpoint chain: worst position error %.3e, final E %.6e" % (worst, E))
+
+
+def test_bonded_chain_with_overlap_checks():
+    """Bonded neighbours closer than their radii' sum are exempt from the
+    overlap test (bead.py:401-413, bead.c:180): the run starts and moves, with
+    rejections of non-bonded contacts as in the oracle."""
+    beads, xyz = _chain(24, 1.8, 7, min_sep=2.3)
+    inp = {"box_length": 1.0e6, "T": 298.0, "viscosity": 0.01, "seed": 8, "hydrodynamics": "rpy",
+           "energy_unit": "kcal/mol", "external_force": [0.0, 0.0, 0.0], "check_overlaps": True,
+           "overlap_treshold": 0.3}
+    d = np.linalg.norm(xyz[1:] - xyz[:-1], axis=1)
+    assert d.max() < 2.0  # every bonded pair overlaps by the radii
+    worst, E = _trajectory(beads, xyz, inp, 15, 0.05)
+    print("bonded chain with overlap checks: worst position error %.3e, E %.4f" % (worst, E))
