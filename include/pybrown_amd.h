@@ -129,6 +129,12 @@ typedef struct pbx_config {
                                    (box.py:211-215, 560-610)                */
     int n_overlap_exempt;       /* pairs the overlap test skips (bonded      */
     const int* overlap_exempt;  /* beads: bead.py:401-413), 2 indices each   */
+    int n_immobile;             /* immobile beads (box.py:649-668): fixed    */
+    const double* immobile_xyz; /* positions (n_immobile x 3) and radii.     */
+    const double* immobile_radii; /* They take bead indices N .. N+n_immobile-1
+                                   in overlap_exempt and the force field, are
+                                   tested for overlaps against mobile beads
+                                   only, and are absent from D              */
 } pbx_config;
 
 typedef struct PbxSim pbx_sim;
@@ -170,12 +176,15 @@ int pbx_rng_normals(unsigned int seed, long long skip, double* out, long long co
  *               + Lennard-Jones over all bead pairs + harmonic bonds + angles),
  *   E = sum of the interaction energies (input energy unit; pbx_sim_get_energy).
  * Bonds and angles list bead indices (0-based, the order of the reference's
- * loops); angles in degrees.  Set once, after pbx_sim_create. */
+ * loops); angles in degrees.  Set once, after pbx_sim_create.  With immobile
+ * beads (pbx_config.n_immobile) indices N.. name them: they act on mobile
+ * beads (Lennard-Jones partners, bond and angle members, with the pair
+ * vectors of interactions.py:185-416) and receive no force. */
 typedef struct pbx_forces {
     int lennard_jones_6, lennard_jones_12;
     double lennard_jones_alpha;
-    const double* hard_core_radius;  /* N (sigma = sum of the pair's), or NULL */
-    const double* epsilon_LJ;        /* N (epsilon = sqrt of the product), or NULL */
+    const double* hard_core_radius;  /* N + n_immobile (sigma = sum of the pair's), or NULL */
+    const double* epsilon_LJ;        /* N + n_immobile (epsilon = sqrt of the product), or NULL */
     int n_bonds;
     const int* bond_beads;           /* 2 per bond: i (the bead listing it), j */
     const double* bond_params;       /* 2 per bond: equilibrium distance, force constant */

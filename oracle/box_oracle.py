@@ -43,15 +43,30 @@ DEFAULTS = {
 class OracleBox:
     """CPU twin of pyBrown.box.Box restricted to the hydrodynamic step.
 
-    xyz: (N, 3) positions; radii: (N,); labels: list of N strings."""
+    xyz: (Nt, 3) positions of every bead; radii: (Nt,); labels: Nt strings;
+    mobile: Nt flags (default all mobile; box.py:649-668 also makes beads
+    with a label in immobile_labels immobile).  Immobile beads stay put: D,
+    the noise and the moves cover the N mobile beads (in bead order), the
+    overlap test covers every bead, with pairs of two immobile beads exempt
+    (bead.py:401-413), and the force field lets immobile beads act on mobile
+    ones."""
 
-    def __init__(self, xyz, radii, labels, inp, lib="orc", hard_core=None, epsilon_LJ=None, bonds=(), angles=()):
+    def __init__(self, xyz, radii, labels, inp, lib="orc", hard_core=None, epsilon_LJ=None, bonds=(), angles=(),
+                 mobile=None):
         self.inp = dict(DEFAULTS)
         self.inp.update(inp)
         self.c = Oracle(lib)
         self.r = np.array(xyz, dtype=np.float64).reshape(-1, 3).copy()
-        self.a = np.array(radii, dtype=np.float64).copy()
-        self.labels = list(labels)
+        self.a_all = np.array(radii, dtype=np.float64).copy()
+        self.labels_all = list(labels)
+        Nt = self.a_all.size
+        flags = [True] * Nt if mobile is None else [bool(m) for m in mobile]
+        flags = [f and (self.labels_all[i] not in self.inp["immobile_labels"]) for i, f in enumerate(flags)]
+        self.mobile = np.array(flags, dtype=bool)
+        # rows of the mobile beads in self.r (a slice, hence views, when all move)
+        self.mob = slice(None) if self.mobile.all() else np.nonzero(self.mobile)[0]
+        self.a = self.a_all[self.mob]
+        self.labels = [lab for lab, f in zip(self.labels_all, flags) if f]
         self.N = self.a.size
         self.box_length = float(self.inp["box_length"])
         # box.py:621-629
@@ -76,10 +91,13 @@ class OracleBox:
         self.propagation_scheme = self.inp["propagation_scheme"]
         self.m_midpoint = self.inp["m_midpoint"]
         self.overlaps = self.inp["check_overlaps"]
-        # bead.py:401-413: bonded pairs are not tested for overlaps
-        self.mask = np.zeros((self.N, self.N), dtype=np.int32)
+        # bead.py:401-413: bonded pairs and pairs of immobile beads are not
+        # tested for overlaps
+        self.mask = np.zeros((Nt, Nt), dtype=np.int32)
         for bond in bonds:
             self.mask[bond[0]][bond[1]] = self.mask[bond[1]][bond[0]] = 1
+        imm = np.nonzero(~self.mobile)[0]
+        self.mask[np.ix_(imm, imm)] = 1
         if self.hydrodynamics == "nohi":
             self._compute_Dff_matrix()
             self._decompose_D_matrix()
@@ -87,10 +105,13 @@ class OracleBox:
             assert not self._check_overlaps(), "starting configuration contains overlaps"
         self.Fex = np.array(self.inp["external_force"], dtype=np.float64)
         self.F = np.zeros(3 * self.N)
-        self.mobile_labels = []
-        for lab in self.labels:
-            if lab not in self.mobile_labels:
-                self.mobile_labels.append(lab)
+        # box.py:672-681: a label is mobile when its first bead is
+        self.mobile_labels, seen = [], []
+        for lab, f in zip(self.labels_all, flags):
+            if lab not in seen:
+                seen.append(lab)
+                if f:
+                    self.mobile_labels.append(lab)
         self.is_flux = "measure_flux" in self.inp
         if self.is_flux:
             self.flux_normal = np.array(self.inp["measure_flux"]["normal"], float)
@@ -98,17 +119,23 @@ class OracleBox:
             self.net_flux = {lab: 0 for lab in self.mobile_labels}
         self.attempt_log = []
         # force field (oracle/forces.py restates box.py:404-461 and interactions.py)
-        self.hard_core = np.array(hard_core if hard_core is not None else self.a, dtype=np.float64)
-        self.epsilon_LJ = np.array(epsilon_LJ if epsilon_LJ is not None else np.zeros(self.N), dtype=np.float64)
+        self.hard_core = np.array(hard_core if hard_core is not None else self.a_all, dtype=np.float64)
+        self.epsilon_LJ = np.array(epsilon_LJ if epsilon_LJ is not None else np.zeros(Nt), dtype=np.float64)
         self.bonds, self.angles = list(bonds), list(angles)
         self.E = 0.0
 
     # -- geometry ---------------------------------------------------------
+    @property
+    def rm(self):
+        """Positions of the mobile beads (a view when every bead moves)."""
+        return self.r[self.mob]
+
     def _pointers(self):
-        return self.c.pointer_pbc_matrix(self.r, self.box_length)
+        return self.c.pointer_pbc_matrix(self.rm, self.box_length)
 
     def _check_overlaps(self):
-        return self.c.check_overlaps(self.r, self.a, self.box_length, self.inp["overlap_treshold"], self.mask) == 1
+        return self.c.check_overlaps(self.r, self.a_all, self.box_length, self.inp["overlap_treshold"],
+                                     self.mask) == 1
 
     # -- matrices (box.py:486-556) ---------------------------------------
     def _compute_Dff_matrix(self):
@@ -151,11 +178,11 @@ class OracleBox:
         v = np.asarray(v, dtype=np.float64).reshape(-1, 3)
         if self.is_flux:
             n, p = self.flux_normal, self.flux_plane_point
-            d0 = self.r - p
+            d0 = self.rm - p
             s0 = (n[0] * d0[:, 0] + n[1] * d0[:, 1] + n[2] * d0[:, 2]) > 0.0
-        self.r += v
+        self.r[self.mob] += v
         if self.is_flux:
-            d1 = self.r - p
+            d1 = self.rm - p
             s1 = (n[0] * d1[:, 0] + n[1] * d1[:, 1] + n[2] * d1[:, 2]) > 0.0
             delta = s1.astype(np.int64) - s0.astype(np.int64)
             for k in np.nonzero(delta)[0]:
@@ -187,16 +214,17 @@ class OracleBox:
         if lub:
             Mff0, Rff0, R0 = np.copy(self.Mff), np.copy(self.Rff), np.copy(self.R)
         div = -np.sum(self.D, axis=1)
+        rows = np.arange(len(self.r))[self.mob]
         for i in range(self.N):
             for j, vector in enumerate(([eps, 0.0, 0.0], [0.0, eps, 0.0], [0.0, 0.0, eps])):
                 vector = np.array(vector)
-                self.r[i] += vector
+                self.r[rows[i]] += vector
                 if build_Dff:
                     self._compute_Dff_matrix()
                 if lub and build_Dnf:
                     self._compute_Dtot_matrix()
                 div += self.D[:][3 * i + j]
-                self.r[i] += -vector
+                self.r[rows[i]] += -vector
         self.D = D0
         if lub:
             self.Mff, self.Rff, self.R = Mff0, Rff0, R0
@@ -267,7 +295,7 @@ class OracleBox:
 
     def _keep_in_box(self):
         L = self.box_length
-        for i in range(self.N):
+        for i in np.arange(len(self.r))[self.mob]:
             for c in range(3):
                 while self.r[i, c] < -L / 2:
                     self.r[i, c] += L
@@ -289,7 +317,7 @@ class OracleBox:
             self.r, self.box_length, Fex=self.Fex, region=self.inp.get("external_force_region"),
             lj6=bool(self.inp["lennard_jones_6"]), lj12=bool(self.inp["lennard_jones_12"]),
             alpha=self.inp["lennard_jones_alpha"], hc=self.hard_core, eps=self.epsilon_LJ, bonds=self.bonds,
-            angles=self.angles, unit=self.inp["energy_unit"])
+            angles=self.angles, unit=self.inp["energy_unit"], mobile=self.mobile)
         if self.propagation_scheme == "ermak":
             self._ermak_step(dt, build_Dff, build_Dnf)
         elif self.propagation_scheme == "midpoint":

@@ -10,15 +10,19 @@ Follows, in the reference's own accumulation order:
                         region per axis
   interactions.py:483-589  interaction order: Lennard-Jones, harmonic bonds,
                         harmonic angles
-  interactions.py:185-247, 249-289, 291-416  loops (mobile beads only)
+  interactions.py:185-247, 249-289, 291-416  loops over mobile beads, then
+                        immobile ones, with the mobile-immobile pointers
+                        (rik, bead.c:99) and their negations and differences
+                        exactly where the reference uses them
   interactions.py:641-821  force and energy formulas
   interactions.py:823-841  unit factors
   bead.py:272-278       angle_pbc
   bead.c:44-50          minimum image with the `while` semantics
 
 Pinned by tests/test_forces_oracle.py against the known answers of the
-reference's tests/test_interactions.py and against -grad E by finite
-differences.
+reference's tests/test_interactions.py (with and without immobile beads),
+against -grad E by finite differences, and against the invariant that making
+a bead immobile leaves the forces on the other beads unchanged.
 """
 import math
 
@@ -112,39 +116,85 @@ def external(xyz, Fex, region=None):
 
 
 def forces_and_energy(xyz, L, Fex=(0.0, 0.0, 0.0), region=None, lj6=False, lj12=False, alpha=4.0, hc=None,
-                      eps=None, bonds=(), angles=(), unit="joule"):
-    """F (3N) and E of box.py:404-428 for mobile beads at xyz.
+                      eps=None, bonds=(), angles=(), unit="joule", mobile=None):
+    """F (3 per mobile bead, in bead order) and E of box.py:404-428.
 
+    xyz, hc, eps: every bead; mobile: one flag per bead (default: all mobile)
     bonds:  (i, j, dist_eq, k) in the reference's loop order (bead i lists j)
     angles: (i, j, k, angle_eq_degrees, force_constant), likewise
     """
     N = len(xyz)
-    F = external(xyz, Fex, region)
+    mob = [i for i in range(N) if mobile is None or mobile[i]]
+    imm = [i for i in range(N) if not (mobile is None or mobile[i])]
+    is_mob = [mobile is None or bool(mobile[i]) for i in range(N)]
+    row = {b: n for n, b in enumerate(mob)}  # force row of a mobile bead
+
+    def add(b, f):
+        F[3 * row[b]:3 * row[b] + 3] += f
+
+    def P(a, b):  # rij[a][b] / rik[a][b] = wrap(r_b - r_a)
+        return pointer(xyz[a], xyz[b], L)
+
+    F = external(np.asarray(xyz)[mob], Fex, region) if mob else np.zeros(0)
     E = 0.0
     if lj6 or lj12:
+        # interactions.py:249-289: mobile pairs, then mobile-immobile pairs
         e = 0.0
-        for i in range(1, N):
-            for j in range(i):
-                f, ee = lj(pointer(xyz[i], xyz[j], L), hc[i], hc[j], eps[i], eps[j], alpha, lj6, lj12)
-                F[3 * i:3 * i + 3] += f
-                F[3 * j:3 * j + 3] -= f
+        for ii in range(1, len(mob)):
+            for jj in range(ii):
+                i, j = mob[ii], mob[jj]
+                f, ee = lj(P(i, j), hc[i], hc[j], eps[i], eps[j], alpha, lj6, lj12)
+                add(i, f)
+                add(j, -f)
+                e += ee
+        for i in mob:
+            for j in imm:
+                f, ee = lj(P(i, j), hc[i], hc[j], eps[i], eps[j], alpha, lj6, lj12)
+                add(i, f)
                 e += ee
         E += e
     if len(bonds):
+        # interactions.py:185-247
         e = 0.0
         for i, j, deq, k in bonds:
-            f, ee = bond(pointer(xyz[i], xyz[j], L), deq, k)
-            F[3 * i:3 * i + 3] += f
-            F[3 * j:3 * j + 3] -= f
+            if is_mob[i]:
+                f, ee = bond(P(i, j), deq, k)
+                add(i, f)
+                if is_mob[j]:
+                    add(j, -f)
+            elif is_mob[j]:
+                f, ee = bond(-P(j, i), deq, k)
+                add(j, -f)
+            else:
+                continue
             e += ee
         E += e
     if len(angles):
+        # interactions.py:291-416
         e = 0.0
         for i, j, k, aeq, kc in angles:
-            f, ee = angle(pointer(xyz[i], xyz[j], L), pointer(xyz[j], xyz[k], L), aeq, kc)
-            F[3 * i:3 * i + 3] += f[:3]
-            F[3 * j:3 * j + 3] += f[3:6]
-            F[3 * k:3 * k + 3] += f[6:]
+            mj, mk = is_mob[j], is_mob[k]
+            if is_mob[i]:
+                p12 = P(i, j)
+                if mk:
+                    p23 = P(j, k) if mj else -P(k, j)
+                else:
+                    p23 = P(j, k) if mj else P(i, k) - p12
+            elif not (mj or mk):
+                continue
+            elif mj:
+                p12 = -P(j, i)
+                p23 = P(j, k)
+            else:
+                p23 = -P(k, j)
+                p12 = -P(k, i) - p23
+            f, ee = angle(p12, p23, aeq, kc)
+            if is_mob[i]:
+                add(i, f[:3])
+            if mj:
+                add(j, f[3:6])
+            if mk:
+                add(k, f[6:])
             e += ee
         E += e
     if unit != "joule":
@@ -152,18 +202,25 @@ def forces_and_energy(xyz, L, Fex=(0.0, 0.0, 0.0), region=None, lj6=False, lj12=
     return F, E
 
 
-def bonded_lists(beads):
-    """Bond and angle lists from Bead objects in the reference's loop order
-    (interactions.py:189-205, 295-316; ids resolved as bead.py:447-453)."""
+def bonded_lists(beads, mobile=None):
+    """Bond and angle lists (indices into `beads`) in the reference's loop
+    order: beads listing them, mobile ones first, then immobile ones
+    (interactions.py:189-245, 295-416); an id resolves to the first mobile
+    bead carrying it, else the first immobile one (bead.py:447-453 applied to
+    the mobile list, then the immobile list)."""
+    is_mob = [(mobile is None or bool(mobile[n])) for n in range(len(beads))]
+    order = [n for n in range(len(beads)) if is_mob[n]] + [n for n in range(len(beads)) if not is_mob[n]]
     idx = {}
-    for n, b in enumerate(beads):
-        idx.setdefault(b.bead_id, n)
+    for n in order:
+        idx.setdefault(beads[n].bead_id, n)
     bonds, angles = [], []
-    for i, b in enumerate(beads):
+    for i in order:
+        b = beads[i]
         for other in b.bonded_with:
             deq, k = b.bonded_how[other]
             bonds.append((i, idx[other], deq, k))
-    for i, b in enumerate(beads):
+    for i in order:
+        b = beads[i]
         for id1, id2 in b.angled_with:
             aeq, k = b.angled_how[(id1, id2)]
             angles.append((i, idx[id1], idx[id2], aeq, k))

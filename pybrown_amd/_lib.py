@@ -42,6 +42,7 @@ class Config(ctypes.Structure):
         ("lubrication_cutoff", _d), ("m_midpoint", _d), ("overlap_treshold", _d),
         ("flux_normal", _d * 3), ("flux_plane_point", _d * 3), ("external_force", _d * 3),
         ("divergence_term", _i), ("n_overlap_exempt", _i), ("overlap_exempt", _ip),
+        ("n_immobile", _i), ("immobile_xyz", _dp), ("immobile_radii", _dp),
     ]
 
 

@@ -129,7 +129,10 @@ def bonded_lists(beads):
     """Harmonic bonds (i, j, dist_eq, k) and angles (i, j, k, angle_eq, k_angle)
     as bead indices, in the order of the reference's loops
     (interactions.py:189-205, 295-316); bead ids resolve to the first bead
-    carrying them (bead.py:447-453)."""
+    carrying them (bead.py:447-453).  Given mobile beads followed by immobile
+    ones, this is also the reference's order with immobile beads (their
+    loops come after the mobile ones, :232-245, :349-416) and its id lookup
+    (mobile beads first)."""
     first = {}
     for n, b in enumerate(beads):
         first.setdefault(b.bead_id, n)

@@ -15,9 +15,15 @@ thread; ``pseudorandom_number_generator`` is then not advanced.
 ``noise="callback"`` draws from ``pseudorandom_number_generator`` instead,
 one chunk of ``xi_pool_attempts`` attempts at a time.
 
+Immobile beads (``immobile_labels`` or ``Bead.mobile = False``,
+box.py:649-668) stay where they are: they are absent from D and the noise,
+take part in the overlap test against mobile beads and act on mobile beads
+through the force field.  On the device they follow the mobile beads in
+index order; their positions are uploaded once, at construction.
+
 Scope: the hydrodynamic step.  Keywords the step does not cover (walls,
-reactions, force fields, immobile beads, divergence term, force or
-concentration regions) raise NotImplementedError instead of being ignored.
+reactions, concentration, custom interactions) raise NotImplementedError
+instead of being ignored.
 """
 import math
 
@@ -85,21 +91,30 @@ class Box:
             self.seed = inp["seed"]
         self.pseudorandom_number_generator = np.random.RandomState(self.seed)
 
-        # box.py:649-668 mobility split (immobile beads are out of scope)
+        # box.py:649-668 mobility split
         for b in self.beads:
             if b.label in inp["immobile_labels"]:
                 b.mobile = False
-        if not all(b.mobile for b in self.beads):
-            raise NotImplementedError("immobile beads are outside the hydrodynamic step")
-        self.mobile_beads = list(self.beads)
-        self.immobile_beads = []
-        # box.py:672-681 labels in order of first appearance
+        self.mobile_beads = [b for b in self.beads if b.mobile]
+        self.immobile_beads = [b for b in self.beads if not b.mobile]
+        self.mobile_bead_indices = [i for i, b in enumerate(self.beads) if b.mobile]
+        self.immobile_bead_indices = [i for i, b in enumerate(self.beads) if not b.mobile]
+        if not self.mobile_beads:
+            raise ValueError("Box needs at least one mobile bead")
+        # box.py:672-681 labels in order of first appearance; a label counts
+        # as mobile when the first bead carrying it is mobile
         self.labels, self.mobile_labels = [], []
         for b in self.beads:
             if b.label not in self.labels:
                 self.labels.append(b.label)
-                self.mobile_labels.append(b.label)
-        label_idx = np.array([self.mobile_labels.index(b.label) for b in self.beads], dtype=np.int32)
+                if b.mobile:
+                    self.mobile_labels.append(b.label)
+        if "measure_flux" in inp and any(b.label not in self.mobile_labels for b in self.mobile_beads):
+            raise ValueError("measure_flux: a mobile bead carries a label first seen on an immobile bead")
+        label_idx = np.array([self.mobile_labels.index(b.label) if b.label in self.mobile_labels else 0
+                              for b in self.mobile_beads], dtype=np.int32)
+        # device order: mobile beads, then immobile ones
+        self._dev_beads = self.mobile_beads + self.immobile_beads
 
         # box.py:633-637
         self.T = inp["T"]
@@ -144,7 +159,7 @@ class Box:
         cfg.factor_precision = 1 if precision == "mixed" else 0
         cfg.divergence_term = 1 if inp["divergence_term"] else 0
         # bead.py:401-413 connection matrix: bonded pairs skip the overlap test
-        self._exempt = _overlap_exemptions(self.beads)
+        self._exempt = _overlap_exemptions(self._dev_beads)
         cfg.n_overlap_exempt = len(self._exempt) // 2
         cfg.overlap_exempt = _lib.iptr(self._exempt) if len(self._exempt) else None
         cfg.box_length = float(self.box_length)
@@ -162,11 +177,18 @@ class Box:
             cfg.external_force[c] = float(self.Fex[c])
         self._cfg = cfg
 
-        N = len(self.beads)
+        N = len(self.mobile_beads)
         self.N = N
-        self._pos = np.ascontiguousarray(np.array([b.r for b in self.beads], dtype=np.float64).reshape(N, 3))
+        Ni = len(self.immobile_beads)
+        self._imm_xyz = np.ascontiguousarray(np.array([b.r for b in self.immobile_beads], dtype=np.float64)
+                                             .reshape(-1))
+        self._imm_radii = np.ascontiguousarray([b.a for b in self.immobile_beads], dtype=np.float64)
+        cfg.n_immobile = Ni
+        cfg.immobile_xyz = ptr(self._imm_xyz) if Ni else None
+        cfg.immobile_radii = ptr(self._imm_radii) if Ni else None
+        self._pos = np.ascontiguousarray(np.array([b.r for b in self.mobile_beads], dtype=np.float64).reshape(N, 3))
         self._bind_views()
-        radii = np.ascontiguousarray([b.a for b in self.beads], dtype=np.float64)
+        radii = np.ascontiguousarray([b.a for b in self.mobile_beads], dtype=np.float64)
         handle = ctypes_void_p()
         rc = lib.pbx_sim_create(cfg, N, ptr(radii), ptr(self._pos.reshape(-1)), _lib.iptr(label_idx), handle)
         _lib.check_rc(rc, "Box")
@@ -196,11 +218,13 @@ class Box:
         f.lennard_jones_6 = 1 if inp["lennard_jones_6"] else 0
         f.lennard_jones_12 = 1 if inp["lennard_jones_12"] else 0
         f.lennard_jones_alpha = float(inp["lennard_jones_alpha"])
-        self._hc = np.ascontiguousarray([b.hard_core_radius for b in self.beads], dtype=np.float64)
-        self._eps = np.ascontiguousarray([b.epsilon_LJ for b in self.beads], dtype=np.float64)
+        self._hc = np.ascontiguousarray([b.hard_core_radius for b in self._dev_beads], dtype=np.float64)
+        self._eps = np.ascontiguousarray([b.epsilon_LJ for b in self._dev_beads], dtype=np.float64)
         f.hard_core_radius = ptr(self._hc)
         f.epsilon_LJ = ptr(self._eps)
-        bonds, angles = bonded_lists(self.beads)
+        # device order = the reference's loop order (mobile beads, then
+        # immobile ones) and its id lookup (mobile beads first)
+        bonds, angles = bonded_lists(self._dev_beads)
         self.bonds, self.angles = bonds, angles
         self._bond_beads = np.ascontiguousarray([[q[0], q[1]] for q in bonds] or [[0, 0]], dtype=np.int32)
         self._bond_params = np.ascontiguousarray([[q[2], q[3]] for q in bonds] or [[0.0, 0.0]], dtype=np.float64)
@@ -226,7 +250,7 @@ class Box:
 
     # -- plumbing ---------------------------------------------------------
     def _bind_views(self):
-        for i, b in enumerate(self.beads):
+        for i, b in enumerate(self.mobile_beads):
             if hasattr(b, "_bind"):
                 b._bind(self._pos[i])
             else:

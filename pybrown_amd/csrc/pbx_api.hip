@@ -232,7 +232,7 @@ extern "C" int check_overlaps(double* positions, double* as, int number_of_beads
         if (connection_matrix) TRY(up(dm, connection_matrix, sizeof(int) * (size_t)N * N));
         TRY(df.alloc(sizeof(int)));
         PBX_HIP(hipMemset(df.p, 0, sizeof(int)));
-        TRY(pbx_launch_overlaps(dx.as<double>(), da.as<double>(), N, box_length, overlap_treshold,
+        TRY(pbx_launch_overlaps(dx.as<double>(), da.as<double>(), N, N, box_length, overlap_treshold,
                                 connection_matrix ? dm.as<int>() : nullptr, df.as<int>(), 0));
         PBX_HIP(hipMemcpy(&result, df.p, sizeof(int), hipMemcpyDeviceToHost));
         return PBX_OK;
@@ -327,6 +327,7 @@ extern "C" int pbx_gemm_nt(const double* A, const double* B, double* C, int M, i
 struct PbxSim {
     pbx_config cfg;
     int N = 0, n = 0, np = 0, ld = 0;
+    int Nt = 0;  // N mobile beads, then Nt - N immobile ones (pos and a only)
     double kBT = 0.0;
     int F_nonzero = 0;
     hipStream_t stream = nullptr;
@@ -376,7 +377,7 @@ struct PbxSim {
     std::vector<void*> owned;
     size_t device_bytes = 0;         // sum of the allocations in `owned`
     // force field (pbx_sim_set_forces): recomputed at the start of every step
-    int* ovl_mask = nullptr;         // connection matrix of the overlap test (N x N), or null
+    int* ovl_mask = nullptr;         // connection matrix of the overlap test (Nt x Nt), or null
     int forces_set = 0, forces_on = 0;
     PbxForceArgs fa{};
     double* e_part = nullptr;        // per-bead energies
@@ -598,8 +599,8 @@ int next_xi(PbxSim* s, const double** out) {
 
 int overlap_now(PbxSim* s, int* hit) {
     PBX_HIP(hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
-    TRY(pbx_launch_overlaps(s->pos, s->a, s->N, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask, s->flag,
-                            s->stream));
+    TRY(pbx_launch_overlaps(s->pos, s->a, s->Nt, s->N, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask,
+                            s->flag, s->stream));
     PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     PBX_HIP(hipStreamSynchronize(s->stream));
     *hit = s->flag_host[0];
@@ -684,7 +685,7 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         if (K > s->cfg.max_move_attempts - attempts) K = s->cfg.max_move_attempts - attempts;
         if (K < 1) K = 1;
         TRY(noise_ensure(s, K));
-        TRY(pbx_launch_attempt_batch(nohi ? nullptr : s->B, nohi ? s->Bd : nullptr, s->ld, s->np, s->N, s->pos,
+        TRY(pbx_launch_attempt_batch(nohi ? nullptr : s->B, nohi ? s->Bd : nullptr, s->ld, s->np, s->N, s->Nt, s->pos,
                                      s->xi_dev, ring_pos(s), ring_attempts(s), (int)K, c.sq2dt, s->Pbuf, s->cand,
                                      s->cflag, s->a, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask,
                                      s->stream));
@@ -790,9 +791,15 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: factor_precision must be 0 (FP64) or 1 (mixed)");
         return PBX_ERR_ARG;
     }
+    if (cfg->n_immobile < 0 || (cfg->n_immobile > 0 && (!cfg->immobile_xyz || !cfg->immobile_radii)) ||
+        (long long)N + cfg->n_immobile > (1 << 30)) {
+        pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: bad immobile beads");
+        return PBX_ERR_ARG;
+    }
     PbxSim* s = new PbxSim();
     s->cfg = *cfg;
     s->N = N;
+    s->Nt = N + cfg->n_immobile;
     s->n = 3 * N;
     s->np = (int)pbx_round_up(s->n, 64);
     s->ld = s->np;
@@ -809,8 +816,9 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         return PBX_ERR_HIP;
     }
     const size_t n3 = s->n;
-    if ((rc = sim_alloc(s, (void**)&s->pos, sizeof(double) * n3))) return fail(rc);
-    if ((rc = sim_alloc(s, (void**)&s->a, sizeof(double) * N))) return fail(rc);
+    const int Ni = cfg->n_immobile;
+    if ((rc = sim_alloc(s, (void**)&s->pos, sizeof(double) * 3 * (size_t)s->Nt))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->a, sizeof(double) * s->Nt))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->F, sizeof(double) * s->np))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->disp, sizeof(double) * s->np))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->drift, sizeof(double) * s->np))) return fail(rc);
@@ -850,8 +858,15 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         }
         return true;
     };
+    // immobile beads (box.py:649-668) follow the mobile ones and never move
     if (!hip_ok(hipMemcpyAsync(s->pos, xyz, sizeof(double) * n3, hipMemcpyHostToDevice, s->stream), "pos") ||
         !hip_ok(hipMemcpyAsync(s->a, radii, sizeof(double) * N, hipMemcpyHostToDevice, s->stream), "radii") ||
+        (Ni > 0 && !hip_ok(hipMemcpyAsync(s->pos + n3, cfg->immobile_xyz, sizeof(double) * 3 * (size_t)Ni,
+                                          hipMemcpyHostToDevice, s->stream),
+                           "immobile xyz")) ||
+        (Ni > 0 && !hip_ok(hipMemcpyAsync(s->a + N, cfg->immobile_radii, sizeof(double) * Ni, hipMemcpyHostToDevice,
+                                          s->stream),
+                           "immobile radii")) ||
         !hip_ok(hipMemsetAsync(s->xipad, 0, sizeof(double) * s->np, s->stream), "xipad") ||
         !hip_ok(hipMemsetAsync(s->info, 0, sizeof(int), s->stream), "info") ||
         !hip_ok(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream), "flux"))
@@ -900,16 +915,21 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         }
     }
     if (cfg->n_overlap_exempt > 0) {
-        // bead.py:401-413 build_connection_matrix: pairs the overlap test skips
-        if (!cfg->overlap_exempt) return fail(PBX_ERR_ARG);
-        std::vector<int> m((size_t)N * N, 0);
+        // bead.py:401-413 build_connection_matrix: pairs the overlap test
+        // skips (pairs of two immobile beads are skipped without it)
+        const int Nt = s->Nt;
+        if (!cfg->overlap_exempt) {
+            pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: overlap_exempt is null");
+            return fail(PBX_ERR_ARG);
+        }
+        std::vector<int> m((size_t)Nt * Nt, 0);
         for (int q = 0; q < cfg->n_overlap_exempt; ++q) {
             const int i = cfg->overlap_exempt[2 * q], j = cfg->overlap_exempt[2 * q + 1];
-            if (i < 0 || i >= N || j < 0 || j >= N) {
+            if (i < 0 || i >= Nt || j < 0 || j >= Nt) {
                 pbx_set_error(PBX_ERR_ARG, "pbx_sim_create: overlap exemption index out of range");
                 return fail(PBX_ERR_ARG);
             }
-            m[(size_t)i * N + j] = m[(size_t)j * N + i] = 1;
+            m[(size_t)i * Nt + j] = m[(size_t)j * Nt + i] = 1;
         }
         if ((rc = sim_alloc(s, (void**)&s->ovl_mask, sizeof(int) * m.size()))) return fail(rc);
         if (!hip_ok(hipMemcpy(s->ovl_mask, m.data(), sizeof(int) * m.size(), hipMemcpyHostToDevice), "mask"))
@@ -1297,19 +1317,20 @@ extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
         pbx_set_error(PBX_ERR_STATE, "set_forces: the force field is set once per simulation");
         return PBX_ERR_STATE;
     }
-    const int N = s->N;
+    const int N = s->N, Nt = s->Nt;
     for (int q = 0; q < 2 * f->n_bonds; ++q)
-        if (f->bond_beads[q] < 0 || f->bond_beads[q] >= N) {
+        if (f->bond_beads[q] < 0 || f->bond_beads[q] >= Nt) {
             pbx_set_error(PBX_ERR_ARG, "set_forces: bond bead index out of range");
             return PBX_ERR_ARG;
         }
     for (int q = 0; q < 3 * f->n_angles; ++q)
-        if (f->angle_beads[q] < 0 || f->angle_beads[q] >= N) {
+        if (f->angle_beads[q] < 0 || f->angle_beads[q] >= Nt) {
             pbx_set_error(PBX_ERR_ARG, "set_forces: angle bead index out of range");
             return PBX_ERR_ARG;
         }
     PbxForceArgs& a = s->fa;
     a.N = N;
+    a.Nt = Nt;
     a.L = s->cfg.box_length;
     a.unit = f->unit;
     a.lj = (f->lennard_jones_6 && f->lennard_jones_12) ? 3 : f->lennard_jones_6 ? 1 : f->lennard_jones_12 ? 2 : 0;
@@ -1343,32 +1364,35 @@ extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
         return PBX_OK;
     };
     if (a.lj) {
-        std::vector<double> hc(N), ep(N);
-        for (int i = 0; i < N; ++i) {
+        std::vector<double> hc(Nt), ep(Nt);
+        for (int i = 0; i < Nt; ++i) {
             hc[i] = f->hard_core_radius ? f->hard_core_radius[i] : 0.0;
             ep[i] = f->epsilon_LJ ? f->epsilon_LJ[i] : 0.0;
         }
-        TRY(upload((void**)&a.hc, hc.data(), sizeof(double) * N));
-        TRY(upload((void**)&a.eps, ep.data(), sizeof(double) * N));
+        TRY(upload((void**)&a.hc, hc.data(), sizeof(double) * Nt));
+        TRY(upload((void**)&a.eps, ep.data(), sizeof(double) * Nt));
     }
-    // per-bead incidence lists (counting sort by bead, entries in list order)
+    // per-bead incidence lists of the mobile members (counting sort by bead,
+    // entries in list order); immobile beads get no force
     std::vector<int> bptr(N + 1, 0), aptr(N + 1, 0);
-    for (int q = 0; q < f->n_bonds; ++q) {
-        ++bptr[f->bond_beads[2 * q] + 1];
-        ++bptr[f->bond_beads[2 * q + 1] + 1];
-    }
+    for (int q = 0; q < f->n_bonds; ++q)
+        for (int r = 0; r < 2; ++r)
+            if (f->bond_beads[2 * q + r] < N) ++bptr[f->bond_beads[2 * q + r] + 1];
     for (int q = 0; q < f->n_angles; ++q)
-        for (int r = 0; r < 3; ++r) ++aptr[f->angle_beads[3 * q + r] + 1];
+        for (int r = 0; r < 3; ++r)
+            if (f->angle_beads[3 * q + r] < N) ++aptr[f->angle_beads[3 * q + r] + 1];
     for (int i = 0; i < N; ++i) {
         bptr[i + 1] += bptr[i];
         aptr[i + 1] += aptr[i];
     }
-    std::vector<int> bent(2 * f->n_bonds), aent(3 * f->n_angles), bfill(bptr.begin(), bptr.end() - 1),
+    std::vector<int> bent(bptr[N]), aent(aptr[N]), bfill(bptr.begin(), bptr.end() - 1),
         afill(aptr.begin(), aptr.end() - 1);
     for (int q = 0; q < f->n_bonds; ++q)
-        for (int r = 0; r < 2; ++r) bent[bfill[f->bond_beads[2 * q + r]]++] = 2 * q + r;
+        for (int r = 0; r < 2; ++r)
+            if (f->bond_beads[2 * q + r] < N) bent[bfill[f->bond_beads[2 * q + r]]++] = 2 * q + r;
     for (int q = 0; q < f->n_angles; ++q)
-        for (int r = 0; r < 3; ++r) aent[afill[f->angle_beads[3 * q + r]]++] = 4 * q + r;
+        for (int r = 0; r < 3; ++r)
+            if (f->angle_beads[3 * q + r] < N) aent[afill[f->angle_beads[3 * q + r]]++] = 4 * q + r;
     TRY(upload((void**)&a.bond_ptr, bptr.data(), sizeof(int) * (N + 1)));
     TRY(upload((void**)&a.bond_ent, bent.data(), sizeof(int) * bent.size()));
     TRY(upload((void**)&a.bond_ij, f->bond_beads, sizeof(int) * 2 * f->n_bonds));

@@ -72,6 +72,32 @@ __device__ __forceinline__ bool pbx_pair_overlaps(double dx, double dy, double d
     return dist <= ai + aj + thr;
 }
 
+// Tiles of 16 x 16 bead pairs for an overlap test over Nt beads of which the
+// first Nm move (the rest are immobile, box.py:649-668): the lower triangle
+// of the Tm = ceil(Nm / 16) mobile tile rows, then Tt - Tm further tile rows
+// of Tm tiles each.  Every pair (i, j) with j < i and j < Nm lies in exactly
+// one tile; pairs of two immobile beads, which the reference's connection
+// matrix exempts (bead.py:401-413), lie in none except the few sharing a
+// tile with mobile ones (callers test j < Nm).
+__device__ __forceinline__ void pbx_pair_tile(int t, int Tm, int& bi, int& bj) {
+    const int tri = Tm * (Tm + 1) / 2;
+    if (t >= tri) {
+        t -= tri;
+        bi = Tm + t / Tm;
+        bj = t % Tm;
+        return;
+    }
+    int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    bi = r;
+    bj = t - r * (r + 1) / 2;
+}
+static inline int pbx_pair_tiles(int Nm, int Nt) {
+    const int Tm = (Nm + 15) / 16, Tt = (Nt + 15) / 16;
+    return Tm * (Tm + 1) / 2 + (Tt - Tm) * Tm;
+}
+
 // Host-side check shared by every launcher that reduces by a box length.
 static inline bool pbx_box_ok(double L) { return L > 0.0 && L < 1e300; }
 #define PBX_CHECK_BOX(L)                                                                   \
@@ -96,7 +122,8 @@ int pbx_launch_smith_packed(const double* a, const double* P, int N, double L, d
 int pbx_launch_pointers(const double* xyz, int N, double L, double* P3, hipStream_t s);
 int pbx_launch_immobile_pointers(const double* xm, const double* xi, int nm, int ni, double L, double* P3,
                                  hipStream_t s);
-int pbx_launch_overlaps(const double* xyz, const double* a, int N, double L, double thr, const int* mask,
+// overlap test over Nt beads whose first Nm move; mask (Nt x Nt) or null
+int pbx_launch_overlaps(const double* xyz, const double* a, int Nt, int Nm, double L, double thr, const int* mask,
                         int* flag, hipStream_t s);
 
 // lubrication (pbx_lub.hip): C = ((dR * mul) / div) + C
@@ -111,8 +138,10 @@ int pbx_launch_jo_2b(double ai, double aj, double rx, double ry, double rz, doub
 int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, int mixed = 0);
 // Force field of the step (pbx_forces.hip): per-bead incidence lists hold
 // entries 2*bond+role (role 0: first bead) and 4*angle+role (roles 0..2).
+// Bead indices >= N are immobile (no force row; box.py:649-668).
 struct PbxForceArgs {
-    int N;
+    int N;   // mobile beads
+    int Nt;  // mobile + immobile
     double L;
     double fex[3];
     int reg_on[3];
@@ -148,7 +177,8 @@ size_t pbx_dense_work_doubles(int np);
 int pbx_attempt_kmax();
 int pbx_attempt_slab_width(int np);
 size_t pbx_attempt_partial_doubles(int np);
-int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
+// (pos and a hold the N mobile beads, then Nt - N immobile ones)
+int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, int Nt, const double* pos,
                              const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
                              double* cand, int* cflag, const double* a, double L, double thr, const int* mask,
                              hipStream_t s);

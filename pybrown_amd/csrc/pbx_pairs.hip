@@ -403,16 +403,17 @@ __global__ void immobile_pointers_kernel(const double* __restrict__ xm, const do
 
 // clib/bead.c:169-260: overlap iff no raw axis difference lies in the open
 // skip band (S, L-S) / (-(L-S), -S) and the minimum-image distance is at most
-// S + threshold (S = a_i + a_j).
+// S + threshold (S = a_i + a_j).  Beads Nm.. are immobile: their mutual pairs
+// are not tested (the connection matrix of bead.py:401-413 exempts them).
 __global__ __launch_bounds__(256) void overlaps_kernel(const double* __restrict__ xyz, const double* __restrict__ a,
-                                                       int N, double L, double thr, const int* __restrict__ mask,
-                                                       int* flag) {
+                                                       int N, int Nm, double L, double thr,
+                                                       const int* __restrict__ mask, int* flag) {
     int bi, bj;
-    tile_decode(blockIdx.x, bi, bj);
+    pbx_pair_tile(blockIdx.x, (Nm + TB - 1) / TB, bi, bj);
     const int ti = threadIdx.x / TB, tj = threadIdx.x % TB;
     const int i = bi * TB + ti, j = bj * TB + tj;
     bool hit = false;
-    if (i < N && j < i && !(mask && mask[(size_t)j + (size_t)N * i]))
+    if (i < N && j < i && j < Nm && !(mask && mask[(size_t)j + (size_t)N * i]))
         hit = pbx_pair_overlaps(xyz[3 * i] - xyz[3 * j], xyz[3 * i + 1] - xyz[3 * j + 1], xyz[3 * i + 2] - xyz[3 * j + 2],
                                 a[i], a[j], L, thr);
     if (__any(hit) && (threadIdx.x % 64) == 0) atomicOr(flag, 1);
@@ -517,11 +518,16 @@ int pbx_launch_immobile_pointers(const double* xm, const double* xi, int nm, int
     return PBX_OK;
 }
 
-int pbx_launch_overlaps(const double* xyz, const double* a, int N, double L, double thr, const int* mask, int* flag,
-                        hipStream_t s) {
-    if (N <= 1) return PBX_OK;
+int pbx_launch_overlaps(const double* xyz, const double* a, int Nt, int Nm, double L, double thr, const int* mask,
+                        int* flag, hipStream_t s) {
+    if (Nt <= 1 || Nm <= 0) return PBX_OK;
+    if (Nm > Nt) {
+        pbx_set_error(PBX_ERR_ARG, "overlaps: %d mobile beads of %d", Nm, Nt);
+        return PBX_ERR_ARG;
+    }
     PBX_CHECK_BOX(L);
-    hipLaunchKernelGGL(overlaps_kernel, dim3(tiles_for(N)), dim3(256), 0, s, xyz, a, N, L, thr, mask, flag);
+    hipLaunchKernelGGL(overlaps_kernel, dim3(pbx_pair_tiles(Nm, Nt)), dim3(256), 0, s, xyz, a, Nt, Nm, L, thr, mask,
+                       flag);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -372,27 +372,29 @@ __global__ __launch_bounds__(128) void cand_kernel(const double* __restrict__ po
     }
 }
 
-// Overlap flags of K candidates: lower 16 x 16 bead tiles, one pair per
-// thread; bit k of cflag[0] is set when candidate k overlaps; pairs marked
-// in `mask` (N x N, may be null) are exempt, as in bead.c:180.  A pair can overlap in some candidate only if
-// its current minimum-image distance is at most S + thr + delta_i + delta_j
-// (the torus distance moves by at most the two displacements, and the
-// reference test flags only pairs within S + thr); other pairs skip the loop.
+// Overlap flags of K candidates: 16 x 16 bead tiles (pbx_pair_tile), one
+// pair per thread; bit k of cflag[0] is set when candidate k overlaps; pairs
+// marked in `mask` (Nt x Nt, may be null) are exempt, as in bead.c:180.
+// Beads N..Nt-1 are immobile: the same position in every candidate, no
+// displacement, and no pairs among themselves.  A pair can overlap in some
+// candidate only if its current minimum-image distance is at most
+// S + thr + delta_i + delta_j (the torus distance moves by at most the two
+// displacements, and the reference test flags only pairs within S + thr);
+// other pairs skip the loop.
 __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __restrict__ cand,
                                                              const double* __restrict__ pos,
                                                              const double* __restrict__ delta,
-                                                             const double* __restrict__ a, int N, int K, double L,
-                                                             double thr, const int* __restrict__ mask, int* cflag) {
-    const int t = blockIdx.x;
-    int bi = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
-    while (bi * (bi + 1) / 2 > t) --bi;
-    const int bj = t - bi * (bi + 1) / 2;
+                                                             const double* __restrict__ a, int N, int Nt, int K,
+                                                             double L, double thr, const int* __restrict__ mask,
+                                                             int* cflag) {
+    int bi, bj;
+    pbx_pair_tile(blockIdx.x, (N + 15) / 16, bi, bj);
     const int i = bi * 16 + (threadIdx.x >> 4), j = bj * 16 + (threadIdx.x & 15);
     bool near = false;
     double ai = 0.0, aj = 0.0;
-    // mask: the reference's connection matrix (bonded pairs are not tested)
-    if (i < N && j < i && !(mask && mask[(size_t)j + (size_t)N * i])) {
+    // j < i and j < N: j moves; i moves when i < N.  mask: the reference's
+    // connection matrix (bonded pairs are not tested)
+    if (i < Nt && j < i && j < N && !(mask && mask[(size_t)j + (size_t)Nt * i])) {
         ai = a[i];
         aj = a[j];
         double s2 = 0.0;
@@ -400,20 +402,21 @@ __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __res
             const double d = pbx_min_image(pos[3 * i + c] - pos[3 * j + c], L);
             s2 += d * d;
         }
-        const double reach = ai + aj + thr + delta[i] + delta[j];
+        const double reach = ai + aj + thr + (i < N ? delta[i] : 0.0) + delta[j];
         // a relative margin far above the rounding of these few operations
         near = !(sqrt(s2) > reach * (1.0 + 1e-9) + 1e-9 * L);
     }
     if (!__any(near)) return;
     const int n = 3 * N;
+    const bool fixed_i = i >= N;
     unsigned m = 0;  // bit k: some pair of this wave overlaps in candidate k
 #pragma unroll 4
     for (int k = 0; k < K; ++k) {
         bool hit = false;
         if (near) {
             const double* C = cand + (size_t)k * n;
-            hit = pbx_pair_overlaps(C[3 * i] - C[3 * j], C[3 * i + 1] - C[3 * j + 1], C[3 * i + 2] - C[3 * j + 2], ai,
-                                    aj, L, thr);
+            const double* Ri = fixed_i ? pos + 3 * i : C + 3 * i;
+            hit = pbx_pair_overlaps(Ri[0] - C[3 * j], Ri[1] - C[3 * j + 1], Ri[2] - C[3 * j + 2], ai, aj, L, thr);
         }
         if (__any(hit)) m |= 1u << k;
     }
@@ -605,12 +608,13 @@ size_t pbx_attempt_partial_doubles(int np) {
 
 int pbx_attempt_kmax() { return KC_MAX; }
 
-int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, const double* pos,
+int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, int N, int Nt, const double* pos,
                              const double* ring, int ring_pos, int ring_att, int K, double sq2dt, double* P,
                              double* cand, int* cflag, const double* a, double L, double thr, const int* mask,
                              hipStream_t s) {
-    if (K < 1 || K > KC_MAX || ring_pos < 0 || ring_pos >= ring_att) {
-        pbx_set_error(PBX_ERR_ARG, "attempt batch: K=%d ring_pos=%d ring_att=%d", K, ring_pos, ring_att);
+    if (K < 1 || K > KC_MAX || ring_pos < 0 || ring_pos >= ring_att || Nt < N) {
+        pbx_set_error(PBX_ERR_ARG, "attempt batch: K=%d ring_pos=%d ring_att=%d N=%d Nt=%d", K, ring_pos, ring_att, N,
+                      Nt);
         return PBX_ERR_ARG;
     }
     PBX_CHECK_BOX(L);
@@ -636,10 +640,9 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
     hipLaunchKernelGGL(cand_kernel, dim3((N + CB - 1) / CB), dim3(32 * CB), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
                        ring_pos, ring_att, cand, delta, cflag);
     PBX_HIP(hipGetLastError());
-    if (N > 1) {
-        const int T16 = (N + 15) / 16;
-        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(T16 * (T16 + 1) / 2), dim3(256), 0, s, cand, pos, delta, a, N, K,
-                           L, thr, mask, cflag);
+    if (Nt > 1) {
+        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(pbx_pair_tiles(N, Nt)), dim3(256), 0, s, cand, pos, delta, a, N,
+                           Nt, K, L, thr, mask, cflag);
         PBX_HIP(hipGetLastError());
     }
     return PBX_OK;

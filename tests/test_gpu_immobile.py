@@ -1,0 +1,170 @@
+"""Immobile beads in the device simulation (box.py:649-668) against the
+oracle step: they stay put, are absent from D and the noise, are tested for
+overlaps against mobile beads only, and act on mobile beads through the force
+field (interactions.py:185-416).
+
+Bar: forces within 1e-12 relative; positions within 1e-10 relative per step
+(bit-exact for nohi without forces); attempt, draw and flux counts equal."""
+import numpy as np
+import pytest
+
+from oracle import forces as of
+from oracle.box_oracle import OracleBox
+
+pytestmark = pytest.mark.gpu
+
+FLUX = {"normal": [1.0, 0.0, 0.0], "plane_point": [0.0, 0.0, 0.0]}
+BASE = {"box_length": 35.0, "T": 298.0, "viscosity": 0.01, "seed": 0}
+
+
+def _beads(xyz, a, labels, mobile=None, hc=None, eps=None):
+    from pybrown_amd.bead import Bead
+    out = []
+    for k, (p, r, lab) in enumerate(zip(xyz, a, labels)):
+        b = Bead(np.array(p, float), r, label=lab, bead_id=k + 1,
+                 hard_core_radius=None if hc is None else hc[k], epsilon_LJ=0.0 if eps is None else eps[k])
+        if mobile is not None:
+            b.mobile = bool(mobile[k])
+        out.append(b)
+    return out
+
+
+def _oracle(beads, xyz, inp):
+    mobile = [b.mobile and b.label not in inp.get("immobile_labels", []) for b in beads]
+    bonds, angles = of.bonded_lists(beads, mobile=mobile)
+    return OracleBox(xyz, [b.a for b in beads], [b.label for b in beads], inp,
+                     hard_core=[b.hard_core_radius for b in beads], epsilon_LJ=[b.epsilon_LJ for b in beads],
+                     bonds=bonds, angles=angles, mobile=mobile)
+
+
+def _run(beads, xyz, inp, steps, dt, tol=1e-10):
+    from pybrown_amd.box import Box
+    xyz = np.array(xyz, float)
+    g = Box(beads, inp)
+    o = _oracle(beads, xyz, inp)
+    fixed = {id(b): b.r.copy() for b in g.immobile_beads}
+    try:
+        worst = 0.0
+        for j in range(steps):
+            g.propagate(dt)
+            o.propagate(dt)
+            pos_g = np.array([b.r for b in g.beads])
+            err = np.abs(pos_g - o.r).max() / np.abs(o.r).max()
+            worst = max(worst, err)
+            assert err <= tol, "step %d: position error %.3e" % (j, err)
+            assert g.last_move_attempts == o.attempt_log[-1], j
+            if "measure_flux" in inp:
+                assert g.net_flux == o.net_flux, (j, g.net_flux, o.net_flux)
+        assert g.draw_count == o.draw_count == 3 * len(g.mobile_beads) * sum(o.attempt_log)
+        for b in g.immobile_beads:
+            assert np.array_equal(b.r, fixed[id(b)])
+        return worst, o
+    finally:
+        g.close()
+
+
+def _grid(coords, a):
+    xyz = np.array([[x, y, z] for x in coords for y in coords for z in coords], float)
+    return xyz, np.full(len(xyz), a)
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_forces_with_immobile_beads_match_oracle(seed):
+    """LJ 6-12 over mobile-mobile and mobile-immobile pairs, bonds and angles
+    listed by mobile and by immobile beads, region-limited external force."""
+    from pybrown_amd.box import Box
+    rng = np.random.RandomState(seed)
+    N = 48
+    xyz = np.cumsum(rng.normal(size=(N, 3)) * 1.4, axis=0)
+    xyz -= xyz.mean(axis=0)
+    mobile = rng.rand(N) < 0.6
+    beads = _beads(xyz, np.ones(N), ["A"] * N, mobile=mobile, hc=rng.uniform(0.8, 1.2, N),
+                   eps=rng.uniform(0.5, 1.5, N))
+    for i in range(N - 1):  # bonds listed in both directions along the walk
+        a_, b_ = (beads[i], beads[i + 1]) if i % 2 else (beads[i + 1], beads[i])
+        a_.bonded_with.append(b_.bead_id)
+        a_.bonded_how[b_.bead_id] = [2.0, rng.uniform(0.5, 2.0)]
+    for i in range(N - 2):
+        a_ = beads[i] if i % 3 else beads[i + 2]
+        ids = [beads[i + 1].bead_id, beads[i + 2].bead_id] if i % 3 else [beads[i + 1].bead_id, beads[i].bead_id]
+        a_.angled_with.append(ids)
+        a_.angled_how[tuple(ids)] = [rng.uniform(90.0, 150.0), rng.uniform(0.1, 1.0)]
+    L = 40.0
+    inp = {"box_length": L, "T": 298.0, "viscosity": 0.01, "seed": 3, "hydrodynamics": "nohi",
+           "lennard_jones_6": True, "lennard_jones_12": True, "lennard_jones_alpha": 3.0, "energy_unit": "kcal/mol",
+           "external_force": [0.3, -0.2, 0.1], "external_force_region": {"x": [-4.0, 6.0]}, "check_overlaps": False}
+    box = Box(beads, inp)
+    try:
+        box.propagate(1e-9)
+        bonds, angles = of.bonded_lists(beads, mobile=mobile)
+        Fo, Eo = of.forces_and_energy(xyz, L, Fex=inp["external_force"], region=inp["external_force_region"],
+                                      lj6=True, lj12=True, alpha=3.0, hc=[b.hard_core_radius for b in beads],
+                                      eps=[b.epsilon_LJ for b in beads], bonds=bonds, angles=angles,
+                                      unit="kcal/mol", mobile=mobile)
+        Fg = box.F
+        print("seed %d: %d mobile of %d, max |dF| %.3e of %.3e, E %.8e vs %.8e" % (
+            seed, mobile.sum(), N, np.abs(Fg - Fo).max(), np.abs(Fo).max(), box.E, Eo))
+        assert Fg.size == 3 * mobile.sum()
+        assert np.abs(Fg - Fo).max() <= 1e-12 * np.abs(Fo).max()
+        assert box.E == pytest.approx(Eo, rel=1e-12)
+    finally:
+        box.close()
+
+
+def test_nohi_rejections_against_immobile_beads_bit_exact():
+    xyz, a = _grid([-2.2, 0.0, 2.2], 1.0)
+    labels = ["OBS" if k % 3 == 0 else ("A" if k % 2 else "B") for k in range(len(a))]
+    inp = dict(BASE, hydrodynamics="nohi", seed=4, measure_flux=FLUX, immobile_labels=["OBS"])
+    worst, o = _run(_beads(xyz, a, labels), xyz, inp, 40, 0.02, tol=0.0)
+    assert worst == 0.0 and sum(o.attempt_log) > len(o.attempt_log)
+
+
+def test_ermak_rpy_with_obstacles_and_lj():
+    """Mobile beads among fixed obstacles: hydrodynamics over the mobile beads
+    only, LJ-12 walls from the obstacles, overlap rejections against them."""
+    xyz, a = _grid([-5.2, -2.6, 0.0, 2.6, 5.2], 1.0)
+    mobile = np.array([(k * 7) % 5 != 3 for k in range(len(a))])  # bead 0 moves: its label is mobile
+    beads = _beads(xyz, a, ["XXX"] * len(a), mobile=mobile, hc=np.full(len(a), 1.0), eps=np.full(len(a), 0.6))
+    inp = dict(BASE, hydrodynamics="rpy", seed=5, lennard_jones_12=True, energy_unit="kcal/mol",
+               overlap_treshold=0.05, measure_flux=FLUX)
+    worst, o = _run(beads, xyz, inp, 20, 0.02)
+    print("rpy with %d obstacles: worst %.3e, attempts %s" % ((~mobile).sum(), worst, o.attempt_log))
+
+
+def test_midpoint_rpy_smith_with_immobile_beads():
+    xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
+    labels = ["WALL" if x[2] < -1.0 else "XXX" for x in xyz]
+    inp = dict(BASE, hydrodynamics="rpy_smith", ewald_real=1, ewald_imag=1, seed=9, propagation_scheme="midpoint",
+               immobile_labels=["WALL"], measure_flux=FLUX, box_length=12.0)
+    worst, o = _run(_beads(xyz, a, labels), xyz, inp, 10, 0.05)
+    print("midpoint with immobile wall: worst %.3e, attempts %s" % (worst, o.attempt_log))
+
+
+def test_chain_tethered_to_immobile_anchor():
+    """Bonds and angles between mobile and immobile beads (listed by either),
+    with bonded neighbours closer than their radii and overlap checks on."""
+    rng = np.random.RandomState(21)
+    N = 16
+    xyz = np.zeros((N, 3))
+    for i in range(1, N):
+        while True:
+            v = rng.normal(size=3)
+            p = xyz[i - 1] + 1.8 * v / np.linalg.norm(v)
+            if i < 2 or np.linalg.norm(p - xyz[:i - 1], axis=1).min() > 2.3:
+                break
+        xyz[i] = p
+    mobile = np.ones(N, bool)
+    mobile[[0, 1, 7]] = False  # anchor at the start and one fixed bead inside
+    beads = _beads(xyz, np.ones(N), ["XXX"] * N, mobile=mobile)
+    for i in range(N - 1):
+        a_, b_ = (beads[i], beads[i + 1]) if i % 2 else (beads[i + 1], beads[i])
+        a_.bonded_with.append(b_.bead_id)
+        a_.bonded_how[b_.bead_id] = [1.8, 0.7]
+    for i in range(N - 2):
+        ids = [beads[i + 1].bead_id, beads[i + 2].bead_id]
+        beads[i].angled_with.append(ids)
+        beads[i].angled_how[tuple(ids)] = [120.0, 0.3]
+    inp = {"box_length": 1.0e6, "T": 298.0, "viscosity": 0.01, "seed": 8, "hydrodynamics": "rpy",
+           "energy_unit": "kcal/mol", "check_overlaps": True, "overlap_treshold": 0.3}
+    worst, o = _run(beads, xyz, inp, 15, 0.05)
+    print("tethered chain: worst %.3e, E %.4f, attempts %s" % (worst, o.E, o.attempt_log))
