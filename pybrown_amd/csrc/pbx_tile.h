@@ -186,16 +186,16 @@ __device__ __forceinline__ double bcast(double x, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-// 1/sqrt(d): hardware estimate refined by two Newton steps (within an ulp
-// or two of the correctly rounded value; d <= 0 or NaN propagates NaN).
+// 1/sqrt(d): the hardware estimate y0 (relative error below 6e-8 on
+// gfx950) corrected to second order, y0 (1 + e/2 + 3 e^2 / 8) with
+// e = 1 - d y0^2: the truncation term is O(e^3), far below an ulp, and the
+// result is within one ulp of 1/sqrt(d) (as two Newton steps are), with four
+// dependent operations after the estimate instead of six.  d <= 0 or NaN
+// propagates NaN.
 __device__ __forceinline__ double rsqrt_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    double h = 0.5 * d;
-    double r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    r = fma(-h * y, y, 0.5);
-    y = fma(y, r, y);
-    return y;
+    const double y = __builtin_amdgcn_rsq(d);
+    const double e = fma(-d * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
 }
 
 // Factor the SPD tile held in LDS (lower triangle, stride SP) in place and
@@ -318,9 +318,14 @@ __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0,
 // Wave-specialised tile factorisation: same contract as tile_factor (L in sL,
 // inv(L) in sW, diagonal in sd, status in info), with the four waves working
 // concurrently on one 64 x 64 tile instead of in lock step:
-//   wave 0  the serial L chain: factors 8-column panel p (lane = row, pivots
-//           broadcast with readlane), publishes it, then applies panel p to
-//           panel p+1's columns itself (row entries read as LDS broadcasts);
+//   wave 0  the serial L chain: factors 8-column panel p (lane = row),
+//           publishes it, then applies panel p to panel p+1's columns
+//           itself (row entries read as LDS broadcasts).  Every lane first
+//           factors the panel's 8 x 8 diagonal block in its own registers
+//           (read once from LDS) and then solves for its row, so no column
+//           waits on a cross-lane broadcast.  The operations and their order
+//           are those of a column-by-column chain with the pivots and
+//           multipliers broadcast from their lanes, and give the same bits;
 //   waves 2, 3  hold the trailing L blocks in MFMA accumulators, apply each
 //           published panel and stage the columns of panel p+2 (updated
 //           through p) one panel ahead of wave 0;
@@ -351,6 +356,7 @@ __device__ __forceinline__ void lds_release_add(int* f, int v) {
 __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int col0, int* info) {
     __shared__ int fl[2];      // [0]: panels published by wave 0; [1]: staged panel inputs (2 per panel)
     __shared__ double sy[64];  // 1 / L_jj
+    __shared__ double sB[64];  // the next panel's diagonal block, row-major 8 x 8
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
     // trailing L blocks (block row X, block column Y) of waves 2 and 3; the
@@ -391,17 +397,37 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
             const int c0 = 8 * p;
             int first_bad = -1;
             double yq[8];
+            // Dm: lower 8 x 8 diagonal block, packed (k, m) -> k (k + 1) / 2 + m,
+            // from sL for the first panel, else as the previous step left it in sB
+            double Dm[36];
+            const double* src = (p == 0) ? sL : sB;
+            const int lds = (p == 0) ? SP : 8;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int m = 0; m <= k; ++m) Dm[k * (k + 1) / 2 + m] = src[k * lds + m];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
-                const int j = c0 + q;
-                const double d = bcast(v[q], j);
+                const double d = Dm[q * (q + 1) / 2 + q];
                 if (!(d > 0.0) && first_bad < 0) first_bad = q;
                 const double y = rsqrt_nr(d);
                 yq[q] = y;
-                // the whole column is scaled: entries above the pivot are dead
-                v[q] = v[q] * y;
 #pragma unroll
-                for (int q2 = q + 1; q2 < 8; ++q2) v[q2] = fma(-v[q], bcast(v[q], c0 + q2), v[q2]);
+                for (int k = q; k < 8; ++k) Dm[k * (k + 1) / 2 + q] *= y;
+#pragma unroll
+                for (int m = q + 1; m < 8; ++m)
+#pragma unroll
+                    for (int k = m; k < 8; ++k)
+                        Dm[k * (k + 1) / 2 + m] =
+                            fma(-Dm[k * (k + 1) / 2 + q], Dm[m * (m + 1) / 2 + q], Dm[k * (k + 1) / 2 + m]);
+            }
+            // this lane's row of the panel (rows above c0 are dead)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                double t = v[q];
+#pragma unroll
+                for (int k = 0; k < q; ++k) t = fma(-v[k], Dm[q * (q + 1) / 2 + k], t);
+                v[q] = t * yq[q];
             }
             if (lane >= c0) {
 #pragma unroll
@@ -430,8 +456,13 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
                 for (int q = 0; q < 8; ++q) t = fma(-v[q], Lr[q], t);
                 nv[q2] = t;
             }
+            if (lane >= n0 && lane < n0 + 8) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) sB[(lane - n0) * 8 + q] = nv[q];
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) v[q] = nv[q];
+            __builtin_amdgcn_wave_barrier();
         }
     } else if (w >= 2) {
         for (int p = 0; p < 7; ++p) {
