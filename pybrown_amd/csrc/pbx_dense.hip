@@ -149,6 +149,38 @@ __device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
     jj = t - r * (r + 1) / 2;
 }
 
+// Trailing tile (ii, jj), jj <= ii < m, of the r-th trailing workgroup of a
+// launch (the first tile, (0, 0), skipped when skip0).  The tiles of one
+// launch are independent, so the order only decides locality.  Workgroups
+// are dealt to the eight XCDs round-robin by index, so XCD slot r % 8 takes
+// one contiguous chunk of the tile sequence, and the sequence runs through
+// 16-tile column bands (within a band, rows top-down): an XCD then keeps one
+// band's panel rows in its L2 while the row panels stream past once.
+__device__ __forceinline__ void trailing_decode(int r, int m, bool skip0, int& ii, int& jj) {
+    const int off = skip0 ? 1 : 0;
+    const int Nt = m * (m + 1) / 2 - off;
+    const int s = r & 7, k = r >> 3;
+    const int base = Nt >> 3, extra = Nt & 7;
+    const int t = s * base + min(s, extra) + k + off;
+    int c = 0, start = 0;
+    for (;;) {  // ends: t < m (m + 1) / 2, the sum of the band counts
+        const int w = min(16, m - c);
+        const int cnt = w * (w + 1) / 2 + max(0, m - c - 16) * 16;
+        if (t < start + cnt) break;
+        start += cnt;
+        c += 16;
+    }
+    const int w = min(16, m - c), loc = t - start, tri = w * (w + 1) / 2;
+    if (loc < tri) {
+        tri_decode(loc, ii, jj);
+        ii += c;
+        jj += c;
+    } else {
+        ii = c + 16 + (loc - tri) / 16;
+        jj = c + (loc - tri) % 16;
+    }
+}
+
 // A_ij -= L_ik L_jk^T over the trailing lower tiles; the workgroup owning
 // (k+1, k+1) (blockIdx 0) then factors that tile.
 template <int V>
@@ -341,6 +373,27 @@ __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int
                                              double* sB, v4d S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
+    if (P == 2) {
+        // both panels' operand tiles are requested before the first product,
+        // so their latencies overlap each other and the first product
+        v2d a0[8], a1[8], b0[8], b1[8];
+        stage_fetch(a0, Ai + (size_t)m0 * T64, ld);
+        stage_fetch(a1, Ai + (size_t)(m0 + 1) * T64, ld);
+        if (i != j) {
+            stage_fetch(b0, Aj + (size_t)m0 * T64, ld);
+            stage_fetch(b1, Aj + (size_t)(m0 + 1) * T64, ld);
+        }
+        stage_put<true>(sA, a0);
+        if (i != j) stage_put<true>(sB, b0);
+        __syncthreads();
+        mma64(sA, i == j ? sA : sB, S);
+        __syncthreads();
+        stage_put<true>(sA, a1);
+        if (i != j) stage_put<true>(sB, b1);
+        __syncthreads();
+        mma64(sA, i == j ? sA : sB, S);
+        return;
+    }
     for (int m = m0; m < m0 + P; ++m) {
         if (m > m0) __syncthreads();
         stage<true>(sA, Ai + (size_t)m * T64, ld);
@@ -356,6 +409,25 @@ __device__ __forceinline__ void panel_update32(const double* A, int ld, int i, i
                                                float* sB, v4f S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
+    if (P == 2) {  // operand tiles of both panels in flight at once, as in panel_update
+        v2d a0[8], a1[8], b0[8], b1[8];
+        stage_fetch(a0, Ai + (size_t)m0 * T64, ld);
+        stage_fetch(a1, Ai + (size_t)(m0 + 1) * T64, ld);
+        if (i != j) {
+            stage_fetch(b0, Aj + (size_t)m0 * T64, ld);
+            stage_fetch(b1, Aj + (size_t)(m0 + 1) * T64, ld);
+        }
+        stage_put_f32t(sA, a0);
+        if (i != j) stage_put_f32t(sB, b0);
+        __syncthreads();
+        mma64_f32(sA, i == j ? sA : sB, S);
+        __syncthreads();
+        stage_put_f32t(sA, a1);
+        if (i != j) stage_put_f32t(sB, b1);
+        __syncthreads();
+        mma64_f32(sA, i == j ? sA : sB, S);
+        return;
+    }
     for (int m = m0; m < m0 + P; ++m) {
         if (m > m0) __syncthreads();
         stage_f32t(sA, Ai + (size_t)m * T64, ld);
@@ -651,7 +723,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     // trailing tile (i, j), j >= c0 + nc; with two columns, tile
     // (c0+2, c0+2) (first in this order) belongs to row c0+2
     int ii, jj;
-    tri_decode(b - 1 - nh - nw + (nc == 2 ? 1 : 0), ii, jj);
+    trailing_decode(b - 1 - nh - nw, T - c0 - nc, nc == 2, ii, jj);
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
     update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);

@@ -128,10 +128,9 @@ __device__ __forceinline__ v4f mfma32(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// s[col * SPF + row] = (float)X[row][col] (op-layout staging of an FP64 tile)
-__device__ __forceinline__ void stage_f32t(float* s, const double* X, int ld) {
-    v2d v[8];
-    stage_fetch(v, X, ld);
+// s[col * SPF + row] = (float)X[row][col] (op-layout staging of an FP64
+// tile fetched with stage_fetch)
+__device__ __forceinline__ void stage_put_f32t(float* s, const v2d v[8]) {
 #pragma unroll
     for (int it = 0; it < 8; ++it) {
         const int e = threadIdx.x + it * 256;
@@ -139,6 +138,12 @@ __device__ __forceinline__ void stage_f32t(float* s, const double* X, int ld) {
         s[col * SPF + row] = (float)v[it].x;
         s[(col + 1) * SPF + row] = (float)v[it].y;
     }
+}
+
+__device__ __forceinline__ void stage_f32t(float* s, const double* X, int ld) {
+    v2d v[8];
+    stage_fetch(v, X, ld);
+    stage_put_f32t(s, v);
 }
 
 __device__ __forceinline__ void mma64_f32(const float* sA, const float* sB, v4f acc[2][2]) {
