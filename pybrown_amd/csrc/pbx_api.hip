@@ -44,7 +44,9 @@ int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const dou
                      double* y, hipStream_t s);
 int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
                          hipStream_t s);
-int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, hipStream_t s);
+// flags: np / 64 ints, zero before the first call; epoch: 1, 2, ... per call
+int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
+                       int epoch, int* info, hipStream_t s);
 int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
                              int cich, double mul, double div, double* C, int ld, hipStream_t s);
 int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
@@ -339,6 +341,10 @@ struct PbxSim {
     int *flag = nullptr, *info = nullptr;
     double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
     double *Wchol = nullptr, *Winv = nullptr;
+    // midpoint: per-tile-row completion flags of the B0^T solve (zeroed at
+    // create, written with a per-call epoch; not part of a snapshot)
+    int* trsv_flags = nullptr;
+    int trsv_epoch = 0;
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
     // noise: chunks of xi_cap attempts (3N draws each) in order; the device
     // ring holds two chunks (chunk q in slot q % 2), the pinned host side
@@ -748,7 +754,8 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         }
         // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi
         PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
-        TRY(pbx_launch_trsv_lt(s->B, s->np, s->ld, s->Wchol, s->xipad, s->w, s->stream));
+        TRY(pbx_launch_trsv_lt(s->B, s->np, s->ld, s->Wchol, s->xipad, s->w, s->trsv_flags, ++s->trsv_epoch,
+                               s->info, s->stream));
         TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
         if (s->F_nonzero)
             TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));
@@ -903,8 +910,12 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         if ((rc = sim_matrix(s, &s->D))) return fail(rc);
         if ((rc = sim_matrix(s, &s->B))) return fail(rc);
         if ((rc = sim_alloc(s, (void**)&s->Wchol, sizeof(double) * pbx_dense_work_doubles(s->np)))) return fail(rc);
-        if (cfg->scheme == PBX_MIDPOINT)
+        if (cfg->scheme == PBX_MIDPOINT) {
             if ((rc = sim_matrix(s, &s->Dmid))) return fail(rc);
+            if ((rc = sim_alloc(s, (void**)&s->trsv_flags, sizeof(int) * (s->np / 64)))) return fail(rc);
+            if (!hip_ok(hipMemsetAsync(s->trsv_flags, 0, sizeof(int) * (s->np / 64), s->stream), "trsv flags"))
+                return fail(PBX_ERR_HIP);
+        }
         if (is_lub(s)) {
             if ((rc = sim_matrix(s, &s->Rff))) return fail(rc);
             if ((rc = sim_matrix(s, &s->X))) return fail(rc);

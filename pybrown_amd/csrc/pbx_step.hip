@@ -170,42 +170,74 @@ __global__ __launch_bounds__(256) void det_drift_kernel(const double* __restrict
     if (lane == 0) y[r] += s;
 }
 
-// Solve L^T y = x for lower-triangular L (order np, multiple of 64) given the
-// inverses W_k of its diagonal tiles: back substitution over tile rows in one
-// 1024-thread workgroup.  y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m).
-__global__ __launch_bounds__(1024) void trsv_lt_kernel(const double* __restrict__ Lm, int np, int ld,
-                                                       const double* __restrict__ W, const double* __restrict__ x,
-                                                       double* y) {
-    __shared__ double part[16][64];
-    __shared__ double rvec[64];
-    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+// Solve L^T y = x for lower-triangular L (order np = 64 T) given the inverses
+// W_k of its diagonal tiles: y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m).
+// One workgroup per tile row, dispatched from the last row up (workgroup b
+// solves row T-1-b), so each waits only on workgroups dispatched before it.
+// Row k accumulates L_mk^T y_m for m = T-1 .. k+1 as each y_m is published
+// (flags[m] = epoch; the tile is requested before the wait), then publishes
+// y_k.  The critical path is one tile product, one W_k product and one
+// publication per tile row.  Waits are bounded: an exhausted wait sets *info
+// to PBX_INFO_DAG_WAIT, and the other workgroups then stop waiting.
+__global__ __launch_bounds__(256) void trsv_lt_rows_kernel(const double* __restrict__ Lm, int np, int ld,
+                                                           const double* __restrict__ W,
+                                                           const double* __restrict__ x, double* y, int* flags,
+                                                           int epoch, int* info) {
+    __shared__ double part[4][64];
+    __shared__ double rv[64];
+    __shared__ int ok;
     const int T = np / 64;
-    for (int k = T - 1; k >= 0; --k) {
-        double s = 0.0;
-        for (int rho = 64 * (k + 1) + g; rho < np; rho += 16) s += Lm[(size_t)rho * ld + 64 * k + c] * y[rho];
-        part[g][c] = s;
-        __syncthreads();
-        if (g == 0) {
-            double t = 0.0;
-            for (int q = 0; q < 16; ++q) t += part[q][c];
-            rvec[c] = x[64 * k + c] - t;
-        }
-        __syncthreads();
-        // y_k[c] = sum_{c'} W_k[c'][c] rvec[c']  (W_k lower: c' >= c)
-        const double* Wk = W + (size_t)k * 64 * 64;
-        double u = 0.0;
-        for (int cp = c + g; cp < 64; cp += 16)
-            if (cp >= c) u += Wk[cp * 64 + c] * rvec[cp];
-        part[g][c] = u;
-        __syncthreads();
-        if (g == 0) {
-            double t = 0.0;
-            for (int q = 0; q < 16; ++q) t += part[q][c];
-            y[64 * k + c] = t;
-        }
-        __threadfence_block();
-        __syncthreads();
+    const int k = T - 1 - (int)blockIdx.x;
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    // this thread's entries of W_k^T: W_k[cp][c] for cp = c + g, c + g + 4, ... (W_k lower)
+    const double* Wk = W + (size_t)k * 64 * 64;
+    double wv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int cp = c + g + 4 * q;
+        wv[q] = cp < 64 ? Wk[cp * 64 + c] : 0.0;
     }
+    double acc = 0.0;
+    for (int m = T - 1; m > k; --m) {
+        double lv[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lv[q] = Lm[(size_t)(64 * m + g + 4 * q) * ld + 64 * k + c];
+        if (threadIdx.x == 0) {
+            int good = 1;
+            long long spins = 0;
+            while (__hip_atomic_load(flags + m, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+                if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1 << 22)) {
+                    atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
+                    good = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            ok = good;
+        }
+        __syncthreads();
+        if (!ok) return;
+        const double* ym = y + 64 * m;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) acc += lv[q] * ym[g + 4 * q];
+    }
+    part[g][c] = acc;
+    __syncthreads();
+    if (g == 0) rv[c] = x[64 * k + c] - ((part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
+    __syncthreads();
+    double u = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int cp = c + g + 4 * q;
+        if (cp < 64) u += wv[q] * rv[cp];
+    }
+    part[g][c] = u;
+    __syncthreads();
+    if (g == 0) y[64 * k + c] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(flags + k, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,8 +616,13 @@ int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, cons
     return PBX_OK;
 }
 
-int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, hipStream_t s) {
-    hipLaunchKernelGGL(trsv_lt_kernel, dim3(1), dim3(1024), 0, s, Lm, np, ld, W, x, y);
+int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
+                       int epoch, int* info, hipStream_t s) {
+    if (np <= 0 || np % 64 || epoch <= 0) {
+        pbx_set_error(PBX_ERR_ARG, "trsv_lt: np=%d epoch=%d", np, epoch);
+        return PBX_ERR_ARG;
+    }
+    hipLaunchKernelGGL(trsv_lt_rows_kernel, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, flags, epoch, info);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -115,6 +115,19 @@ def test_midpoint_c3_like_bidisperse():
     _run_pair(xyz, a, inp, 20, 0.1)
 
 
+def test_midpoint_many_tile_rows():
+    """N = 400 (19 tile rows of 64): the B0^T solve of the midpoint runs as
+    one workgroup per tile row chained through completion flags."""
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(400, 5.0)
+    Lb = box_for(a, 0.05)
+    xyz = rsa_positions(a, Lb, 9, min_gap=0.5)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 2, "hydrodynamics": "rpy_smith",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 1, "ewald_imag": 1, "propagation_scheme": "midpoint",
+           "m_midpoint": 100, "measure_flux": FLUX, "check_overlaps": False}
+    _run_pair(xyz, a, inp, 6, 1.0)
+
+
 @pytest.mark.parametrize("hyd", ["rpy_lub", "rpy_smith_lub"])
 def test_ermak_lub_with_cadence_flags(hyd):
     from pybrown_amd.workloads import rsa_positions, box_for
