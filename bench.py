@@ -69,6 +69,8 @@ def main():
     ap.add_argument("--requests", type=int, default=64, help="c5: requests taken from the head of the stream")
     ap.add_argument("--slots", type=int, default=16, help="c5: concurrent simulations")
     ap.add_argument("--request-steps", type=int, default=200, help="c5: steps per request")
+    ap.add_argument("--graphs", action="store_true",
+                    help="c5: replay the factorisation and SPD-inverse launch sequences from HIP graphs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -187,7 +189,8 @@ def serve_bench(args):
     # hardware queues the HIP runtime spreads the simulations' streams over;
     # read once when the runtime starts, so set before anything touches it
     os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.slots, 4), 16))
-    from pybrown_amd.serving import Server, make_requests_parallel, stream_params, summary
+    import functools
+    from pybrown_amd.serving import Server, make_requests_parallel, run_one, stream_params, summary
     params = stream_params(args.requests)
     t = time.perf_counter()
     reqs = make_requests_parallel(params, steps=args.request_steps, workers=min(16, os.cpu_count() or 1))
@@ -197,19 +200,20 @@ def serve_bench(args):
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     budget = 250e9
+    runner = functools.partial(run_one, graphs=args.graphs)
     small = sorted(reqs, key=lambda r: r.N)[:2]
-    Server(slots=2, hbm_budget=budget).run(small, realtime=False)  # warm-up: code objects, allocator
-    one = Server(slots=1, hbm_budget=budget).run(reqs, realtime=False)
+    Server(slots=2, hbm_budget=budget, runner=runner).run(small, realtime=False)  # warm-up: code objects, allocator
+    one = Server(slots=1, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
     base = summary(one)
     print("c5 one at a time: %s" % base, file=sys.stderr)
     for r in sorted(one, key=lambda r: r.start - r.end)[:8]:
         print("   request %4d  N %5d  %-14s service %.3f s" % (r.rid, r.N, r.mode, r.end - r.start), file=sys.stderr)
-    burst = summary(Server(slots=args.slots, hbm_budget=budget).run(reqs, realtime=False))
+    burst = summary(Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False))
     print("c5 served, all at start: %s" % burst, file=sys.stderr)
     rate = 1.5 * burst["sims_per_s"]
     for r, p in zip(reqs, stream_params(args.requests, rate=rate)):
         r.arrival = p["arrival"]
-    res = Server(slots=args.slots, hbm_budget=budget).run(reqs, realtime=True)
+    res = Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=True)
     served = summary(res)
     print("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served), file=sys.stderr)
     failed = [(r.rid, r.N, r.mode, r.status) for r in res if r.status != "ok"]
@@ -233,7 +237,8 @@ def serve_bench(args):
                                "%d steps each, flux x=0, Poisson arrivals at 1.5x the burst throughput"
                                % (args.requests, args.request_steps),
                    "requests": args.requests, "slots": args.slots, "hbm_budget_bytes": budget,
-                   "arrival_rate_per_s": rate, "gpu_max_hw_queues": os.environ["GPU_MAX_HW_QUEUES"]},
+                   "arrival_rate_per_s": rate, "gpu_max_hw_queues": os.environ["GPU_MAX_HW_QUEUES"],
+                   "graphs": args.graphs},
         "completed": served["completed"],
         "failed": served["failed"],
         "p50_s": served["p50_s"],

@@ -157,6 +157,13 @@ int pbx_sim_set_seed(pbx_sim* sim, unsigned int seed);
 /* Move attempts evaluated per batch in the Ermak scheme with overlap checks
  * (1..32; 0 = adaptive, the default).  Trajectories do not depend on it. */
 int pbx_sim_set_attempt_batch(pbx_sim* sim, int K);
+/* Launch sequences with fixed arguments -- the noise factorisation and the
+ * SPD inverses of the lubrication modes -- captured into HIP graphs on first
+ * use and replayed as one graph launch per sequence afterwards (on = 1), or
+ * launched kernel by kernel (on = 0, the default).  Results are identical.
+ * No reference counterpart: it cuts host launch overhead where many small
+ * simulations share the GPU (the serving loop). */
+int pbx_sim_set_graphs(pbx_sim* sim, int on);
 /* Checkpoint: the device state (positions, flux, D, B[, Rff], diagonal
  * inverses, counters) and the noise state (unconsumed draws; the native
  * generator's state) as one byte buffer; restoring it into a simulation

@@ -62,7 +62,8 @@ UNIT_FACTORS = {"joule": 1.0, "kcal/mol": 10 ** (-21) * 6.9477, "eV": 10 ** (-19
 class Box:
     """Simulation box (pyBrown/box.py:30-109)."""
 
-    def __init__(self, beads, input_data, xi_pool_attempts=64, noise="native", attempt_batch=0, precision="fp64"):
+    def __init__(self, beads, input_data, xi_pool_attempts=64, noise="native", attempt_batch=0, precision="fp64",
+                 graphs=False):
         self.inp = dict(DEFAULTS)
         self.inp.update(input_data)
         inp = self.inp
@@ -81,7 +82,7 @@ class Box:
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
-        self._init_args = (xi_pool_attempts, noise, attempt_batch, precision)
+        self._init_args = (xi_pool_attempts, noise, attempt_batch, precision, bool(graphs))
         self.beads = beads
         self.box_length = inp["box_length"]
         # box.py:621-629
@@ -204,6 +205,8 @@ class Box:
             _lib.check_rc(lib.pbx_sim_set_noise_source(self._sim, self._fill_cb, None), "set_noise_source")
             self.noise = "callback"
         _lib.check_rc(lib.pbx_sim_set_attempt_batch(self._sim, int(attempt_batch)), "set_attempt_batch")
+        # the factorisation and SPD-inverse launch sequences as HIP graphs
+        _lib.check_rc(lib.pbx_sim_set_graphs(self._sim, 1 if graphs else 0), "set_graphs")
         self._set_forces()
         self._synced = self._pos.copy()
         self._counts = np.zeros(max(1, len(self.mobile_labels)), dtype=np.int64)
@@ -378,8 +381,9 @@ class Box:
     def __setstate__(self, st):
         xi, noise, batch = st["init"][:3]
         precision = st["init"][3] if len(st["init"]) > 3 else "fp64"
+        graphs = st["init"][4] if len(st["init"]) > 4 else False
         self.__init__(st["beads"], st["input_data"], xi_pool_attempts=xi, noise=noise, attempt_batch=batch,
-                      precision=precision)
+                      precision=precision, graphs=graphs)
         self.pseudorandom_number_generator = st["rng"]
         self.restore(st["snapshot"])
         if st["net_flux"] is not None:

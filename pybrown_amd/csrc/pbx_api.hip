@@ -341,6 +341,18 @@ struct PbxSim {
     int *flag = nullptr, *info = nullptr;
     double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
     double *Wchol = nullptr, *Winv = nullptr;
+    // launch sequences with fixed arguments (the factorisation and the SPD
+    // inverses) replayed from HIP graphs when `graphs` is on: captured on
+    // first use per (operation, output buffer, Cholesky variant)
+    struct GraphSlot {
+        int op;
+        const void* key;
+        int variant;
+        hipGraphExec_t exec;
+    };
+    int graphs = 0;
+    int n_graphs = 0;
+    GraphSlot graph_slots[8];
     // midpoint: per-tile-row completion flags of the B0^T solve (zeroed at
     // create, written with a per-call epoch; not part of a snapshot)
     int* trsv_flags = nullptr;
@@ -473,6 +485,52 @@ int check_info(PbxSim* s) {
     return info;
 }
 
+// Run `body` (stream work with fixed arguments) on the simulation's stream:
+// eagerly, or with graphs on from a HIP graph captured on first use and
+// launched as one unit afterwards.  The sequences captured here hold kernel
+// launches only (no memset nodes); the table is small, and a full table
+// falls back to eager launches.
+template <class F>
+int run_graphed(PbxSim* s, int op, const void* key, F&& body) {
+    if (!s->graphs) return body();
+    const int variant = pbx_get_potrf_variant();
+    for (int q = 0; q < s->n_graphs; ++q) {
+        const PbxSim::GraphSlot& g = s->graph_slots[q];
+        if (g.op == op && g.key == key && g.variant == variant) {
+            PBX_HIP(hipGraphLaunch(g.exec, s->stream));
+            return PBX_OK;
+        }
+    }
+    if (s->n_graphs == (int)(sizeof(s->graph_slots) / sizeof(s->graph_slots[0]))) return body();
+    PBX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = body();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        pbx_set_error(PBX_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        return PBX_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        return PBX_ERR_HIP;
+    }
+    s->graph_slots[s->n_graphs++] = {op, key, variant, exec};
+    PBX_HIP(hipGraphLaunch(exec, s->stream));
+    return PBX_OK;
+}
+
+void drop_graphs(PbxSim* s) {
+    for (int q = 0; q < s->n_graphs; ++q) (void)hipGraphExecDestroy(s->graph_slots[q].exec);
+    s->n_graphs = 0;
+}
+
+enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3 };
+
 // box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
 int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
     const pbx_config& c = s->cfg;
@@ -483,8 +541,8 @@ int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
     }
     // Mff = M 1e19 / eta ; Rff = inv(Mff)
     TRY(assemble_M(s, RffOut, 1e19 / c.viscosity));
-    TRY(pbx_spd_inverse(RffOut, s->np, s->ld, s->Winv, s->X, s->info, s->stream, 1.0));
-    return PBX_OK;
+    return run_graphed(s, GRAPH_INV_RFF, RffOut,
+                       [&]() { return pbx_spd_inverse(RffOut, s->np, s->ld, s->Winv, s->X, s->info, s->stream, 1.0); });
 }
 
 // box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R)
@@ -493,16 +551,18 @@ int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
     PBX_HIP(hipMemcpyAsync(Dout, RffIn, sizeof(double) * (size_t)s->np * s->ld, hipMemcpyDeviceToDevice, s->stream));
     TRY(pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff, c.cichocki_correction,
                                  c.viscosity, 1e19, Dout, s->ld, s->stream));
-    TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT));
-    TRY(pbx_fill_pad(Dout, s->n, s->np, s->ld, s->stream));
-    return PBX_OK;
+    return run_graphed(s, GRAPH_INV_R, Dout, [&]() {
+        TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT));
+        return pbx_fill_pad(Dout, s->n, s->np, s->ld, s->stream);
+    });
 }
 
 // box.py:546-556 _decompose_D_matrix
 int factor(PbxSim* s) {
-    TRY(pbx_copy_lower(s->D, s->B, s->np, s->ld, s->stream));
-    TRY(pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision));
-    return PBX_OK;
+    return run_graphed(s, GRAPH_FACTOR, s->B, [&]() {
+        TRY(pbx_copy_lower(s->D, s->B, s->np, s->ld, s->stream));
+        return pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision);
+    });
 }
 
 // ---- noise ---------------------------------------------------------------
@@ -967,6 +1027,7 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
         if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
     for (int k = 0; k < 3; ++k)
         if (s->xi_ev[k]) (void)hipEventDestroy(s->xi_ev[k]);
+    drop_graphs(s);
     for (void* p : s->owned) (void)hipFree(p);
     if (s->xi_host) (void)hipHostFree(s->xi_host);
     if (s->flag_host) (void)hipHostFree(s->flag_host);
@@ -1190,6 +1251,20 @@ extern "C" int pbx_sim_set_attempt_batch(pbx_sim* s, int K) {
         return PBX_ERR_ARG;
     }
     s->batch_fixed = K;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_graphs(pbx_sim* s, int on) {
+    pbx_clear_error();
+    if (!s || (on != 0 && on != 1)) {
+        pbx_set_error(PBX_ERR_ARG, "set_graphs: on must be 0 or 1");
+        return PBX_ERR_ARG;
+    }
+    if (!on) {
+        PBX_HIP(hipStreamSynchronize(s->stream));
+        drop_graphs(s);
+    }
+    s->graphs = on;
     return PBX_OK;
 }
 

@@ -181,14 +181,15 @@ class Result:
         return None if self.end is None else self.end - self.arrival
 
 
-def run_one(req):
+def run_one(req, graphs=False):
     """Create the simulation on the device, run all steps, free it.
     Returns (per-step flux (steps, labels), final positions (N, 3), device
-    bytes held)."""
+    bytes held).  graphs: replay the factorisation and SPD-inverse launch
+    sequences from HIP graphs (Box(..., graphs=True))."""
     from pybrown_amd.bead import Bead
     from pybrown_amd.box import Box
     beads = [Bead(req.xyz[i].copy(), float(req.radii[i]), label="XXX") for i in range(req.N)]
-    box = Box(beads, req.inp)
+    box = Box(beads, req.inp, graphs=graphs)
     try:
         held = box.device_bytes
         flux, _ = box.run(req.dt, req.steps)

@@ -297,3 +297,32 @@ def test_restore_rejects_a_different_simulation():
     h = _gpu_box(xyz, a, dict(BASE, hydrodynamics="nohi"))
     with pytest.raises(RuntimeError):
         h.restore(g.snapshot())
+
+
+@pytest.mark.parametrize("case", ["rpy_smith_ermak", "lub_midpoint", "mixed"])
+def test_graph_replays_match_eager(case):
+    """The factorisation and SPD-inverse sequences replayed from HIP graphs
+    (captured on the first step, replayed on every later one with the step's
+    eager launches and synchronisations in between) give the eager
+    trajectory bit for bit."""
+    from pybrown_amd.workloads import rsa_positions, box_for
+    if case == "lub_midpoint":
+        a = np.array([20.0] * 30 + [10.0] * 30)
+        Lb = box_for(a, 0.25)
+        xyz = rsa_positions(a, Lb, 4, min_gap=0.1)
+        inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 1, "hydrodynamics": "rpy_smith_lub",
+               "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "lubrication_cutoff": 0.5,
+               "propagation_scheme": "midpoint", "m_midpoint": 100, "overlap_treshold": 0.1, "measure_flux": FLUX}
+        steps, dt, kw = 12, 0.1, {}
+    else:
+        a = np.full(300, 10.0)
+        Lb = box_for(a, 0.05)
+        xyz = rsa_positions(a, Lb, 3)
+        inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 0, "hydrodynamics": "rpy_smith",
+               "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "measure_flux": FLUX}
+        steps, dt, kw = 15, 1.0, ({"precision": "mixed"} if case == "mixed" else {})
+    eager, d0 = _trajectory(xyz, a, inp, steps, dt, graphs=False, **kw)
+    graph, d1 = _trajectory(xyz, a, inp, steps, dt, graphs=True, **kw)
+    assert d0 == d1
+    for j, ((p0, f0, n0), (p1, f1, n1)) in enumerate(zip(eager, graph)):
+        assert np.array_equal(p0, p1) and f0 == f1 and n0 == n1, "step %d" % j
