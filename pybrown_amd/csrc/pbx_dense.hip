@@ -574,7 +574,16 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     __shared__ double sd[T64];
     const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
     const int nw = T - c0 - nc;
-    const int b = blockIdx.x;
+    // One-workgroup-per-CU build: workgroups are dealt to the XCDs
+    // round-robin, and those that would share XCD 0 with workgroup 0 leave at
+    // once (the others take roles 1, 2, ... in order), so the diagonal
+    // chain's agent-scope releases (which write back its XCD's L2) and
+    // acquires (which invalidate it) meet its own data only.
+    int b = blockIdx.x;
+    if (OCC == 1 && b != 0) {
+        if ((b & 7) == 0) return;
+        b -= b >> 3;
+    }
     auto tile = [&](int i, int j) { return A + (size_t)i * T64 * ld + (size_t)j * T64; };
     int* err = la + LA_ERR;
     double* W0 = work + (size_t)c0 * T64 * T64;
@@ -1023,8 +1032,17 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         const int nw = T - c0 - nc;
         const int ntr = (P > 0 && nw > 0) ? nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0) : 0;
         upd_target += nh;
+        // own-CU build: roles 1 .. W sit at the indices that are not
+        // multiples of 8 (the smallest grid holding W of them; role
+        // b - floor(b / 8) of index b runs over 1 .. W exactly once)
+        int grid = 1 + nh + nw + ntr;
+        if (own_cu) {
+            const int W = nh + nw + ntr;
+            grid = 1;
+            while ((grid - 1) - (grid - 1) / 8 < W) ++grid;
+        }
         if (own_cu)
-            hipLaunchKernelGGL((potrf2_kernel<5, 1, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), dyn_lds, s, A, ld, T,
+            hipLaunchKernelGGL((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
                                c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
         else
             hipLaunchKernelGGL((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
