@@ -24,12 +24,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
-def cpu_baseline(name, steps=3):
+def cpu_baseline(name, steps=3, threads=1):
     """Reference CPU path on the host cores: the reference's clib kernels
     (oracle/_ref, built from its sources; the restatement oracle/liborc.so if
     that build is absent) driven by the box.py step restatement with
-    numpy.linalg.cholesky, one BLAS thread as the reference's inert OpenMP
-    build runs.  Bounded sample: 1 warm-up step + `steps` timed steps."""
+    numpy.linalg.cholesky.  threads = 1: one BLAS thread, as the reference's
+    default build runs (its OpenMP pragma is inert); threads > 1: the OpenMP
+    build of the same sources (the reference's parallel loop over bead rows)
+    and that many BLAS threads.  Bounded sample: 1 warm-up step + `steps`
+    timed steps."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
@@ -37,10 +40,14 @@ def cpu_baseline(name, steps=3):
     from oracle.orc import ref_available
     from pybrown_amd.workloads import workload
 
-    kind = "reference" if ref_available() else "port"
+    omp = threads > 1
+    if omp:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+    kind = "reference" if ref_available(omp) else "port"
+    lib = ("ref_omp" if omp else "ref") if kind == "reference" else "orc"
     xyz, a, inp, dt, _ = workload(name, seed=0)
-    with threadpool_limits(limits=1):
-        box = OracleBox(xyz, a, ["XXX"] * len(a), inp, lib="ref" if kind == "reference" else "orc")
+    with threadpool_limits(limits=threads):
+        box = OracleBox(xyz, a, ["XXX"] * len(a), inp, lib=lib)
         box.propagate(dt)
         times = []
         for _ in range(steps):
@@ -48,10 +55,18 @@ def cpu_baseline(name, steps=3):
             box.propagate(dt)
             times.append(time.perf_counter() - t0)
     per_step = float(np.median(times))
-    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": 1, "kind": kind,
-            "sample": "%d timed steps (after 1 warm-up) of the same N=%d workload; median %.3f s/step; "
-                      "clib kernels built -O2 from the reference sources + numpy.linalg.cholesky, 1 thread"
-                      % (steps, len(a), per_step)}
+    how = ("clib kernels built -O2 from the reference sources + numpy.linalg.cholesky, 1 thread" if not omp else
+           "clib kernels built -O2 -fopenmp from the reference sources (its parallel loop over bead rows) + "
+           "numpy.linalg.cholesky, %d OpenMP and BLAS threads" % threads)
+    return {"value": 1.0 / per_step, "unit": "steps/s", "cores": threads, "kind": kind,
+            "sample": "%d timed steps (after 1 warm-up) of the same N=%d workload; median %.3f s/step; %s"
+                      % (steps, len(a), per_step, how)}
+
+
+def host_threads():
+    """Threads for the multi-core baseline: the host's cores, at most 16 (one
+    GPU's share of the host on the machines this runs on)."""
+    return max(1, min(16, os.cpu_count() or 1))
 
 
 def main():
@@ -152,6 +167,8 @@ def main():
         out["accuracy"] = accuracy
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.workload)
+        # the same reference path on all host cores (at most 16), beside it
+        out["cpu_baseline_multicore"] = cpu_baseline(args.workload, steps=5, threads=host_threads())
     print(json.dumps(out))
 
 

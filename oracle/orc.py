@@ -10,6 +10,9 @@ Two libraries are exposed with the same Python signatures:
   oracle/Makefile into oracle/_ref/libpyBrown_ref.so (present only where the
   reference tree was available at build time).  Its lubrication entry point
   needs LAPACK dgetri_, which this image lacks, so it is never called.
+* ``Oracle(lib='ref_omp')`` -- the same sources built with OpenMP
+  (oracle/_ref/libpyBrown_ref_omp.so), for the multi-core CPU timing; the
+  thread count is the OpenMP runtime's (OMP_NUM_THREADS).
 """
 import ctypes
 import os
@@ -19,6 +22,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORC_PATH = os.path.join(HERE, "liborc.so")
 REF_PATH = os.path.join(HERE, "_ref", "libpyBrown_ref.so")
+REF_OMP_PATH = os.path.join(HERE, "_ref", "libpyBrown_ref_omp.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -40,8 +44,8 @@ def _lazy_open(path):
     return ctypes.CDLL(path, handle=h)
 
 
-def ref_available():
-    return os.path.exists(REF_PATH)
+def ref_available(omp=False):
+    return os.path.exists(REF_OMP_PATH if omp else REF_PATH)
 
 
 class Oracle:
@@ -62,10 +66,10 @@ class Oracle:
                 immobile=p + "pointer_immobile_pbc_matrix",
                 rpy=p + "rpy_m_matrix", smith=p + "rpy_smith_m_matrix",
                 jo2b=p + "jo_2b_r_matrix", lub=p + "jo_r_lubrication_correction_matrix")
-        elif lib == "ref":
-            if not ref_available():
-                raise OSError("oracle/_ref/libpyBrown_ref.so not built")
-            self.lib = _lazy_open(REF_PATH)
+        elif lib in ("ref", "ref_omp"):
+            if not ref_available(lib == "ref_omp"):
+                raise OSError("oracle/_ref/libpyBrown_%s.so not built" % lib)
+            self.lib = _lazy_open(REF_OMP_PATH if lib == "ref_omp" else REF_PATH)
             self._names = dict(
                 pointer="pointer_pbc_matrix", overlaps="check_overlaps",
                 immobile="pointer_immobile_pbc_matrix", rpy="RPY_M_matrix", smith="RPY_Smith_M_matrix",

@@ -74,3 +74,20 @@ def test_live_reference_random_configs(orc):
                               ref.rpy_smith_m_matrix(a, P, L, 1.3, mn, mn + 1))
         for thr in (0.0, 0.1, 0.5):
             assert orc.check_overlaps(xyz, a, L, thr) == ref.check_overlaps(xyz, a, L, thr)
+
+
+@pytest.mark.skipif(not ref_available(omp=True), reason="oracle/_ref OpenMP build absent (reference sources absent)")
+def test_openmp_reference_build_gives_the_same_bits():
+    """The multi-core CPU baseline uses the reference's own parallel loop over
+    bead rows (diff_tensor.c:185); each entry is computed by one thread from
+    the same operands, so the matrix equals the serial build's bit for bit."""
+    rng = np.random.RandomState(5)
+    N, L = 60, 40.0
+    xyz = rng.uniform(-L / 2, L / 2, (N, 3))
+    a = rng.uniform(0.8, 1.6, N)
+    serial, par = Oracle("ref"), Oracle("ref_omp")
+    P = Oracle.packed(serial.pointer_pbc_matrix(xyz, L))
+    for mn in (0, 2):
+        M0 = serial.rpy_smith_m_matrix(a, P, L, np.sqrt(np.pi), mn, mn)
+        M1 = par.rpy_smith_m_matrix(a, P, L, np.sqrt(np.pi), mn, mn)
+        assert np.array_equal(M0, M1)
