@@ -152,15 +152,17 @@ __device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
 // Trailing tile (ii, jj), jj <= ii < m, of the r-th trailing workgroup of a
 // launch (the first tile, (0, 0), skipped when skip0).  The tiles of one
 // launch are independent, so the order only decides locality.  Workgroups
-// are dealt to the eight XCDs round-robin by index, so XCD slot r % 8 takes
-// one contiguous chunk of the tile sequence, and the sequence runs through
-// 16-tile column bands (within a band, rows top-down): an XCD then keeps one
-// band's panel rows in its L2 while the row panels stream past once.
-__device__ __forceinline__ void trailing_decode(int r, int m, bool skip0, int& ii, int& jj) {
+// are dealt to the XCDs round-robin by index, so consecutive trailing
+// workgroups cycle through `ns` XCDs (8, or 7 when XCD 0 is left to the
+// diagonal chain); residue class r % ns takes one contiguous chunk of the
+// tile sequence, and the sequence runs through 16-tile column bands (within
+// a band, rows top-down): an XCD then keeps one band's panel rows in its L2
+// while the row panels stream past once.
+__device__ __forceinline__ void trailing_decode(int r, int m, bool skip0, int ns, int& ii, int& jj) {
     const int off = skip0 ? 1 : 0;
     const int Nt = m * (m + 1) / 2 - off;
-    const int s = r & 7, k = r >> 3;
-    const int base = Nt >> 3, extra = Nt & 7;
+    const int s = r % ns, k = r / ns;
+    const int base = Nt / ns, extra = Nt % ns;
     const int t = s * base + min(s, extra) + k + off;
     int c = 0, start = 0;
     for (;;) {  // ends: t < m (m + 1) / 2, the sum of the band counts
@@ -732,7 +734,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     // trailing tile (i, j), j >= c0 + nc; with two columns, tile
     // (c0+2, c0+2) (first in this order) belongs to row c0+2
     int ii, jj;
-    trailing_decode(b - 1 - nh - nw, T - c0 - nc, nc == 2, ii, jj);
+    trailing_decode(b - 1 - nh - nw, T - c0 - nc, nc == 2, OCC == 1 ? 7 : 8, ii, jj);
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
     update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
