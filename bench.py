@@ -205,7 +205,7 @@ def serve_bench(args):
     trajectory."""
     # hardware queues the HIP runtime spreads the simulations' streams over;
     # read once when the runtime starts, so set before anything touches it
-    os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.slots, 4), 16))
+    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or str(min(max(args.slots, 4), 16))
     import functools
     from pybrown_amd.serving import Server, make_requests_parallel, run_one, stream_params, summary
     params = stream_params(args.requests)
@@ -225,8 +225,12 @@ def serve_bench(args):
     print("c5 one at a time: %s" % base, file=sys.stderr)
     for r in sorted(one, key=lambda r: r.start - r.end)[:8]:
         print("   request %4d  N %5d  %-14s service %.3f s" % (r.rid, r.N, r.mode, r.end - r.start), file=sys.stderr)
-    burst = summary(Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False))
+    burst_res = Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
+    burst = summary(burst_res)
     print("c5 served, all at start: %s" % burst, file=sys.stderr)
+    for r in burst_res:
+        if r.status != "ok":
+            print("c5 burst: request %d (N %d, %s) failed: %s" % (r.rid, r.N, r.mode, r.status), file=sys.stderr)
     rate = 1.5 * burst["sims_per_s"]
     for r, p in zip(reqs, stream_params(args.requests, rate=rate)):
         r.arrival = p["arrival"]

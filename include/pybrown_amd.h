@@ -86,6 +86,13 @@ int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
 int pbx_set_potrf_variant(int variant);
 /* The Cholesky variant in use (1-6). */
 int pbx_get_potrf_variant(void);
+/* Variant 6: when on, the two helper workgroups of each launch leave their
+ * jobs (the panel updates of tiles (c0+1, c0) and (c0+1, c0+1)) to the
+ * diagonal chain, which is the path taken when they are not yet running
+ * (other launches fill the compute units).  The factor is the same to the
+ * bit either way.  For tests; off by default.  No counterpart in the
+ * reference (numpy.linalg.cholesky has no such choice). */
+int pbx_set_potrf_chain_jobs(int on);
 
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
  * FP64 MFMA tile product used by the factorisations, for testing. */

@@ -82,6 +82,7 @@ _sig("pbx_spd_inverse_host", _i, [_dp, _i, _d, _dp])
 _sig("pbx_gemm_nt", _i, [_dp, _dp, _dp, _i, _i, _i])
 _sig("pbx_set_potrf_variant", _i, [_i])
 _sig("pbx_get_potrf_variant", _i, [])
+_sig("pbx_set_potrf_chain_jobs", _i, [_i])
 # simulation
 _sig("pbx_sim_create", _i, [ctypes.POINTER(Config), _i, _dp, _dp, _ip, ctypes.POINTER(ctypes.c_void_p)])
 _sig("pbx_sim_destroy", _i, [ctypes.c_void_p])
@@ -123,6 +124,7 @@ EXPORTED = [
     "pbx_spd_inverse_host", "pbx_gemm_nt",
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
+    "pbx_set_potrf_chain_jobs",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
     "pbx_sim_set_attempt_batch", "pbx_sim_set_graphs", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
     "pbx_sim_restore", "pbx_sim_set_forces", "pbx_sim_get_energy", "pbx_sim_get_force", "pbx_sim_propagate",

@@ -44,7 +44,8 @@ int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const dou
                      double* y, hipStream_t s);
 int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
                          hipStream_t s);
-// flags: np / 64 ints, zero before the first call; epoch: 1, 2, ... per call
+// flags: np / 64 + 1 ints (row flags, then the row ticket counter), zero
+// before the first call; epoch: 1, 2, ... per call
 int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
                        int epoch, int* info, hipStream_t s);
 int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
@@ -353,8 +354,9 @@ struct PbxSim {
     int graphs = 0;
     int n_graphs = 0;
     GraphSlot graph_slots[8];
-    // midpoint: per-tile-row completion flags of the B0^T solve (zeroed at
-    // create, written with a per-call epoch; not part of a snapshot)
+    // midpoint: per-tile-row completion flags of the B0^T solve and its row
+    // ticket counter (zeroed at create, flags written with a per-call epoch;
+    // not part of a snapshot)
     int* trsv_flags = nullptr;
     int trsv_epoch = 0;
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
@@ -972,8 +974,8 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         if ((rc = sim_alloc(s, (void**)&s->Wchol, sizeof(double) * pbx_dense_work_doubles(s->np)))) return fail(rc);
         if (cfg->scheme == PBX_MIDPOINT) {
             if ((rc = sim_matrix(s, &s->Dmid))) return fail(rc);
-            if ((rc = sim_alloc(s, (void**)&s->trsv_flags, sizeof(int) * (s->np / 64)))) return fail(rc);
-            if (!hip_ok(hipMemsetAsync(s->trsv_flags, 0, sizeof(int) * (s->np / 64), s->stream), "trsv flags"))
+            if ((rc = sim_alloc(s, (void**)&s->trsv_flags, sizeof(int) * (s->np / 64 + 1)))) return fail(rc);
+            if (!hip_ok(hipMemsetAsync(s->trsv_flags, 0, sizeof(int) * (s->np / 64 + 1), s->stream), "trsv flags"))
                 return fail(PBX_ERR_HIP);
         }
         if (is_lub(s)) {

@@ -326,9 +326,10 @@ __global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, in
     acc_for_each(out, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] = v; });
 }
 
-// An exhausted look-ahead wait becomes the factorisation status word.
+// An exhausted look-ahead wait becomes the factorisation status word, also
+// over a pivot status that the unfinished factor may have raised after it.
 __global__ void la_finish_kernel(const int* la, int* info) {
-    if (threadIdx.x == 0 && la[LA_ERR] != 0) atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
+    if (threadIdx.x == 0 && la[LA_ERR] != 0) atomicExch(info, PBX_INFO_DAG_WAIT);
 }
 
 // ---------------------------------------------------------------------------
@@ -341,11 +342,13 @@ __global__ void la_finish_kernel(const int* la, int* info) {
 //             (c0+1, c0+1) less L_{c0+1,c0} L_{c0+1,c0}^T, factor, publish
 //             ([LA_FLAG] = c0+2).  L tiles are stored after each publish:
 //             only the inverses are waited for;
-//   1         (nc = 2) tile (c0+1, c0) through the panels, written to a
-//             scratch tile (its place in A receives L_{c0+1,c0} from workgroup
-//             0 while the rows still read the updated value);
-//   2         (nc = 2, P > 0) tile (c0+1, c0+1) through the panels; both
-//             counted in [LA_UPD];
+//   1         (nc = 2) job 0: tile (c0+1, c0) through the panels, written to
+//             a scratch tile (its place in A receives L_{c0+1,c0} from
+//             workgroup 0 while the rows still read the updated value);
+//   2         (nc = 2, P > 0) job 1: tile (c0+1, c0+1) through the panels;
+//             each job is done by whoever claims it first ([LA_JOB + job]):
+//             its own workgroup, or workgroup 0 when it finds the job
+//             unclaimed after its first factor; both counted in [LA_UPD];
 //   row i     (i >= c0+nc) tiles (i, c0) and (i, c0+1): update, then
 //             L_{i,c0} = C W_{c0}^T, its own copy of L_{c0+1,c0},
 //             C_{i,c0+1} -= L_{i,c0} L_{c0+1,c0}^T, L_{i,c0+1} = C W_{c0+1}^T.
@@ -354,13 +357,29 @@ __global__ void la_finish_kernel(const int* la, int* info) {
 //             the next launch's diagonal chain starts without an update;
 //   the rest  trailing tiles (i, j), j >= c0+nc, except (c0+2, c0+2): the
 //             panel update.
-// Workgroup 0 waits only on workgroups 1 and 2, which wait on nothing; row
-// workgroups wait only on workgroups 0, 1 and 2.  Lower indices are
-// dispatched first, so every wait ends (each is also bounded).
-constexpr int LA_UPD = 4;
+// Every wait ends, also when other launches on other streams fill the
+// compute units: workgroup 0 is the first of its launch on its XCD (lower
+// indices are dispatched first there) and waits only for jobs that a
+// running workgroup has claimed; row workgroups wait only for workgroup 0
+// and for those jobs.  (Waiting for workgroups 1 and 2 themselves could
+// stall for good: they may sit undispatched behind other launches' row
+// workgroups, which wait on chains that wait on their own helpers.)  Each
+// wait is also bounded.
+constexpr int LA_UPD = 4, LA_JOB = 5;
 
 __global__ void la_reset_kernel(int* la) {
-    if (threadIdx.x >= LA_FLAG && threadIdx.x <= LA_UPD) la[threadIdx.x] = 0;
+    if (threadIdx.x >= LA_FLAG && threadIdx.x <= LA_JOB + 1) la[threadIdx.x] = 0;
+}
+
+// Claims job j of launch c0 for this workgroup (true for exactly one
+// claimant per launch: the word holds the id of the last launch that
+// claimed it, and ids grow from launch to launch).
+__device__ __forceinline__ bool claim_job(int* la, int j, int c0) {
+    __shared__ int got[2];
+    if (threadIdx.x == 0)
+        got[j] = __hip_atomic_exchange(la + LA_JOB + j, c0 + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != c0 + 1;
+    __syncthreads();
+    return got[j] != 0;
 }
 
 __device__ __forceinline__ void publish_agent(int* flag, int v) {
@@ -371,11 +390,13 @@ __device__ __forceinline__ void publish_agent(int* flag, int v) {
 
 // S += sum_{m = m0 .. m0+P-1} L_im L_jm^T (i == j: one operand).  Other
 // waves may still read sA/sB on return: synchronise before reusing them.
+// LEAN: one panel's operands in flight at a time (fewer registers).
+template <bool LEAN = false>
 __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int j, int m0, int P, double* sA,
                                              double* sB, v4d S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
-    if (P == 2) {
+    if (!LEAN && P == 2) {
         // both panels' operand tiles are requested before the first product,
         // so their latencies overlap each other and the first product
         v2d a0[8], a1[8], b0[8], b1[8];
@@ -407,11 +428,12 @@ __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int
 
 // FP32 bulk: S += sum_m L_im L_jm^T with the operands rounded to FP32 in LDS
 // (the FP64 sA/sB buffers hold the FP32 tiles).
+template <bool LEAN = false>
 __device__ __forceinline__ void panel_update32(const double* A, int ld, int i, int j, int m0, int P, float* sA,
                                                float* sB, v4f S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
-    if (P == 2) {  // operand tiles of both panels in flight at once, as in panel_update
+    if (!LEAN && P == 2) {  // operand tiles of both panels in flight at once, as in panel_update
         v2d a0[8], a1[8], b0[8], b1[8];
         stage_fetch(a0, Ai + (size_t)m0 * T64, ld);
         stage_fetch(a1, Ai + (size_t)(m0 + 1) * T64, ld);
@@ -451,7 +473,7 @@ struct Updated {
     __device__ __forceinline__ int col(int q) const { return acc_col((q >> 2) & 1); }
 };
 
-template <bool MIXED>
+template <bool MIXED, bool LEAN = false>
 __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int j, int m0, int P, double* sA,
                                             double* sB, Updated& u) {
     const double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
@@ -460,7 +482,7 @@ __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int 
         acc32_load(Aij, ld, cv);
         v4f S[2][2];
         acc32_zero(S);
-        panel_update32(A, ld, i, j, m0, P, (float*)sA, (float*)sB, S);
+        panel_update32<LEAN>(A, ld, i, j, m0, P, (float*)sA, (float*)sB, S);
         __syncthreads();
         int q = 0;
 #pragma unroll
@@ -479,7 +501,7 @@ __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int 
     acc_load(Aij, ld, u.v);
     v4d S[2][2];
     acc_zero(S);
-    if (P > 0) panel_update(A, ld, i, j, m0, P, sA, sB, S);
+    if (P > 0) panel_update<LEAN>(A, ld, i, j, m0, P, sA, sB, S);
     int q = 0;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -567,10 +589,27 @@ __device__ __forceinline__ void store_l(const double* sL, double* Akk, int ld) {
     }
 }
 
+// Job j of launch (c0, P): tile (c0+1, c0+j) through the panels, job 0 to
+// the scratch tile, job 1 in place; then counted in [LA_UPD].  sA and sB are
+// free again on return.
+template <bool MIXED, bool LEAN>
+__device__ __forceinline__ void job(double* A, int ld, int c0, int P, int j, double* scr, double* sA, double* sB,
+                                    int* la) {
+    Updated u;
+    update_tile<MIXED, LEAN>(A, ld, c0 + 1, c0 + j, c0 - P, P, sA, sB, u);
+    if (j == 0)
+        store_updated(scr, T64, u, false);
+    else
+        store_updated(A + (size_t)(c0 + 1) * T64 * ld + (size_t)(c0 + 1) * T64, ld, u, true);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(la + LA_UPD, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <int V, int OCC, bool MIXED>
 __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
                                                         int upd_target, double* work, double* scr, int* info,
-                                                        int* la) {
+                                                        int* la, int chain_jobs) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
@@ -599,14 +638,33 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         publish_agent(la + LA_FLAG, c0 + 1);
         store_l(sA, tile(c0, c0), ld);
         if (nc == 1) return;
+        // jobs still unclaimed (their workgroups not yet running) are done
+        // here, one panel at a time (this path is rare; registers are kept
+        // for the chain)
+        __shared__ int pending;
+        if (threadIdx.x == 0)
+            pending = __hip_atomic_load(la + LA_UPD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < upd_target;
+        __syncthreads();
+        bool helped = false;
+        if (pending) {
+            for (int j = 0; j < nh; ++j)
+                if (claim_job(la, j, c0)) {
+                    job<MIXED, true>(A, ld, c0, P, j, scr, sA, sB, la);
+                    helped = true;
+                }
+        }
         if (!wait_ge_agent(la + LA_UPD, upd_target, err)) return;
-        // W^T in sB (op(B)(kk, c) = W[c][kk]); W is lower triangular
-        for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
-            const int r = e >> 6, c = e & 63;
-            if (c > r) {
-                const double t = sB[r * SP + c];
-                sB[r * SP + c] = sB[c * SP + r];
-                sB[c * SP + r] = t;
+        if (helped) {
+            stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+        } else {
+            // W^T in sB (op(B)(kk, c) = W[c][kk]); W is lower triangular
+            for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+                const int r = e >> 6, c = e & 63;
+                if (c > r) {
+                    const double t = sB[r * SP + c];
+                    sB[r * SP + c] = sB[c * SP + r];
+                    sB[c * SP + r] = t;
+                }
             }
         }
         double* Aoff = tile(c0 + 1, c0);
@@ -633,17 +691,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         return;
     }
     if (b <= nh) {
-        // tiles (c0+1, c0) (to the scratch tile) and (c0+1, c0+1)
-        const int i = c0 + 1, j = (b == 1) ? c0 : c0 + 1;
-        Updated u;
-        update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
-        if (b == 1)
-            store_updated(scr, T64, u, false);
-        else
-            store_updated(tile(i, j), ld, u, true);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_fetch_add(la + LA_UPD, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (!chain_jobs && claim_job(la, b - 1, c0)) job<MIXED, false>(A, ld, c0, P, b - 1, scr, sA, sB, la);
         return;
     }
     if (b <= nh + nw) {
@@ -992,6 +1040,11 @@ size_t pbx_dense_work_doubles(int np) { return dense_scratch_offset(np) + (size_
 constexpr int kDefaultPotrf = 6;
 static int g_potrf_variant = kDefaultPotrf;
 static int g_own_cu_tiles = 64;
+static int g_chain_jobs = 0;
+extern "C" int pbx_set_potrf_chain_jobs(int on) {
+    g_chain_jobs = on != 0;
+    return PBX_OK;
+}
 extern "C" int pbx_set_potrf_variant(int v) {
     if (v < 0 || v > 6) return PBX_ERR_ARG;
     g_potrf_variant = v ? v : kDefaultPotrf;
@@ -1045,10 +1098,12 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         }
         if (own_cu)
             hipLaunchKernelGGL((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
-                               c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+                               c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
+                               g_chain_jobs);
         else
             hipLaunchKernelGGL((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
-                               P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la);
+                               P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
+                               g_chain_jobs);
         P = nc;
         c0 += nc;
     }

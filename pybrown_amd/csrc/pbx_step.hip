@@ -172,8 +172,10 @@ __global__ __launch_bounds__(256) void det_drift_kernel(const double* __restrict
 
 // Solve L^T y = x for lower-triangular L (order np = 64 T) given the inverses
 // W_k of its diagonal tiles: y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m).
-// One workgroup per tile row, dispatched from the last row up (workgroup b
-// solves row T-1-b), so each waits only on workgroups dispatched before it.
+// One workgroup per tile row, rows handed out from the last row up in the
+// order the workgroups start (a ticket counter at flags[T], which grows by T
+// per call), so each waits only on rows held by workgroups already running,
+// whatever the dispatch order and whatever else occupies the compute units.
 // Row k accumulates L_mk^T y_m for m = T-1 .. k+1 as each y_m is published
 // (flags[m] = epoch; the tile is requested before the wait), then publishes
 // y_k.  The critical path is one tile product, one W_k product and one
@@ -186,8 +188,13 @@ __global__ __launch_bounds__(256) void trsv_lt_rows_kernel(const double* __restr
     __shared__ double part[4][64];
     __shared__ double rv[64];
     __shared__ int ok;
+    __shared__ int row;
     const int T = np / 64;
-    const int k = T - 1 - (int)blockIdx.x;
+    if (threadIdx.x == 0)
+        row = T - 1 - (int)(__hip_atomic_fetch_add((unsigned*)flags + T, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) %
+                            (unsigned)T);
+    __syncthreads();
+    const int k = row;
     const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
     // this thread's entries of W_k^T: W_k[cp][c] for cp = c + g, c + g + 4, ... (W_k lower)
     const double* Wk = W + (size_t)k * 64 * 64;

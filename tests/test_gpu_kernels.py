@@ -179,6 +179,35 @@ def test_cholesky_vs_numpy(orc, n, potrf_variant):
     assert np.all(np.triu(Lg, 1) == 0)
 
 
+@pytest.mark.parametrize("n,precision", [(300, "fp64"), (1000, "fp64"), (3000, "fp64"), (4500, "fp64"),
+                                         (1000, "mixed"), (4500, "mixed")])
+def test_two_col_chain_jobs_same_bits(n, precision):
+    """Two-column Cholesky with the diagonal chain doing both helper jobs of
+    every launch (the path taken when the helper workgroups are not yet
+    running because other launches fill the compute units): the same factor
+    to the bit as with the helpers doing them.  4500 takes the build with two
+    workgroups per compute unit (more than 64 tile columns)."""
+    from pybrown_amd._lib import lib
+    from pybrown_amd.diffusion import cholesky
+    rng = np.random.RandomState(n + 7)
+    X = rng.normal(size=(n, n))
+    D = X @ X.T / n + np.eye(n)
+    assert lib.pbx_set_potrf_variant(6) == 0
+    try:
+        L_help = cholesky(D, precision)
+        assert lib.pbx_set_potrf_chain_jobs(1) == 0
+        try:
+            L_chain = cholesky(D, precision)
+        finally:
+            lib.pbx_set_potrf_chain_jobs(0)
+    finally:
+        lib.pbx_set_potrf_variant(0)
+    assert np.array_equal(L_help, L_chain)
+    Ln = np.linalg.cholesky(D)
+    tol = 1e-12 if precision == "fp64" else 1e-6
+    assert np.linalg.norm(L_chain - Ln) / np.linalg.norm(Ln) <= tol
+
+
 def test_cholesky_reports_non_spd(potrf_variant):
     from pybrown_amd.diffusion import cholesky
     A = np.eye(130)
