@@ -562,8 +562,7 @@ int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
 // box.py:546-556 _decompose_D_matrix
 int factor(PbxSim* s) {
     return run_graphed(s, GRAPH_FACTOR, s->B, [&]() {
-        TRY(pbx_copy_lower(s->D, s->B, s->np, s->ld, s->stream));
-        return pbx_potrf(s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision);
+        return pbx_potrf_copy(s->D, s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision);
     });
 }
 

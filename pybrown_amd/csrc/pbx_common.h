@@ -169,6 +169,8 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
                     double scale);
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
+int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
+                   int mixed);
 size_t pbx_dense_work_doubles(int np);
 
 // batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()

@@ -232,8 +232,11 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(double* A, int ld, int k, 
 }
 
 // Wait until *flag >= target (thread 0 polls with device-scope acquire),
-// bounded; false if the budget ran out or another wait already failed.
-__device__ bool wait_ge_agent(const int* flag, int target, int* err) {
+// bounded; false if the budget ran out or another wait already failed.  An
+// exhausted budget sets *err (so the other waits stop) and makes the status
+// word PBX_INFO_DAG_WAIT, also over a pivot status that the unfinished
+// factor may raise after it or has raised before it.
+__device__ bool wait_ge_agent(const int* flag, int target, int* err, int* info) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
         int good = 1;
@@ -241,6 +244,7 @@ __device__ bool wait_ge_agent(const int* flag, int target, int* err) {
         while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
             if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1 << 22)) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicExch(info, PBX_INFO_DAG_WAIT);
                 good = 0;
                 break;
             }
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, in
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v, ++q) sA[acc_col(y) * SP + acc_row(x, v)] = cv[q] - acc[x][y][v];
-    if (!wait_ge_agent(la + LA_FLAG, k + 1, la + LA_ERR)) return;
+    if (!wait_ge_agent(la + LA_FLAG, k + 1, la + LA_ERR, info)) return;
     stage<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
     __syncthreads();
     v4d out[2][2];
@@ -326,11 +330,6 @@ __global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, in
     acc_for_each(out, [&](int r, int c, double v) { Aij[(size_t)r * ld + c] = v; });
 }
 
-// An exhausted look-ahead wait becomes the factorisation status word, also
-// over a pivot status that the unfinished factor may have raised after it.
-__global__ void la_finish_kernel(const int* la, int* info) {
-    if (threadIdx.x == 0 && la[LA_ERR] != 0) atomicExch(info, PBX_INFO_DAG_WAIT);
-}
 
 // ---------------------------------------------------------------------------
 // Variant 6: two tile columns per launch.  Launch (c0, P, nc) applies panels
@@ -367,9 +366,11 @@ __global__ void la_finish_kernel(const int* la, int* info) {
 // wait is also bounded.
 constexpr int LA_UPD = 4, LA_JOB = 5;
 
-__global__ void la_reset_kernel(int* la) {
+__device__ __forceinline__ void la_reset(int* la) {
     if (threadIdx.x >= LA_FLAG && threadIdx.x <= LA_JOB + 1) la[threadIdx.x] = 0;
 }
+
+__global__ void la_reset_kernel(int* la) { la_reset(la); }
 
 // Claims job j of launch c0 for this workgroup (true for exactly one
 // claimant per launch: the word holds the id of the last launch that
@@ -638,22 +639,27 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         publish_agent(la + LA_FLAG, c0 + 1);
         store_l(sA, tile(c0, c0), ld);
         if (nc == 1) return;
-        // jobs still unclaimed (their workgroups not yet running) are done
-        // here, one panel at a time (this path is rare; registers are kept
-        // for the chain)
+        // Usually the jobs are done by now (one acquire load sees it);
+        // otherwise the unclaimed ones (their workgroups not yet running) are
+        // done here, one panel at a time (a rare path; registers are kept for
+        // the chain), and the claimed ones waited for.
         __shared__ int pending;
-        if (threadIdx.x == 0)
-            pending = __hip_atomic_load(la + LA_UPD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < upd_target;
+        if (threadIdx.x == 0) {
+            pending = __hip_atomic_load(la + LA_UPD, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < upd_target;
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
         __syncthreads();
         bool helped = false;
-        if (pending) {
+        // (marked unlikely: the compiler then keeps this code out of the
+        // chain's straight-line path, which is measurably faster)
+        if (__builtin_expect(pending != 0, 0)) {
             for (int j = 0; j < nh; ++j)
                 if (claim_job(la, j, c0)) {
                     job<MIXED, true>(A, ld, c0, P, j, scr, sA, sB, la);
                     helped = true;
                 }
+            if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         }
-        if (!wait_ge_agent(la + LA_UPD, upd_target, err)) return;
         if (helped) {
             stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         } else {
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             update_tile<MIXED>(A, ld, i, c0, c0 - P, P, sA, sB, u);
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
-            if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
+            if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
             stage<true>(sB, W0, T64);
             __syncthreads();
             v4d out[2][2];
@@ -736,14 +742,14 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
         }
-        if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err)) return;
+        if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
         stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         __syncthreads();
         acc_zero(L0);
         mma64(sA, sB, L0);
         acc_for_each(L0, [&](int r, int c, double v) { A0[(size_t)r * ld + c] = v; });
         // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
-        if (!wait_ge_agent(la + LA_UPD, upd_target, err)) return;
+        if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         stage<true>(sA, scr, T64);
         __syncthreads();
         v4d Lo[2][2];
@@ -764,7 +770,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             __syncthreads();
             put_diff_t(sA, c1, S);
         }
-        if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err)) return;
+        if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err, info)) return;
         stage<true>(sB, W1, T64);
         __syncthreads();
         v4d L1[2][2];
@@ -962,7 +968,9 @@ __global__ void fill_pad_kernel(double* A, int n, int np, int ld) {
 }
 
 // dst = lower tiles of src (diagonal tiles copied whole)
-__global__ void copy_lower_kernel(const double* src, double* dst, int np, int ld) {
+// (la: look-ahead words of the factorisation that follows, reset here)
+__global__ void copy_lower_kernel(const double* src, double* dst, int np, int ld, int* la) {
+    if (la && blockIdx.x == 0) la_reset(la);
     int ti, tj;
     tri_decode(blockIdx.x, ti, tj);
     const double* s = src + (size_t)ti * T64 * ld + (size_t)tj * T64;
@@ -1062,14 +1070,17 @@ int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s) {
 
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s) {
     int T = np / T64;
-    hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, src, dst, np, ld);
+    hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, src, dst, np, ld, nullptr);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 // Variant 6 launch sequence: launch (c0, P, nc) for c0 = 0, 2, 4, ...
+// la_ready: the look-ahead words are already zero (reset by the kernel
+// before, on the same stream).
 template <bool MIXED>
-static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s) {
+static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s,
+                                 bool la_ready) {
     const int T = np / T64;
     int* la = (int*)(work + (size_t)np * T64);
     // Up to 64 tile columns the trailing updates of a launch finish well
@@ -1079,7 +1090,7 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // workgroups per CU.
     const bool own_cu = T <= g_own_cu_tiles;
     const size_t dyn_lds = own_cu ? 24576 : 0;
-    hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
+    if (!la_ready) hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
     for (int c0 = 0, P = 0; c0 < T;) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
@@ -1107,14 +1118,14 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         P = nc;
         c0 += nc;
     }
-    hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
 }
 
-static void potrf_two_col(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, bool mixed) {
+static void potrf_two_col(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, bool mixed,
+                          bool la_ready = false) {
     if (mixed)
-        potrf_two_col_launch<true>(A, np, ld, work, info_dev, s);
+        potrf_two_col_launch<true>(A, np, ld, work, info_dev, s, la_ready);
     else
-        potrf_two_col_launch<false>(A, np, ld, work, info_dev, s);
+        potrf_two_col_launch<false>(A, np, ld, work, info_dev, s, la_ready);
 }
 
 // In-place lower Cholesky of the np x np matrix A; inverses of the diagonal
@@ -1151,7 +1162,6 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
             int cnt = (T - 1 - k) * (T - k) / 2;
             hipLaunchKernelGGL(syrk_trsm_kernel<5>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev, la);
         }
-        hipLaunchKernelGGL(la_finish_kernel, dim3(1), dim3(64), 0, s, la, info_dev);
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
@@ -1168,6 +1178,25 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         else
             hipLaunchKernelGGL(syrk_kernel<1>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
     }
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// B = the lower part of D, then B = its Cholesky factor as pbx_potrf.  With
+// the two-column launches the copy also resets their look-ahead words, one
+// launch fewer per factorisation.
+int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
+                   int mixed) {
+    if (np % T64 != 0 || np <= 0) {
+        pbx_set_error(PBX_ERR_ARG, "potrf: order %d is not a positive multiple of 64", np);
+        return PBX_ERR_ARG;
+    }
+    const int T = np / T64;
+    const bool two_col = g_potrf_variant == 6 || mixed;
+    int* la = two_col ? (int*)(work + (size_t)np * T64) : nullptr;
+    hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, np, ld, la);
+    if (!two_col) return pbx_potrf(B, np, ld, work, info_dev, s, mixed);
+    potrf_two_col(B, np, ld, work, info_dev, s, mixed != 0, true);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
