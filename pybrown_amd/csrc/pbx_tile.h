@@ -438,27 +438,40 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
 #pragma unroll
                 for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
             }
+            {
+                // lane c0+q stores its L_jj = v[q], lane q stores yq[q]: one
+                // store each, operands picked by selects (no per-q branches)
+                double dsel = v[0], ysel = yq[0];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                if (lane == c0 + q) sd[c0 + q] = v[q];
-                if (lane == q) sy[c0 + q] = yq[q];
+                for (int q = 1; q < 8; ++q) {
+                    dsel = (lane - c0 == q) ? v[q] : dsel;
+                    ysel = ((lane & 7) == q) ? yq[q] : ysel;
+                }
+                if (lane >= c0 && lane < c0 + 8) sd[lane] = dsel;
+                if (lane < 8) sy[c0 + lane] = ysel;
             }
             if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
             if (lane == 0) lds_release_add(&fl[0], 1);
             if (p == 7) break;
             // panel p+1's columns (updated through p-1 by waves 2 and 3) less
             // panel p's contribution
-            if (p > 0) lds_wait_ge(&fl[1], 2 * p, info);
             const int n0 = c0 + 8;
+            // panel p's L at rows n0..n0+7 (this wave's own stores above),
+            // all requested before the wait so their latency overlaps it
+            // (read as needed, each LDS round trip was exposed in turn)
+            double lr[64];
+#pragma unroll
+            for (int i = 0; i < 64; ++i) lr[i] = sL[(n0 + (i >> 3)) * SP + c0 + (i & 7)];
+            __builtin_amdgcn_sched_barrier(0);
+            if (p > 0) lds_wait_ge(&fl[1], 2 * p, info);
             double nv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) nv[q] = (lane >= n0) ? sL[lane * SP + n0 + q] : 0.0;
 #pragma unroll
             for (int q2 = 0; q2 < 8; ++q2) {
-                const double* Lr = sL + (n0 + q2) * SP + c0;
                 double t = nv[q2];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) t = fma(-v[q], Lr[q], t);
+                for (int q = 0; q < 8; ++q) t = fma(-v[q], lr[q2 * 8 + q], t);
                 nv[q2] = t;
             }
             if (lane >= n0 && lane < n0 + 8) {
