@@ -362,10 +362,16 @@ struct PbxSim {
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
     // noise: chunks of xi_cap attempts (3N draws each) in order; the device
     // ring holds two chunks (chunk q in slot q % 2), the pinned host side
-    // three (chunk q in buffer q % 3, reusable once xi_ev[q % 3] completes)
+    // three (chunk q in buffer q % 3, reusable once xi_ev[q % 3] completes).
+    // The uploads run on their own stream `up`, ordered against `stream` by
+    // events both ways, so the producer thread's waits on xi_ev never touch
+    // `stream` (which may be capturing a graph at the time: waiting on an
+    // event of a capturing stream invalidates the capture).
     double* xi_dev = nullptr;
     double* xi_host = nullptr;
     hipEvent_t xi_ev[3] = {nullptr, nullptr, nullptr};
+    hipStream_t up = nullptr;
+    hipEvent_t rd_ev = nullptr;  // recorded on `stream` before each upload
     long long xi_cap = 0;
     long long xi_used = 0;   // attempts consumed from the current source
     long long xi_taken = 0;  // chunks whose upload has been queued
@@ -605,6 +611,7 @@ void noise_stop(PbxSim* s) {
 int noise_reset(PbxSim* s) {
     noise_stop(s);
     PBX_HIP(hipStreamSynchronize(s->stream));
+    if (s->up) PBX_HIP(hipStreamSynchronize(s->up));
     s->xi_used = s->xi_taken = s->xi_produced = 0;
     s->native = 0;
     s->xi_fill = nullptr;
@@ -628,9 +635,14 @@ int noise_take(PbxSim* s, long long q) {
             return PBX_ERR_STATE;
         }
     }
+    // after the stream's readers of the slot's previous chunk, before its
+    // next readers
+    PBX_HIP(hipEventRecord(s->rd_ev, s->stream));
+    PBX_HIP(hipStreamWaitEvent(s->up, s->rd_ev, 0));
     PBX_HIP(hipMemcpyAsync(s->xi_dev + (q % 2) * chunk_doubles(s), hb, sizeof(double) * chunk_doubles(s),
-                           hipMemcpyHostToDevice, s->stream));
-    PBX_HIP(hipEventRecord(s->xi_ev[q % 3], s->stream));
+                           hipMemcpyHostToDevice, s->up));
+    PBX_HIP(hipEventRecord(s->xi_ev[q % 3], s->up));
+    PBX_HIP(hipStreamWaitEvent(s->stream, s->xi_ev[q % 3], 0));
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->xi_taken = q + 1;
@@ -912,6 +924,11 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             pbx_set_error(PBX_ERR_HIP, "hipEventCreate failed");
             return fail(PBX_ERR_HIP);
         }
+    if (hipEventCreateWithFlags(&s->rd_ev, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "upload stream or event creation failed");
+        return fail(PBX_ERR_HIP);
+    }
     if (hipHostMalloc((void**)&s->xi_host, sizeof(double) * 3 * s->xi_cap * n3, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->flag_host, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void**)&s->pos_host, sizeof(double) * n3, hipHostMallocDefault) != hipSuccess ||
@@ -1024,6 +1041,7 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     if (!s) return PBX_OK;
     noise_stop(s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->up) (void)hipStreamSynchronize(s->up);
     for (int k = 0; k < 4; ++k)
         if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
     for (int k = 0; k < 3; ++k)
@@ -1035,6 +1053,8 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     if (s->pos_host) (void)hipHostFree(s->pos_host);
     if (s->flux_host) (void)hipHostFree(s->flux_host);
     if (s->E_host) (void)hipHostFree(s->E_host);
+    if (s->rd_ev) (void)hipEventDestroy(s->rd_ev);
+    if (s->up) (void)hipStreamDestroy(s->up);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return PBX_OK;
