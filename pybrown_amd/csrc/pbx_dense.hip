@@ -579,6 +579,21 @@ __device__ __forceinline__ void syrk_store(double* X, int ld, const double* sA) 
     store_diff(X, ld, cv, S, true);
 }
 
+// Device-scope (write-through) store: the diagonal chain's stores then do
+// not sit dirty in its XCD's L2 for its next release to write back.
+__device__ __forceinline__ void st_dev(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_w_dev(const double* sW, double* Wk) {
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) st_dev(Wk + e, sW[(e >> 6) * SP + (e & 63)]);
+}
+__device__ __forceinline__ void store_l_dev(const double* sL, double* Akk, int ld) {
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+        const int r = e >> 6, c = e & 63;
+        if (c <= r) st_dev(Akk + (size_t)r * ld + c, sL[r * SP + c]);
+    }
+}
+
 __device__ __forceinline__ void store_w(const double* sW, double* Wk) {
     for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) Wk[e] = sW[(e >> 6) * SP + (e & 63)];
 }
@@ -635,9 +650,9 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         stage<false>(sA, tile(c0, c0), ld);
         __syncthreads();
         factor_tile<V>(sA, sB, sd, c0 * T64, info);
-        store_w(sB, W0);
+        store_w_dev(sB, W0);
         publish_agent(la + LA_FLAG, c0 + 1);
-        store_l(sA, tile(c0, c0), ld);
+        store_l_dev(sA, tile(c0, c0), ld);
         if (nc == 1) return;
         // Usually the jobs are done by now (one acquire load sees it);
         // otherwise the unclaimed ones (their workgroups not yet running) are
@@ -681,7 +696,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         v4d Lo[2][2];
         acc_zero(Lo);
         mma64(sA, sB, Lo);
-        acc_for_each(Lo, [&](int r, int c, double v) { Aoff[(size_t)r * ld + c] = v; });
+        acc_for_each(Lo, [&](int r, int c, double v) { st_dev(Aoff + (size_t)r * ld + c, v); });
         __syncthreads();
         put_t(sA, Lo);
         __syncthreads();
@@ -691,9 +706,9 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         put_diff(sB, cv, S);
         __syncthreads();
         factor_tile<V>(sB, sA, sd, (c0 + 1) * T64, info);
-        store_w(sA, W1);
+        store_w_dev(sA, W1);
         publish_agent(la + LA_FLAG, c0 + 2);
-        store_l(sB, tile(c0 + 1, c0 + 1), ld);
+        store_l_dev(sB, tile(c0 + 1, c0 + 1), ld);
         return;
     }
     if (b <= nh) {
