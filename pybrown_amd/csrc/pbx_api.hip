@@ -1339,9 +1339,16 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     else
         TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
     TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
-    if (s->timing && cf.hydrodynamics != PBX_NOHI) {
-        PBX_HIP(hipEventRecord(s->ev[3], s->stream));
-        PBX_HIP(hipEventSynchronize(s->ev[3]));
+    const bool timed = s->timing && cf.hydrodynamics != PBX_NOHI;
+    if (timed) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
+    PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    if (s->forces_on) PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    if (cf.measure_flux) {
+        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
+        PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
+    }
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    if (timed) {  // the events are complete: read them after the one synchronisation
         float t01 = 0, t12 = 0, t23 = 0;
         PBX_HIP(hipEventElapsedTime(&t01, s->ev[0], s->ev[1]));
         PBX_HIP(hipEventElapsedTime(&t12, s->ev[1], s->ev[2]));
@@ -1351,13 +1358,6 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         s->stage_ms[2] += t23;
         s->timed_steps += 1;
     }
-    PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
-    if (s->forces_on) PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    if (cf.measure_flux) {
-        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
-        PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
-    }
-    PBX_HIP(hipStreamSynchronize(s->stream));
     return take_info(s);
 }
 
