@@ -171,6 +171,18 @@ int pbx_sim_set_attempt_batch(pbx_sim* sim, int K);
  * No reference counterpart: it cuts host launch overhead where many small
  * simulations share the GPU (the serving loop). */
 int pbx_sim_set_graphs(pbx_sim* sim, int on);
+
+/* Look-ahead assembly (on by default).  In the RPY and Smith-Ewald modes
+ * (not with lubrication, not with HIP graphs on), a step that built D ends
+ * by assembling D for its final positions into a spare matrix, after its
+ * result copies are queued and while the host reads them; the host waits
+ * for the results only.  The next step, when it builds D and the positions
+ * were not replaced (pbx_sim_set_positions, pbx_sim_restore) in between,
+ * takes that matrix instead of assembling.  Results are the same to the bit
+ * either way; pbx_sim_get_matrix(D) always returns the D the last step used.
+ * No counterpart in the reference (box.py:186-198 assembles at the start of
+ * each step); for callers that step one simulation at a time. */
+int pbx_sim_set_lookahead(pbx_sim* sim, int on);
 /* Checkpoint: the device state (positions, flux, D, B[, Rff], diagonal
  * inverses, counters) and the noise state (unconsumed draws; the native
  * generator's state) as one byte buffer; restoring it into a simulation

@@ -90,6 +90,7 @@ _sig("pbx_sim_set_noise_source", _i, [ctypes.c_void_p, XI_FILL, ctypes.c_void_p]
 _sig("pbx_sim_set_seed", _i, [ctypes.c_void_p, ctypes.c_uint])
 _sig("pbx_sim_set_attempt_batch", _i, [ctypes.c_void_p, _i])
 _sig("pbx_sim_set_graphs", _i, [ctypes.c_void_p, _i])
+_sig("pbx_sim_set_lookahead", _i, [ctypes.c_void_p, _i])
 _sig("pbx_sim_snapshot_size", ctypes.c_longlong, [ctypes.c_void_p])
 _sig("pbx_sim_snapshot", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
 _sig("pbx_sim_restore", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
@@ -126,7 +127,7 @@ EXPORTED = [
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
-    "pbx_sim_set_attempt_batch", "pbx_sim_set_graphs", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
+    "pbx_sim_set_attempt_batch", "pbx_sim_set_graphs", "pbx_sim_set_lookahead", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
     "pbx_sim_restore", "pbx_sim_set_forces", "pbx_sim_get_energy", "pbx_sim_get_force", "pbx_sim_propagate",
     "pbx_sim_propagate_out",
     "pbx_sim_run",

@@ -415,7 +415,21 @@ struct PbxSim {
     double* E_host = nullptr;        // pinned: the last step's energy
     // optional stage timing with HIP events on the simulation stream
     int timing = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // ev[0..3]: stage bounds of the step; ev[4..7]: two pairs around the
+    // look-ahead assembly (one pair read while the other is recorded)
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int ahead_pending = -1;  // pair recorded at the end of the last step, not read yet
+    // Look-ahead (non-lubrication matrix modes): a step that built D ends by
+    // assembling D for its final positions into Dspec, after the result
+    // copies, while the host reads the results and comes back; the next
+    // step, if it builds D and the positions were not replaced since, swaps
+    // Dspec in instead of assembling.  The host waits on done_ev (recorded
+    // before that assembly), not on the whole stream.
+    double* Dspec = nullptr;
+    int spec_ok = 0;
+    int lookahead = 1;
+    int next_builds = -1;  // whether the next step builds D: 1, 0, or -1 unknown (then assumed)
+    hipEvent_t done_ev = nullptr;
     double stage_ms[3] = {0, 0, 0};  // assembly (+lubrication), factorisation, propagation
     long long timed_steps = 0;
 };
@@ -925,6 +939,7 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             return fail(PBX_ERR_HIP);
         }
     if (hipEventCreateWithFlags(&s->rd_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->done_ev, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess) {
         pbx_set_error(PBX_ERR_HIP, "upload stream or event creation failed");
         return fail(PBX_ERR_HIP);
@@ -987,6 +1002,7 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     } else {
         if ((rc = sim_matrix(s, &s->D))) return fail(rc);
         if ((rc = sim_matrix(s, &s->B))) return fail(rc);
+        if (!is_lub(s) && (rc = sim_matrix(s, &s->Dspec))) return fail(rc);
         if ((rc = sim_alloc(s, (void**)&s->Wchol, sizeof(double) * pbx_dense_work_doubles(s->np)))) return fail(rc);
         if (cfg->scheme == PBX_MIDPOINT) {
             if ((rc = sim_matrix(s, &s->Dmid))) return fail(rc);
@@ -1042,8 +1058,9 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     noise_stop(s);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     if (s->up) (void)hipStreamSynchronize(s->up);
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < 8; ++k)
         if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    if (s->done_ev) (void)hipEventDestroy(s->done_ev);
     for (int k = 0; k < 3; ++k)
         if (s->xi_ev[k]) (void)hipEventDestroy(s->xi_ev[k]);
     drop_graphs(s);
@@ -1190,6 +1207,7 @@ extern "C" int pbx_sim_snapshot(pbx_sim* s, void* buf, long long bytes) {
 
 extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
     pbx_clear_error();
+    s->spec_ok = 0;
     SnapHeader h;
     if (!buf || bytes < (long long)sizeof(h)) {
         pbx_set_error(PBX_ERR_ARG, "snapshot too short");
@@ -1281,6 +1299,7 @@ extern "C" int pbx_sim_set_graphs(pbx_sim* s, int on) {
         pbx_set_error(PBX_ERR_ARG, "set_graphs: on must be 0 or 1");
         return PBX_ERR_ARG;
     }
+    s->spec_ok = 0;  // graphs hold the matrix pointers the look-ahead swaps
     if (!on) {
         PBX_HIP(hipStreamSynchronize(s->stream));
         drop_graphs(s);
@@ -1303,9 +1322,14 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
     }
     if (s->timing) PBX_HIP(hipEventRecord(s->ev[0], s->stream));
+    const int spec = s->spec_ok;
+    s->spec_ok = 0;
     if (cf.hydrodynamics != PBX_NOHI) {
         if (build_Dff_flag) {
-            TRY(build_Dff(s, s->D, s->Rff));
+            if (spec)
+                std::swap(s->D, s->Dspec);  // assembled for these positions at the end of the last step
+            else
+                TRY(build_Dff(s, s->D, s->Rff));
             if (is_lub(s)) s->have_Rff = 1; else s->have_D = 1;
         }
         if (is_lub(s) && build_Dnf_flag) {
@@ -1347,7 +1371,18 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         int nl = cf.n_labels > 0 ? cf.n_labels : 1;
         PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
     }
-    PBX_HIP(hipStreamSynchronize(s->stream));
+    const bool ahead = s->Dspec && s->lookahead && !s->graphs && build_Dff_flag && s->next_builds != 0 &&
+                       cf.hydrodynamics != PBX_NOHI;
+    const int prev = s->ahead_pending, q = prev == 0 ? 1 : 0;
+    if (ahead) {
+        PBX_HIP(hipEventRecord(s->done_ev, s->stream));
+        if (s->timing) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
+        TRY(assemble_M(s, s->Dspec, s->kBT * 1e19 / cf.viscosity));  // as build_Dff
+        if (s->timing) PBX_HIP(hipEventRecord(s->ev[5 + 2 * q], s->stream));
+        PBX_HIP(hipEventSynchronize(s->done_ev));
+    } else {
+        PBX_HIP(hipStreamSynchronize(s->stream));
+    }
     if (timed) {  // the events are complete: read them after the one synchronisation
         float t01 = 0, t12 = 0, t23 = 0;
         PBX_HIP(hipEventElapsedTime(&t01, s->ev[0], s->ev[1]));
@@ -1358,7 +1393,15 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         s->stage_ms[2] += t23;
         s->timed_steps += 1;
     }
-    return take_info(s);
+    if (s->timing && prev >= 0) {  // the last step's look-ahead assembly preceded this step's kernels
+        float ta = 0;
+        PBX_HIP(hipEventElapsedTime(&ta, s->ev[4 + 2 * prev], s->ev[5 + 2 * prev]));
+        s->stage_ms[0] += ta;
+    }
+    s->ahead_pending = (ahead && s->timing) ? q : -1;
+    const int rc = take_info(s);
+    if (rc == PBX_OK && ahead) s->spec_ok = 1;
+    return rc;
 }
 
 // Box.propagate plus the read-back BD.py needs after every step, in the
@@ -1400,7 +1443,10 @@ extern "C" int pbx_sim_run(pbx_sim* s, double dt, long long j0, long long steps,
                 memcpy(frame, s->pos_host, sizeof(double) * s->n);  // positions after step j-1
             frame += s->n;
         }
-        TRY(pbx_sim_propagate(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0));
+        s->next_builds = (t + 1 < steps && (j + 1) % diff_freq == 0) ? 1 : 0;
+        const int rc = pbx_sim_propagate(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0);
+        s->next_builds = -1;
+        if (rc) return rc;
         if (flux_out) {
             if (s->cfg.measure_flux)
                 memcpy(flux_out + t * nl, s->flux_host, sizeof(long long) * nl);
@@ -1552,8 +1598,9 @@ extern "C" long long pbx_sim_device_bytes(pbx_sim* s) { return s ? (long long)s-
 // [2] propagation (noise products, overlap checks, flux, wrap).
 extern "C" int pbx_sim_set_timing(pbx_sim* s, int on) {
     if (on && !s->ev[0])
-        for (int k = 0; k < 4; ++k) PBX_HIP(hipEventCreate(&s->ev[k]));
+        for (int k = 0; k < 8; ++k) PBX_HIP(hipEventCreate(&s->ev[k]));
     s->timing = on;
+    s->ahead_pending = -1;
     s->stage_ms[0] = s->stage_ms[1] = s->stage_ms[2] = 0.0;
     s->timed_steps = 0;
     return PBX_OK;
@@ -1571,6 +1618,7 @@ extern "C" int pbx_sim_get_positions(pbx_sim* s, double* xyz) {
 }
 
 extern "C" int pbx_sim_set_positions(pbx_sim* s, const double* xyz) {
+    s->spec_ok = 0;
     PBX_HIP(hipMemcpyAsync(s->pos, xyz, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
     PBX_HIP(hipStreamSynchronize(s->stream));
     return PBX_OK;
@@ -1621,5 +1669,13 @@ extern "C" int pbx_sim_last_attempts(pbx_sim* s) { return s->last_attempts; }
 
 extern "C" int pbx_sim_sync(pbx_sim* s) {
     PBX_HIP(hipStreamSynchronize(s->stream));
+    return PBX_OK;
+}
+
+// Look-ahead assembly of D after each step that built it (on by default;
+// non-lubrication matrix modes, HIP graphs off).
+extern "C" int pbx_sim_set_lookahead(pbx_sim* s, int on) {
+    s->lookahead = on != 0;
+    s->spec_ok = 0;
     return PBX_OK;
 }

@@ -135,8 +135,9 @@ def make_requests_parallel(params, steps=200, dt=1.0, workers=8):
 
 def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attempts=64):
     """Upper bound of the device memory one simulation holds: the dense
-    n x n matrices (D and B; with lubrication also the far-field resistance
-    and an inverse scratch; midpoint one more each), the factorisation work
+    n x n matrices (D and B; without lubrication the look-ahead copy of D;
+    with lubrication the far-field resistance and an inverse scratch;
+    midpoint one more each), the factorisation work
     areas (diagonal inverses and state), the noise ring (two chunks of
     xi_pool_attempts x 3N draws) and O(N) vectors."""
     n = 3 * N
@@ -147,7 +148,7 @@ def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attem
     if mode == "nohi":
         mats, works = 0, 0
     else:
-        mats = 2 + (1 if mid else 0) + ((2 + (1 if mid else 0)) if lub else 0)
+        mats = 2 + (1 if mid else 0) + ((2 + (1 if mid else 0)) if lub else 1)
         works = 2 if lub else 1
     work = 8 * (np_ * 64 + (16 + T * T + T) // 2 + 2 + 64 * 64)
     ring = 8 * 2 * xi_pool_attempts * n

@@ -326,3 +326,44 @@ def test_graph_replays_match_eager(case):
     assert d0 == d1
     for j, ((p0, f0, n0), (p1, f1, n1)) in enumerate(zip(eager, graph)):
         assert np.array_equal(p0, p1) and f0 == f1 and n0 == n1, "step %d" % j
+
+
+@pytest.mark.parametrize("hydro", ["rpy", "rpy_smith"])
+def test_lookahead_assembly_matches_plain_steps(hydro):
+    """Steps with the look-ahead assembly of D (on by default: a step that
+    built D assembles the next one's after its result copies) against the
+    same steps with it off: positions, flux and the D of each step to the
+    bit, over D built every step, then every second step, a bead moved from
+    the host between steps, and the driver loop with a build cadence."""
+    from pybrown_amd._lib import lib
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(200, 10.0)
+    Lb = box_for(a, 0.05)
+    xyz = rsa_positions(a, Lb, 5)
+    # free-space RPY in an open box (with minimum-image vectors in the
+    # periodic box its matrix is not positive definite at this density)
+    from pybrown_amd.serving import OPEN_BOX
+    inp = {"box_length": OPEN_BOX if hydro == "rpy" else Lb, "T": 293.15, "viscosity": 0.01005, "seed": 3,
+           "hydrodynamics": hydro, "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2,
+           "measure_flux": FLUX}
+
+    def trajectory(look):
+        box = _gpu_box(xyz, a, inp)
+        assert lib.pbx_sim_set_lookahead(box._sim, 1 if look else 0) == 0
+        out = []
+        for j in range(10):
+            if j == 6:
+                box.beads[7].r[0] += 0.5  # moved from the host: the look-ahead D is stale
+            every = j < 4
+            box.propagate(1.0, build_Dff=every or j % 2 == 0, cholesky=every or j % 2 == 0)
+            out.append((box._pos.copy(), dict(box.net_flux), box.D.copy()))
+        flux, _ = box.run(1.0, 6, j0=10, diff_freq=3, chol_freq=3)
+        out.append((box._pos.copy(), flux.copy(), box.D.copy()))
+        return out
+
+    plain, ahead = trajectory(False), trajectory(True)
+    for j, ((p0, f0, d0), (p1, f1, d1)) in enumerate(zip(plain, ahead)):
+        assert np.array_equal(p0, p1), "positions, step %d" % j
+        assert np.array_equal(np.asarray(f0 if not isinstance(f0, dict) else list(f0.values())),
+                              np.asarray(f1 if not isinstance(f1, dict) else list(f1.values()))), "flux, step %d" % j
+        assert np.array_equal(d0, d1), "D, step %d" % j

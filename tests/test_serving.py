@@ -60,10 +60,12 @@ def test_make_request_boxes():
 def test_footprint_grows_with_size_and_mode():
     f = [footprint_bytes(N, "rpy_smith") for N in (32, 100, 1000, 3000)]
     assert all(x < y for x, y in zip(f, f[1:]))
-    assert footprint_bytes(1000, "rpy_smith_lub") > 1.9 * footprint_bytes(1000, "rpy_smith") - (4 << 20)
+    # lubrication holds four dense 3008 x 3008 FP64 matrices at N = 1000 (D,
+    # B, the far-field resistance, an inverse scratch), the other matrix
+    # modes three (D, B, the look-ahead copy of D)
+    assert footprint_bytes(1000, "rpy_smith_lub") - footprint_bytes(1000, "rpy_smith") >= 8 * 3008 ** 2
     assert footprint_bytes(1000, "rpy_smith", check_overlaps=True) > footprint_bytes(1000, "rpy_smith")
-    # two dense 3008 x 3008 FP64 matrices at N = 1000
-    assert footprint_bytes(1000, "rpy_smith") > 2 * 8 * 3008 ** 2
+    assert footprint_bytes(1000, "rpy_smith") > 3 * 8 * 3008 ** 2
 
 
 class _Probe:
