@@ -807,7 +807,12 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
     update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
-    store_updated(tile(i, j), ld, u, i == j);
+    double* X = tile(i, j);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int r = u.row(q), c = u.col(q);
+        if (i != j || c <= r) st_dev(X + (size_t)r * ld + c, u.v[q]);
+    }
 }
 
 // ---------------------------------------------------------------------------
