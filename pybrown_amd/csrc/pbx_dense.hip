@@ -729,7 +729,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             v4d out[2][2];
             acc_zero(out);
             mma64(sA, sB, out);
-            acc_for_each(out, [&](int r, int c, double v) { A0[(size_t)r * ld + c] = v; });
+            acc_for_each(out, [&](int r, int c, double v) { st_dev(A0 + (size_t)r * ld + c, v); });
             return;
         }
         double* A1 = tile(i, c0 + 1);
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         __syncthreads();
         acc_zero(L0);
         mma64(sA, sB, L0);
-        acc_for_each(L0, [&](int r, int c, double v) { A0[(size_t)r * ld + c] = v; });
+        acc_for_each(L0, [&](int r, int c, double v) { st_dev(A0 + (size_t)r * ld + c, v); });
         // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
         if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         stage<true>(sA, scr, T64);
@@ -791,7 +791,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         v4d L1[2][2];
         acc_zero(L1);
         mma64(sA, sB, L1);
-        acc_for_each(L1, [&](int r, int c, double v) { A1[(size_t)r * ld + c] = v; });
+        acc_for_each(L1, [&](int r, int c, double v) { st_dev(A1 + (size_t)r * ld + c, v); });
         if (diag) {
             __syncthreads();
             put_t(sA, L1);
