@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1-6; 0 = library default)")
+    ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1, 2, 4, 6; 0 = library default)")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="noise factor: FP64, or FP32 trailing updates with FP64 diagonal tiles")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",

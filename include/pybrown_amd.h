@@ -77,14 +77,13 @@ int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
  * 0 = the library default (6);
  * 1 = one launch per panel, column-by-column diagonal tiles;
  * 2 = one launch per panel, 8-column-panel diagonal tiles;
- * 3 = one persistent launch over the tile dependency graph;
  * 4 = one launch per panel, the solve of the next tile column folded in,
  *     wave-specialised diagonal tiles;
- * 5 = one persistent launch, tiles owned by workgroups;
  * 6 = two tile columns per launch (rank-128 trailing updates).
- * Returns PBX_ERR_ARG for any other value. */
+ * Returns PBX_ERR_ARG for any other value (3 and 5 named persistent
+ * single-launch variants that are no longer built). */
 int pbx_set_potrf_variant(int variant);
-/* The Cholesky variant in use (1-6). */
+/* The Cholesky variant in use (1, 2, 4 or 6). */
 int pbx_get_potrf_variant(void);
 /* Variant 6: when on, the two helper workgroups of each launch leave their
  * jobs (the panel updates of tiles (c0+1, c0) and (c0+1, c0+1)) to the
@@ -93,6 +92,14 @@ int pbx_get_potrf_variant(void);
  * bit either way.  For tests; off by default.  No counterpart in the
  * reference (numpy.linalg.cholesky has no such choice). */
 int pbx_set_potrf_chain_jobs(int on);
+/* Test hook: the k-th launch sequence from now that reports through a
+ * simulation's or call's status word (a factorisation, an SPD inverse's
+ * factorisation, the midpoint's B0^T solve) ends with that word set as if
+ * one of its bounded waits had run out, so tests can check from the host
+ * that the failure is reported by the step that ran it.  0 disarms (the
+ * default); every kernel still runs to completion.  No reference
+ * counterpart. */
+int pbx_debug_inject_wait_failure(int k);
 
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
  * FP64 MFMA tile product used by the factorisations, for testing. */

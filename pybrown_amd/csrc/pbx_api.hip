@@ -271,7 +271,7 @@ static int cholesky_host(const double* A, int n, double* L, int mixed) {
     for (int r = 0; r < n; ++r)
         for (int c = r + 1; c < n; ++c) L[(size_t)r * n + c] = 0.0;
     if (info == PBX_INFO_DAG_WAIT) {
-        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
         return PBX_ERR_STATE;
     }
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
@@ -305,7 +305,7 @@ extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double
     PBX_HIP(hipMemcpy2D(X, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
                         hipMemcpyDeviceToHost));
     if (info == PBX_INFO_DAG_WAIT) {
-        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
         return PBX_ERR_STATE;
     }
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
@@ -484,7 +484,7 @@ int take_info(PbxSim* s) {
     if (!info) return PBX_OK;
     PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
     if (info == PBX_INFO_DAG_WAIT) {
-        pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+        pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
         return PBX_ERR_STATE;
     }
     pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
@@ -499,7 +499,7 @@ int check_info(PbxSim* s) {
     if (info) {
         PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
         if (info == PBX_INFO_DAG_WAIT) {
-            pbx_set_error(PBX_ERR_STATE, "single-launch Cholesky: dependency wait exhausted");
+            pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
             return PBX_ERR_STATE;
         }
         pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
@@ -755,7 +755,11 @@ int divergence_step(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
 // batch instead of one flag per attempt.  Results do not depend on K.
 int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     TRY(det(s, s->D, dt, 1.0, c));
-    if (s->cfg.divergence_term && is_lub(s)) TRY(divergence_step(s, dt, bDff, bDnf, c));
+    if (s->cfg.divergence_term && is_lub(s)) {
+        TRY(divergence_step(s, dt, bDff, bDnf, c));
+        // the 3N rebuilds factorised after the step's status copy was queued
+        TRY(post_info(s));
+    }
     if (!s->cfg.check_overlaps) {
         const double* xi;
         TRY(next_xi(s, &xi));
@@ -837,12 +841,14 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
             if (is_lub(s) && bDnf) TRY(build_Dtot(s, bDff ? s->RffMid : s->Rff, s->Dmid));
             if (is_lub(s) || bDff) Dm = s->Dmid;
             if (is_lub(s) && bDff && !bDnf) Dm = s->D;  // self.D untouched by Dff in lub modes
-            if (is_lub(s)) TRY(post_info(s));
         }
         // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi
         PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
         TRY(pbx_launch_trsv_lt(s->B, s->np, s->ld, s->Wchol, s->xipad, s->w, s->trsv_flags, ++s->trsv_epoch,
                                s->info, s->stream));
+        // status of the midpoint inverses and of the solve's bounded waits,
+        // read at this attempt's overlap check or at the step's end
+        TRY(post_info(s));
         TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
         if (s->F_nonzero)
             TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));

@@ -22,8 +22,9 @@
 #define PBX_ERR_OVERLAP -5   // starting configuration overlaps
 #define PBX_ERR_STATE -6
 // k > 0: the Cholesky factorisation met a non-positive pivot at column k-1
-// (except PBX_INFO_DAG_WAIT: a dependency wait of the single-launch Cholesky
-// ran out of budget; the kernel drained and the factor is not valid)
+// (except PBX_INFO_DAG_WAIT: a bounded wait between workgroups of a
+// factorisation or of the B0^T solve ran out of budget; the kernels drained
+// and their result is not valid)
 #define PBX_INFO_DAG_WAIT 0x40000000
 
 static inline size_t pbx_round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
@@ -42,6 +43,15 @@ extern "C" void pbx_set_error(int code, const char* fmt, ...);
 
 // ---------------------------------------------------------------------------
 // device helpers
+
+// Device-scope load (global_load ... sc1): served by the L2, never by the
+// compute unit's L1.  Data that another workgroup of the same launch stored
+// and published is read through these, so what a consumer sees does not
+// depend on which lines its L1 holds or on when another wave's invalidation
+// of that L1 completes.
+__device__ __forceinline__ double pbx_ld_dev(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Minimum-image reduction with the reference's loop semantics
 // (src/bdsim/clib/bead.c:44-50); the result lies in (-L/2, L/2].  Callers

@@ -15,26 +15,20 @@
 //     trsm(k):   L_ik = A_ik * inv(L_kk)^T           (one workgroup per tile row)
 //     syrk(k):   A_ij -= L_ik L_jk^T  for k < j <= i   (one workgroup per tile)
 //                and the workgroup owning tile (k+1,k+1) factors it in LDS
-// Variant 3 is the single-launch tile graph of pbx_chol_dag.hip.
 // Variant 4 folds trsm(k+1) into syrk(k): the workgroups of tile
 // column k+1 keep their updated tile in registers, wait (bounded) for the
 // look-ahead workgroup to publish inv(L_{k+1,k+1}), and solve in place, so
 // each panel costs one launch; its diagonal tiles use the wave-specialised
 // factorisation (tile_factor_ws).
-// Variant 5 is the persistent launch of pbx_chol_persist.hip.  Variant 6
-// (default) factors two tile columns per launch (potrf2_kernel below): one
+// Variant 6 (default) factors two tile columns per launch (potrf2_kernel below): one
 // rank-128 pass over the trailing matrix per two columns instead of two
 // rank-64 passes, and half the launches.
 // inv(L_kk) is produced with each diagonal factorisation and kept in `work`
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
+#include <atomic>
+
 #include "pbx_common.h"
 #include "pbx_tile.h"
-
-#define TRY_RC(x)            \
-    do {                     \
-        int rc_ = (x);       \
-        if (rc_) return rc_; \
-    } while (0)
 
 using namespace pbx_tile;
 
@@ -104,11 +98,11 @@ __device__ void store_diag(const double* sL, const double* sW, double* A, int ld
     }
 }
 
-// Look-ahead state words (variant 4), in the work area after the diagonal
-// inverses: [LA_FLAG] = index of the last diagonal tile published,
-// [LA_ERR] = a dependency wait ran out of budget.  (Words 2 and 3 of the
-// single-launch graph's state block, which that variant does not use.)
-constexpr int LA_FLAG = 2, LA_ERR = 3;
+// Look-ahead state words (variants 4 and 6), in the work area after the
+// diagonal inverses (LA_WORDS ints): [LA_FLAG] = index of the last diagonal
+// tile published, [LA_ERR] = a dependency wait ran out of budget; variant 6
+// adds [LA_UPD] and [LA_JOB + 0..1] (below).
+constexpr int LA_FLAG = 2, LA_ERR = 3, LA_WORDS = 16;
 
 template <int V>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info,
@@ -231,11 +225,22 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(double* A, int ld, int k, 
     store_diag(sL, sW, A, ld, k + 1, work);
 }
 
-// Wait until *flag >= target (thread 0 polls with device-scope acquire),
-// bounded; false if the budget ran out or another wait already failed.  An
-// exhausted budget sets *err (so the other waits stop) and makes the status
-// word PBX_INFO_DAG_WAIT, also over a pivot status that the unfinished
-// factor may raise after it or has raised before it.
+// Every wave of the workgroup acquires at device scope (buffer_inv sc1).  The
+// invalidation an acquire performs is ordered only before later loads of the
+// wave that issues it; a workgroup barrier does not wait for it, so one
+// wave's acquire does not cover the loads the other waves issue after the
+// barrier.
+__device__ __forceinline__ void acquire_all_waves() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Wait until *flag >= target (thread 0 polls with device-scope acquire; then
+// every wave acquires for itself), bounded; false if the budget ran out or
+// another wait already failed.  An exhausted budget sets *err (so the other
+// waits stop) and makes the status word PBX_INFO_DAG_WAIT, also over a pivot
+// status that the unfinished factor may raise after it or has raised before
+// it.  What the producer published is read with device-scope loads
+// (stage_dev, acc_load_dev) by the caller.
 __device__ bool wait_ge_agent(const int* flag, int target, int* err, int* info) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
@@ -254,6 +259,7 @@ __device__ bool wait_ge_agent(const int* flag, int target, int* err, int* info) 
         ok = good;
     }
     __syncthreads();
+    acquire_all_waves();
     return ok != 0;
 }
 
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, in
 #pragma unroll
             for (int v = 0; v < 4; ++v, ++q) sA[acc_col(y) * SP + acc_row(x, v)] = cv[q] - acc[x][y][v];
     if (!wait_ge_agent(la + LA_FLAG, k + 1, la + LA_ERR, info)) return;
-    stage<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
+    stage_dev<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
     __syncthreads();
     v4d out[2][2];
     acc_zero(out);
@@ -664,6 +670,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
+        acquire_all_waves();
         bool helped = false;
         // (marked unlikely: the compiler then keeps this code out of the
         // chain's straight-line path, which is measurably faster)
@@ -676,7 +683,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         }
         if (helped) {
-            stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+            stage_dev<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         } else {
             // W^T in sB (op(B)(kk, c) = W[c][kk]); W is lower triangular
             for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
@@ -690,8 +697,8 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         }
         double* Aoff = tile(c0 + 1, c0);
         double cv[16];
-        acc_load(tile(c0 + 1, c0 + 1), ld, cv);
-        stage<true>(sA, scr, T64);
+        acc_load_dev(tile(c0 + 1, c0 + 1), ld, cv);
+        stage_dev<true>(sA, scr, T64);
         __syncthreads();
         v4d Lo[2][2];
         acc_zero(Lo);
@@ -724,7 +731,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
             if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
-            stage<true>(sB, W0, T64);
+            stage_dev<true>(sB, W0, T64);
             __syncthreads();
             v4d out[2][2];
             acc_zero(out);
@@ -758,14 +765,14 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             put_updated_t(sA, u);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
-        stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+        stage_dev<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         __syncthreads();
         acc_zero(L0);
         mma64(sA, sB, L0);
         acc_for_each(L0, [&](int r, int c, double v) { st_dev(A0 + (size_t)r * ld + c, v); });
         // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
         if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
-        stage<true>(sA, scr, T64);
+        stage_dev<true>(sA, scr, T64);
         __syncthreads();
         v4d Lo[2][2];
         acc_zero(Lo);
@@ -786,7 +793,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             put_diff_t(sA, c1, S);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err, info)) return;
-        stage<true>(sB, W1, T64);
+        stage_dev<true>(sB, W1, T64);
         __syncthreads();
         v4d L1[2][2];
         acc_zero(L1);
@@ -1030,41 +1037,17 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const dou
 
 }  // namespace
 
-size_t pbx_dag_state_ints(int np);
-int pbx_potrf_dag(double* A, int np, int ld, double* work, int* info_dev, int* state, int grid, hipStream_t s);
-bool pbx_persist_fits(int np, int cus);
-int pbx_potrf_persist(double* A, int np, int ld, double* work, int* info_dev, int* state, int cus, hipStream_t s);
-
-// compute units of the current device (queried once)
-static int device_cus() {
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0, n = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
-            cus = n;
-        else
-            cus = 1;
-    }
-    return cus;
-}
-
-// work = [T inverse tiles of the diagonal factor][single-launch DAG state]
-//        [one scratch tile, 16-byte aligned]
-static size_t dense_scratch_offset(int np) {
-    const size_t o = (size_t)np * T64 + (pbx_dag_state_ints(np) + 1) / 2;
-    return (o + 1) & ~(size_t)1;
-}
+// work = [T inverse tiles of the diagonal factor][LA_WORDS look-ahead ints]
+//        [one scratch tile]
+static size_t dense_scratch_offset(int np) { return (size_t)np * T64 + LA_WORDS / 2; }
 size_t pbx_dense_work_doubles(int np) { return dense_scratch_offset(np) + (size_t)T64 * T64; }
 
 // Cholesky variant: 1 = launch per panel, column-by-column diagonal tiles;
-// 2 = launch per panel, 8-column panel diagonal tiles; 3 = single persistent
-// launch over the tile dependency graph (pbx_chol_dag.hip); 4 = one launch
-// per panel with the solve folded in and wave-specialised diagonal tiles;
-// 5 = persistent launch with tiles owned in registers (pbx_chol_persist.hip;
-// falls back to 4 when the tiles do not fit the compute units); 6 (default)
-// = two tile columns per launch (rank-128 trailing updates); 0 selects the
-// default.
+// 2 = launch per panel, 8-column panel diagonal tiles; 4 = one launch per
+// panel with the solve folded in and wave-specialised diagonal tiles; 6
+// (default) = two tile columns per launch (rank-128 trailing updates); 0
+// selects the default.  (3 and 5 were persistent single-launch variants,
+// slower than 6; they are no longer built.)
 constexpr int kDefaultPotrf = 6;
 static int g_potrf_variant = kDefaultPotrf;
 static int g_own_cu_tiles = 64;
@@ -1074,11 +1057,34 @@ extern "C" int pbx_set_potrf_chain_jobs(int on) {
     return PBX_OK;
 }
 extern "C" int pbx_set_potrf_variant(int v) {
-    if (v < 0 || v > 6) return PBX_ERR_ARG;
+    if (v < 0 || v > 6 || v == 3 || v == 5) return PBX_ERR_ARG;
     g_potrf_variant = v ? v : kDefaultPotrf;
     return PBX_OK;
 }
 extern "C" int pbx_get_potrf_variant(void) { return g_potrf_variant; }
+
+// Test hook (pbx_debug_inject_wait_failure): counts down the launch
+// sequences that report through a status word -- factorisations and the
+// midpoint's B0^T solve -- and appends to the k-th one a kernel that sets its
+// status word to PBX_INFO_DAG_WAIT, as an exhausted bounded wait would.
+static std::atomic<int> g_inject{0};
+
+__global__ void inject_status_kernel(int* info) {
+    if (threadIdx.x == 0) atomicExch(info, PBX_INFO_DAG_WAIT);
+}
+
+void pbx_status_sequence_queued(int* info_dev, hipStream_t s) {
+    int k = g_inject.load();
+    while (k > 0 && !g_inject.compare_exchange_weak(k, k - 1)) {
+    }
+    if (k == 1) hipLaunchKernelGGL(inject_status_kernel, dim3(1), dim3(64), 0, s, info_dev);
+}
+
+extern "C" int pbx_debug_inject_wait_failure(int k) {
+    if (k < 0) return PBX_ERR_ARG;
+    g_inject.store(k);
+    return PBX_OK;
+}
 
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s) {
     if (np == n) return PBX_OK;
@@ -1159,18 +1165,9 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         return PBX_ERR_ARG;
     }
     const int T = np / T64;
-    if (g_potrf_variant == 3) {
-        int* state = (int*)(work + (size_t)np * T64);
-        TRY_RC(pbx_potrf_dag(A, np, ld, work, info_dev, state, 0, s));
-        return PBX_OK;
-    }
-    if (g_potrf_variant == 5 && pbx_persist_fits(np, device_cus())) {
-        int* state = (int*)(work + (size_t)np * T64);
-        TRY_RC(pbx_potrf_persist(A, np, ld, work, info_dev, state, device_cus(), s));
-        return PBX_OK;
-    }
     if (g_potrf_variant == 6 || mixed) {
         potrf_two_col(A, np, ld, work, info_dev, s, mixed != 0);
+        pbx_status_sequence_queued(info_dev, s);
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
@@ -1182,6 +1179,7 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
             int cnt = (T - 1 - k) * (T - k) / 2;
             hipLaunchKernelGGL(syrk_trsm_kernel<5>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev, la);
         }
+        pbx_status_sequence_queued(info_dev, s);
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
@@ -1198,6 +1196,7 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         else
             hipLaunchKernelGGL(syrk_kernel<1>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
     }
+    pbx_status_sequence_queued(info_dev, s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -1217,6 +1216,7 @@ int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int
     hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, np, ld, la);
     if (!two_col) return pbx_potrf(B, np, ld, work, info_dev, s, mixed);
     potrf_two_col(B, np, ld, work, info_dev, s, mixed != 0, true);
+    pbx_status_sequence_queued(info_dev, s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -225,9 +225,12 @@ __global__ __launch_bounds__(256) void trsv_lt_rows_kernel(const double* __restr
         }
         __syncthreads();
         if (!ok) return;
+        // y_m was stored by another workgroup of this launch: every wave
+        // acquires for itself and reads it with device-scope loads
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double* ym = y + 64 * m;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) acc += lv[q] * ym[g + 4 * q];
+        for (int q = 0; q < 16; ++q) acc += lv[q] * pbx_ld_dev(ym + g + 4 * q);
     }
     part[g][c] = acc;
     __syncthreads();
@@ -623,6 +626,8 @@ int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, cons
     return PBX_OK;
 }
 
+void pbx_status_sequence_queued(int* info_dev, hipStream_t s);
+
 int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
                        int epoch, int* info, hipStream_t s) {
     if (np <= 0 || np % 64 || epoch <= 0) {
@@ -630,6 +635,7 @@ int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const 
         return PBX_ERR_ARG;
     }
     hipLaunchKernelGGL(trsv_lt_rows_kernel, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, flags, epoch, info);
+    pbx_status_sequence_queued(info, s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -1,5 +1,5 @@
 // 64 x 64 FP64 tile primitives shared by the factorisation kernels
-// (pbx_dense.hip, pbx_chol_dag.hip).  All helpers assume a 256-thread
+// (pbx_dense.hip).  All helpers assume a 256-thread
 // workgroup (four waves).
 //
 // LDS operand layout: s[k * SP + r] = op(X)(r, k) ("k-major"), so a
@@ -61,6 +61,21 @@ __device__ __forceinline__ void stage(double* s, const double* X, int ld) {
     stage_put<RK>(s, v);
 }
 
+// stage() with device-scope loads (pbx_ld_dev): 16 eight-byte loads per
+// thread, all issued before the LDS stores
+template <bool RK>
+__device__ __forceinline__ void stage_dev(double* s, const double* X, int ld) {
+    v2d v[8];
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = threadIdx.x + it * 256;
+        const double* p = X + (size_t)(e >> 5) * ld + (e & 31) * 2;
+        v[it].x = pbx_ld_dev(p);
+        v[it].y = pbx_ld_dev(p + 1);
+    }
+    stage_put<RK>(s, v);
+}
+
 // acc += op(A) op(B) over a 64-deep slab; wave w owns the 32 x 32 quadrant
 // (w/2, w%2) as 2 x 2 MFMA blocks.
 __device__ __forceinline__ void mma64(const double* sA, const double* sB, v4d acc[2][2]) {
@@ -113,6 +128,17 @@ __device__ __forceinline__ void acc_load(const double* C, int ld, double out[16]
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) out[q++] = C[(size_t)acc_row(x, v) * ld + acc_col(y)];
+}
+
+// acc_load with device-scope loads (pbx_ld_dev)
+__device__ __forceinline__ void acc_load_dev(const double* C, int ld, double out[16]) {
+    int q = 0;
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) out[q++] = pbx_ld_dev(C + (size_t)acc_row(x, v) * ld + acc_col(y));
 }
 
 // ---------------------------------------------------------------------------

@@ -150,7 +150,7 @@ def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attem
     else:
         mats = 2 + (1 if mid else 0) + ((2 + (1 if mid else 0)) if lub else 1)
         works = 2 if lub else 1
-    work = 8 * (np_ * 64 + (16 + T * T + T) // 2 + 2 + 64 * 64)
+    work = 8 * (np_ * 64 + 8 + 64 * 64)
     ring = 8 * 2 * xi_pool_attempts * n
     vecs = 8 * (6 * np_ + 2 * N) + 4 * N + 4096
     batch = 0

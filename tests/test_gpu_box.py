@@ -1,8 +1,14 @@
 """Trajectories of the device-resident Box against the oracle step
 (oracle/box_oracle.py) with the same seeded RandomState stream.
 
-Bar: positions within 1e-10 relative (max|dr| / max|r|) after every step,
-flux counts, move-attempt counts and draw counts identical."""
+Bars, after every step:
+  - positions within 1e-10 relative (max|dr| / max|r|);
+  - the step's displacement d = r_j - r_{j-1} (minimum image) within 1e-11
+    relative (max|d_gpu - d_oracle| / max|d_oracle|), so an error in D, B or
+    the lubrication correction is measured against the move, not against
+    box-sized coordinates; lubrication cases take 1e-10 (the near-contact
+    resistance is ill-conditioned);
+  - flux counts, move-attempt counts and draw counts identical."""
 import math
 
 import numpy as np
@@ -23,11 +29,29 @@ def _gpu_box(xyz, a, inp, labels=None, **kw):
     return Box(beads, inp, **kw)
 
 
-def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None, **kw):
+def disp_rel_err(g0, g1, o0, o1, L):
+    """max|d_gpu - d_oracle| / max|d_oracle| for one step's displacements,
+    each reduced to the minimum image of the box (length L)."""
+    dg, do = g1 - g0, o1 - o0
+    if L:
+        dg = dg - L * np.round(dg / L)
+        do = do - L * np.round(do / L)
+    den = np.abs(do).max()
+    num = np.abs(dg - do).max()
+    return num / den if den > 0 else num
+
+
+def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None, disp_tol=None, **kw):
     labels = labels or ["XXX"] * len(a)
+    if disp_tol is None:
+        lub = "lub" in inp.get("hydrodynamics", "")
+        disp_tol = 0.0 if tol == 0.0 else (1e-10 if lub else 1e-11)
     g = _gpu_box(xyz, a, inp, labels, **kw)
     o = OracleBox(xyz, a, labels, inp)
-    worst = 0.0
+    L = inp.get("box_length")
+    worst = worst_d = 0.0
+    prev_g = np.array([b.r for b in g.beads])
+    prev_o = o.r.copy()
     for j in range(steps):
         f = flags(j) if flags else (True, True, True)
         g.propagate(dt, *f)
@@ -36,10 +60,15 @@ def _run_pair(xyz, a, inp, steps, dt, labels=None, tol=1e-10, flags=None, **kw):
         err = np.abs(pos_g - o.r).max() / np.abs(o.r).max()
         worst = max(worst, err)
         assert err <= tol, "step %d: position error %.3e" % (j, err)
+        derr = disp_rel_err(prev_g, pos_g, prev_o, o.r, L)
+        worst_d = max(worst_d, derr)
+        assert derr <= disp_tol, "step %d: displacement error %.3e" % (j, derr)
+        prev_g, prev_o = pos_g, o.r.copy()
         if "measure_flux" in inp:
             assert g.net_flux == o.net_flux, "step %d flux %s vs %s" % (j, g.net_flux, o.net_flux)
         assert g.last_move_attempts == o.attempt_log[-1]
     assert g.draw_count == o.draw_count
+    print("worst position error %.3e, worst displacement error %.3e over %d steps" % (worst, worst_d, steps))
     return g, o, worst
 
 
@@ -58,16 +87,19 @@ def test_ermak_nohi_bit_exact():
     assert worst == 0.0
 
 
+# dt = 0.01 ps moves a 1 A bead about 0.07 A per step (D0 = 0.22 A^2/ps at
+# 298 K, 0.01 P): displacements of physical size, so the per-step
+# displacement bar sees errors of D and B instead of rounding of positions
 def test_ermak_rpy_grid():
     xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
-    _run_pair(xyz, a, dict(BASE, hydrodynamics="rpy"), 100, 1e-10)
+    _run_pair(xyz, a, dict(BASE, hydrodynamics="rpy"), 100, 0.01)
 
 
 @pytest.mark.parametrize("mn", [0, 1, 2])
 def test_ermak_rpy_smith_grid(mn):
     xyz, a = _grid([-3.0, 0.0, 3.0], 1.0)
     inp = dict(BASE, hydrodynamics="rpy_smith", ewald_alpha=math.sqrt(math.pi), ewald_real=mn, ewald_imag=mn)
-    _run_pair(xyz, a, inp, 100, 1e-10)
+    _run_pair(xyz, a, inp, 100, 0.01)
 
 
 def test_c1_rpy_flux_first_200_steps():
@@ -95,13 +127,17 @@ def test_rejections_match():
     assert sum(o.attempt_log) > len(o.attempt_log), "workload produced no rejections"
 
 
-@pytest.mark.parametrize("mn,m_mid", [(1, 2.0), (2, 100.0)])
-def test_midpoint_rpy_smith_lub_grid(mn, m_mid):
-    # test_box.py:409-511 design (cutoff 10, grid spacing 2.2)
+@pytest.mark.parametrize("mn,m_mid,dt", [(1, 2.0, 1e-10), (2, 100.0, 1e-10), (1, 2.0, 0.002), (2, 100.0, 0.002)])
+def test_midpoint_rpy_smith_lub_grid(mn, m_mid, dt):
+    # test_box.py:409-511 design (cutoff 10, grid spacing 2.2), at the
+    # reference test's dt = 1e-10 ps and at dt = 0.002 ps, where a step moves
+    # a bead about 0.02 A across the 0.2 A gaps (rejections included).  At
+    # 1e-10 ps a step moves about 7e-6 A, so rounding of the 2.2 A positions
+    # alone is 6e-11 of a move: the displacement bar there is 1e-9.
     xyz, a = _grid([-2.2, 0.0, 2.2], 1.0)
     inp = dict(BASE, hydrodynamics="rpy_smith_lub", propagation_scheme="midpoint", lubrication_cutoff=10,
                ewald_alpha=math.sqrt(math.pi), ewald_real=mn, ewald_imag=mn, m_midpoint=m_mid)
-    _run_pair(xyz, a, inp, 30, 1e-10)
+    _run_pair(xyz, a, inp, 30, dt, disp_tol=1e-9 if dt < 1e-6 else 1e-10)
 
 
 def test_midpoint_c3_like_bidisperse():
@@ -154,8 +190,9 @@ def test_labels_split_flux():
     assert set(g.net_flux) == {"A", "B"}
 
 
-def test_single_launch_cholesky_trajectory():
-    # the persistent tile-graph Cholesky inside the step (variant 3)
+def test_panel_launch_cholesky_trajectory():
+    # the one-launch-per-panel Cholesky with the folded solve inside the step
+    # (variant 4: its tile-column workgroups wait for the published inverse)
     from pybrown_amd._lib import lib
     from pybrown_amd.workloads import rsa_positions, box_for
     a = np.full(200, 10.0)
@@ -163,7 +200,7 @@ def test_single_launch_cholesky_trajectory():
     xyz = rsa_positions(a, Lb, 12)
     inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 2, "hydrodynamics": "rpy_smith",
            "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "measure_flux": FLUX}
-    assert lib.pbx_set_potrf_variant(3) == 0
+    assert lib.pbx_set_potrf_variant(4) == 0
     try:
         _run_pair(xyz, a, inp, 10, 1.0)
     finally:

@@ -80,8 +80,11 @@ def test_results_are_the_same_alone_and_in_the_mix():
     params = []
     for mode in ("rpy_smith", "rpy", "rpy_smith_lub"):
         params += [p for p in stream if p["N"] <= 160 and p["mode"] == mode][:3]
+    # lubrication requests with 16 and more tile columns (3N > 960): their
+    # factorisations take the one-workgroup-per-CU build with helper jobs
+    params += [p for p in stream if 320 <= p["N"] <= 450 and p["mode"] == "rpy_smith_lub"][:2]
     params.sort(key=lambda p: p["rid"])
-    assert len(params) == 9
+    assert len(params) == 11
     reqs = [make_request(p, steps=10) for p in params]
     alone = Server(slots=1).run(reqs, realtime=False)
     mixed = Server(slots=4).run(reqs, realtime=False)
@@ -94,3 +97,42 @@ def test_results_are_the_same_alone_and_in_the_mix():
     assert np.array_equal(x, alone[0].xyz) and np.array_equal(f, alone[0].flux)
     s = summary(mixed)
     assert s["completed"] == len(reqs)
+
+
+def test_c5_sample_served_against_oracle():
+    """A stratified sample of the served request stream (request k: seed k),
+    served concurrently on 8 slots with 20 steps each (the stream's 200 cut
+    to 20 so the CPU oracle stays within the test's time): every request
+    with N <= 200 follows the oracle trajectory (oracle/box_oracle.py, the
+    reference's noise stream) with its per-step flux exact and final
+    positions within 1e-10 relative; the sample's largest lubrication
+    request (3N above 960, 16+ tile columns) and a large Ewald request give
+    the same bits served in the mix as run alone."""
+    from oracle.box_oracle import OracleBox
+    stream = stream_params(4096)
+    small = []
+    for mode in ("rpy_smith", "rpy", "rpy_smith_lub"):
+        small += [p for p in stream if p["N"] <= 200 and p["mode"] == mode][:2]
+    big_lub = next(p for p in stream if 330 <= p["N"] <= 600 and p["mode"] == "rpy_smith_lub")
+    big_ewald = next(p for p in stream if 700 <= p["N"] <= 1000 and p["mode"] == "rpy_smith")
+    params = sorted(small + [big_lub, big_ewald], key=lambda p: p["rid"])
+    reqs = [make_request(p, steps=20) for p in params]
+    served = Server(slots=8).run(reqs, realtime=False)
+    for r in served:
+        assert r.status == "ok", (r.rid, r.status)
+    by_rid = {r.rid: r for r in served}
+    for req in reqs:
+        res = by_rid[req.rid]
+        if req.N <= 200:
+            o = OracleBox(req.xyz, req.radii, ["XXX"] * req.N, req.inp)
+            flux = []
+            for _ in range(req.steps):
+                o.propagate(req.dt)
+                flux.append(o.net_flux["XXX"])
+            err = np.abs(res.xyz - o.r).max() / np.abs(o.r).max()
+            print("rid %d %s N=%d: position error %.3e after %d steps" % (req.rid, req.mode, req.N, err, req.steps))
+            assert err <= 1e-10, req.rid
+            assert np.array_equal(res.flux[:, 0], np.array(flux)), req.rid
+        if req.rid in (big_lub["rid"], big_ewald["rid"]):
+            f, x, _ = run_one(req)
+            assert np.array_equal(x, res.xyz) and np.array_equal(f, res.flux), req.rid
