@@ -205,7 +205,7 @@ def serve_bench(args):
     trajectory."""
     # hardware queues the HIP runtime spreads the simulations' streams over;
     # read once when the runtime starts, so set before anything touches it
-    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or str(min(max(args.slots, 4), 16))
+    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or str(min(max(args.slots, 4), 32))
     import functools
     from pybrown_amd.serving import Server, make_requests_parallel, run_one, stream_params, summary
     params = stream_params(args.requests)

@@ -44,15 +44,6 @@ extern "C" void pbx_set_error(int code, const char* fmt, ...);
 // ---------------------------------------------------------------------------
 // device helpers
 
-// Device-scope load (global_load ... sc1): served by the L2, never by the
-// compute unit's L1.  Data that another workgroup of the same launch stored
-// and published is read through these, so what a consumer sees does not
-// depend on which lines its L1 holds or on when another wave's invalidation
-// of that L1 completes.
-__device__ __forceinline__ double pbx_ld_dev(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Minimum-image reduction with the reference's loop semantics
 // (src/bdsim/clib/bead.c:44-50); the result lies in (-L/2, L/2].  Callers
 // guarantee L > 0; non-finite d is returned unchanged and |d| > 64 L is first

@@ -239,8 +239,7 @@ __device__ __forceinline__ void acquire_all_waves() {
 // another wait already failed.  An exhausted budget sets *err (so the other
 // waits stop) and makes the status word PBX_INFO_DAG_WAIT, also over a pivot
 // status that the unfinished factor may raise after it or has raised before
-// it.  What the producer published is read with device-scope loads
-// (stage_dev, acc_load_dev) by the caller.
+// it.
 __device__ bool wait_ge_agent(const int* flag, int target, int* err, int* info) {
     __shared__ int ok;
     if (threadIdx.x == 0) {
@@ -328,7 +327,7 @@ __global__ __launch_bounds__(256, 2) void syrk_trsm_kernel(double* A, int ld, in
 #pragma unroll
             for (int v = 0; v < 4; ++v, ++q) sA[acc_col(y) * SP + acc_row(x, v)] = cv[q] - acc[x][y][v];
     if (!wait_ge_agent(la + LA_FLAG, k + 1, la + LA_ERR, info)) return;
-    stage_dev<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
+    stage<true>(sB, work + (size_t)(k + 1) * T64 * T64, T64);  // op(B)(kk, c) = W[c][kk]
     __syncthreads();
     v4d out[2][2];
     acc_zero(out);
@@ -683,7 +682,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         }
         if (helped) {
-            stage_dev<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+            stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         } else {
             // W^T in sB (op(B)(kk, c) = W[c][kk]); W is lower triangular
             for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
@@ -697,8 +696,8 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         }
         double* Aoff = tile(c0 + 1, c0);
         double cv[16];
-        acc_load_dev(tile(c0 + 1, c0 + 1), ld, cv);
-        stage_dev<true>(sA, scr, T64);
+        acc_load(tile(c0 + 1, c0 + 1), ld, cv);
+        stage<true>(sA, scr, T64);
         __syncthreads();
         v4d Lo[2][2];
         acc_zero(Lo);
@@ -731,7 +730,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
             if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
-            stage_dev<true>(sB, W0, T64);
+            stage<true>(sB, W0, T64);
             __syncthreads();
             v4d out[2][2];
             acc_zero(out);
@@ -765,14 +764,14 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             put_updated_t(sA, u);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
-        stage_dev<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+        stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
         __syncthreads();
         acc_zero(L0);
         mma64(sA, sB, L0);
         acc_for_each(L0, [&](int r, int c, double v) { st_dev(A0 + (size_t)r * ld + c, v); });
         // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
         if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
-        stage_dev<true>(sA, scr, T64);
+        stage<true>(sA, scr, T64);
         __syncthreads();
         v4d Lo[2][2];
         acc_zero(Lo);
@@ -793,7 +792,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             put_diff_t(sA, c1, S);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err, info)) return;
-        stage_dev<true>(sB, W1, T64);
+        stage<true>(sB, W1, T64);
         __syncthreads();
         v4d L1[2][2];
         acc_zero(L1);

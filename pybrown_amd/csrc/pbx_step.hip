@@ -226,11 +226,11 @@ __global__ __launch_bounds__(256) void trsv_lt_rows_kernel(const double* __restr
         __syncthreads();
         if (!ok) return;
         // y_m was stored by another workgroup of this launch: every wave
-        // acquires for itself and reads it with device-scope loads
+        // acquires for itself
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const double* ym = y + 64 * m;
 #pragma unroll
-        for (int q = 0; q < 16; ++q) acc += lv[q] * pbx_ld_dev(ym + g + 4 * q);
+        for (int q = 0; q < 16; ++q) acc += lv[q] * ym[g + 4 * q];
     }
     part[g][c] = acc;
     __syncthreads();

@@ -61,21 +61,6 @@ __device__ __forceinline__ void stage(double* s, const double* X, int ld) {
     stage_put<RK>(s, v);
 }
 
-// stage() with device-scope loads (pbx_ld_dev): 16 eight-byte loads per
-// thread, all issued before the LDS stores
-template <bool RK>
-__device__ __forceinline__ void stage_dev(double* s, const double* X, int ld) {
-    v2d v[8];
-#pragma unroll
-    for (int it = 0; it < 8; ++it) {
-        const int e = threadIdx.x + it * 256;
-        const double* p = X + (size_t)(e >> 5) * ld + (e & 31) * 2;
-        v[it].x = pbx_ld_dev(p);
-        v[it].y = pbx_ld_dev(p + 1);
-    }
-    stage_put<RK>(s, v);
-}
-
 // acc += op(A) op(B) over a 64-deep slab; wave w owns the 32 x 32 quadrant
 // (w/2, w%2) as 2 x 2 MFMA blocks.
 __device__ __forceinline__ void mma64(const double* sA, const double* sB, v4d acc[2][2]) {
@@ -128,17 +113,6 @@ __device__ __forceinline__ void acc_load(const double* C, int ld, double out[16]
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) out[q++] = C[(size_t)acc_row(x, v) * ld + acc_col(y)];
-}
-
-// acc_load with device-scope loads (pbx_ld_dev)
-__device__ __forceinline__ void acc_load_dev(const double* C, int ld, double out[16]) {
-    int q = 0;
-#pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) out[q++] = pbx_ld_dev(C + (size_t)acc_row(x, v) * ld + acc_col(y));
 }
 
 // ---------------------------------------------------------------------------
@@ -385,7 +359,11 @@ __device__ __forceinline__ void lds_release_add(int* f, int v) {
 }
 
 __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int col0, int* info) {
-    __shared__ int fl[2];      // [0]: panels published by wave 0; [1]: staged panel inputs (2 per panel)
+    // [0]: panels published by wave 0; [1], [2]: panels whose next-but-one
+    // columns wave 2, wave 3 has staged.  One counter per producer: a shared
+    // count of both waves' stagings could reach its target while one wave was
+    // a panel ahead and the other still staging.
+    __shared__ int fl[3];
     __shared__ double sy[64];  // 1 / L_jj
     __shared__ double sB[64];  // the next panel's diagonal block, row-major 8 x 8
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -404,6 +382,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
     if (tid == 0) {
         fl[0] = 0;
         fl[1] = 0;
+        fl[2] = 0;
     }
     if (w >= 2) {
 #pragma unroll
@@ -489,7 +468,10 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
 #pragma unroll
             for (int i = 0; i < 64; ++i) lr[i] = sL[(n0 + (i >> 3)) * SP + c0 + (i & 7)];
             __builtin_amdgcn_sched_barrier(0);
-            if (p > 0) lds_wait_ge(&fl[1], 2 * p, info);
+            if (p > 0) {
+                lds_wait_ge(&fl[1], p, info);
+                lds_wait_ge(&fl[2], p, info);
+            }
             double nv[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) nv[q] = (lane >= n0) ? sL[lane * SP + n0 + q] : 0.0;
@@ -541,7 +523,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
                             }
                     }
                 }
-                if (pass == 0 && lane == 0) lds_release_add(&fl[1], 1);
+                if (pass == 0 && lane == 0) lds_release_add(&fl[w - 1], 1);
             }
         }
     } else {

@@ -4,6 +4,11 @@ import sys
 import numpy as np
 import pytest
 
+# Hardware queues the HIP runtime spreads streams over, read when it starts:
+# 16, so tests that run simulations concurrently overlap their kernels on the
+# device as the serving loop does.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
