@@ -73,6 +73,22 @@ int pbx_cholesky_mixed(const double* A, int n, double* L);
 /* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
 
+/* pbx_cholesky for a matrix already on the device, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  D_dev: the n x n SPD matrix, row-major, leading
+ * dimension ld_D >= n (read only).  B_dev: np = round_up(n, 64) rows,
+ * leading dimension ld_B >= np (even), 16-byte aligned; receives the lower
+ * factor in its leading n x n block with the upper triangle zero, the
+ * identity in rows and columns n..np-1, the same bits as pbx_cholesky with
+ * the default Cholesky variant.  stream: a hipStream_t (NULL = the null
+ * stream); the call returns after the stream has run it.  Limit: n <= 12288
+ * (larger orders return PBX_ERR_ARG).  The work area and status word live in
+ * one static device array per GPU (6.3 MB, held until the process ends),
+ * shared by every call on that GPU: calls that overlap on one GPU race.
+ * Returns 0, k > 0 for a non-positive pivot at column k-1, or a negative
+ * error. */
+int pbx_cholesky_noalloc(const double* D_dev, int n, int ld_D, double* B_dev, int ld_B, void* stream);
 /* Selects the Cholesky used by every factorisation in this process:
  * 0 = the library default (6);
  * 1 = one launch per panel, column-by-column diagonal tiles;

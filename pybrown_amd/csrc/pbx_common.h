@@ -30,6 +30,7 @@
 static inline size_t pbx_round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
 
 extern "C" void pbx_set_error(int code, const char* fmt, ...);
+extern "C" void pbx_clear_error(void);
 
 #define PBX_HIP(call)                                                                      \
     do {                                                                                   \
@@ -173,6 +174,11 @@ int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s
 int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
                    int mixed);
 size_t pbx_dense_work_doubles(int np);
+// The factorisation's work area: np x 64 doubles of diagonal-tile inverses,
+// then PBX_LA_INTS look-ahead ints (zero before a two-column factorisation),
+// then one 64 x 64 scratch tile.
+constexpr int PBX_LA_INTS = 16;
+int pbx_potrf_two_col_ready(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s);
 
 // batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()
 // consecutive attempts (xi of attempt k in ring slot (ring_pos + k) mod

@@ -102,7 +102,7 @@ __device__ void store_diag(const double* sL, const double* sW, double* A, int ld
 // diagonal inverses (LA_WORDS ints): [LA_FLAG] = index of the last diagonal
 // tile published, [LA_ERR] = a dependency wait ran out of budget; variant 6
 // adds [LA_UPD] and [LA_JOB + 0..1] (below).
-constexpr int LA_FLAG = 2, LA_ERR = 3, LA_WORDS = 16;
+constexpr int LA_FLAG = 2, LA_ERR = 3, LA_WORDS = PBX_LA_INTS;
 
 template <int V>
 __global__ __launch_bounds__(256) void potrf_diag_kernel(double* A, int ld, int k, double* work, int* info,
@@ -1151,6 +1151,20 @@ static void potrf_two_col(double* A, int np, int ld, double* work, int* info_dev
         potrf_two_col_launch<true>(A, np, ld, work, info_dev, s, la_ready);
     else
         potrf_two_col_launch<false>(A, np, ld, work, info_dev, s, la_ready);
+}
+
+// The two-column factorisation (variant 6, FP64) of A in place, its
+// look-ahead words already zero (cleared by the kernel before it on the same
+// stream).  For callers that stage A themselves (pbx_noalloc.hip).
+int pbx_potrf_two_col_ready(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s) {
+    if (np % T64 != 0 || np <= 0) {
+        pbx_set_error(PBX_ERR_ARG, "potrf: order %d is not a positive multiple of 64", np);
+        return PBX_ERR_ARG;
+    }
+    potrf_two_col(A, np, ld, work, info_dev, s, false, true);
+    pbx_status_sequence_queued(info_dev, s);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
 }
 
 // In-place lower Cholesky of the np x np matrix A; inverses of the diagonal
