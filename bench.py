@@ -55,12 +55,29 @@ def cpu_baseline(name, steps=3, threads=1):
             box.propagate(dt)
             times.append(time.perf_counter() - t0)
     per_step = float(np.median(times))
-    how = ("clib kernels built -O2 from the reference sources + numpy.linalg.cholesky, 1 thread" if not omp else
-           "clib kernels built -O2 -fopenmp from the reference sources (its parallel loop over bead rows) + "
-           "numpy.linalg.cholesky, %d OpenMP and BLAS threads" % threads)
+    how = ("the reference's clib kernels (diff_tensor.c, bead.c) compiled from its sources at -O2 (its own "
+           "default build is unoptimised), driven by a numpy restatement of box.py's step (oracle/box_oracle.py) "
+           "with numpy.linalg.cholesky; 1 thread" if not omp else
+           "the same, the clib sources compiled -O2 -fopenmp (the reference's own parallel loop over bead rows), "
+           "%d OpenMP and BLAS threads" % threads)
     return {"value": 1.0 / per_step, "unit": "steps/s", "cores": threads, "kind": kind,
             "sample": "%d timed steps (after 1 warm-up) of the same N=%d workload; median %.3f s/step; %s"
-                      % (steps, len(a), per_step, how)}
+                      % (steps, len(a), per_step, how),
+            "host": host_info()}
+
+
+def host_info():
+    """os.cpu_count() and the CPU model of this host (/proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_count": os.cpu_count(), "cpu_model": model}
 
 
 def host_threads():
@@ -128,11 +145,13 @@ def main():
     # stage timing: HIP events recorded on the simulation's own stream inside
     # every timed step (the step ends in a stream synchronisation anyway)
     lib.pbx_sim_set_timing(box._sim, 1)
+    k0 = lib.pbx_kernel_launches()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         box.propagate(dt)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    kernels_per_step = (lib.pbx_kernel_launches() - k0) / args.steps
     st = np.zeros(3)
     nt = lib.pbx_sim_get_timing(box._sim, st.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     lib.pbx_sim_set_timing(box._sim, 0)
@@ -161,8 +180,11 @@ def main():
         "config": {"workload": WORKLOAD_TEXT[args.workload], "N": N, "box_length": round(inp["box_length"], 4),
                    "precision": args.precision},
         "stage_ms": {"assembly": stage[0], "cholesky": stage[1], "propagation": stage[2]},
+        "kernels_per_step": kernels_per_step,
         "roofline": roofline(args.workload, N, stage, args.precision),
     }
+    if args.workload == "c2":
+        out["c1_leg"] = c1_leg()
     if accuracy is not None:
         out["accuracy"] = accuracy
     if not args.no_cpu:
@@ -170,6 +192,35 @@ def main():
         # the same reference path on all host cores (at most 16), beside it
         out["cpu_baseline_multicore"] = cpu_baseline(args.workload, steps=5, threads=host_threads())
     print(json.dumps(out))
+
+
+def c1_leg(warmup=20, steps=500):
+    """The launch-bound small case beside the headline: free-space RPY,
+    N = 100 (a = 5 A, L = 150 A), Ermak with overlap checks, flux x = 0;
+    steps/s and kernels launched per step."""
+    import torch
+
+    from pybrown_amd._lib import lib
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    from pybrown_amd.workloads import workload
+    xyz, a, inp, dt, _ = workload("c1", seed=0)
+    box = Box([Bead(p, r) for p, r in zip(xyz, a)], inp)
+    try:
+        for _ in range(warmup):
+            box.propagate(dt)
+        torch.cuda.synchronize()
+        k0 = lib.pbx_kernel_launches()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            box.propagate(dt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return {"workload": WORKLOAD_TEXT["c1"], "steps": steps, "steps_per_s": steps / (t1 - t0),
+                "ms_per_step": (t1 - t0) * 1e3 / steps,
+                "kernels_per_step": (lib.pbx_kernel_launches() - k0) / steps}
+    finally:
+        box.close()
 
 
 def mixed_accuracy(box, xyz, a, inp, dt, steps=3):

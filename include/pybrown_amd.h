@@ -116,6 +116,10 @@ int pbx_set_potrf_chain_jobs(int on);
  * default); every kernel still runs to completion.  No reference
  * counterpart. */
 int pbx_debug_inject_wait_failure(int k);
+/* Kernels this library has launched in this process so far (every device,
+ * every simulation).  Launches per step bound the step rate of small
+ * simulations; no reference counterpart. */
+long long pbx_kernel_launches(void);
 
 /* C = A B^T (M x K times (N x K)^T); M, N, K multiples of 64.  Exposes the
  * FP64 MFMA tile product used by the factorisations, for testing. */

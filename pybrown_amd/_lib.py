@@ -85,6 +85,7 @@ _sig("pbx_set_potrf_variant", _i, [_i])
 _sig("pbx_get_potrf_variant", _i, [])
 _sig("pbx_set_potrf_chain_jobs", _i, [_i])
 _sig("pbx_debug_inject_wait_failure", _i, [_i])
+_sig("pbx_kernel_launches", ctypes.c_longlong, [])
 # simulation
 _sig("pbx_sim_create", _i, [ctypes.POINTER(Config), _i, _dp, _dp, _ip, ctypes.POINTER(ctypes.c_void_p)])
 _sig("pbx_sim_destroy", _i, [ctypes.c_void_p])
@@ -129,6 +130,7 @@ EXPORTED = [
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs",
     "pbx_debug_inject_wait_failure",
+    "pbx_kernel_launches",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
     "pbx_sim_set_attempt_batch", "pbx_sim_set_graphs", "pbx_sim_set_lookahead", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",
     "pbx_sim_restore", "pbx_sim_set_forces", "pbx_sim_get_energy", "pbx_sim_get_force", "pbx_sim_propagate",

@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <atomic>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -74,6 +75,10 @@ extern "C" void pbx_clear_error(void) {
     g_err_msg[0] = 0;
 }
 extern "C" const char* pbx_version(void) { return "pybrown_amd 0.1 gfx950"; }
+
+static std::atomic<long long> g_launches{0};
+void pbx_count_launch() { g_launches.fetch_add(1, std::memory_order_relaxed); }
+extern "C" long long pbx_kernel_launches(void) { return g_launches.load(std::memory_order_relaxed); }
 
 // ---------------------------------------------------------------------------
 // small RAII helpers for the stateless layer

@@ -32,6 +32,16 @@ static inline size_t pbx_round_up(size_t x, size_t m) { return (x + m - 1) / m *
 extern "C" void pbx_set_error(int code, const char* fmt, ...);
 extern "C" void pbx_clear_error(void);
 
+// Every kernel launch of the library goes through PBX_LAUNCH, which also
+// counts it (pbx_kernel_launches): launches per step are what bounds the
+// step rate of small simulations.
+void pbx_count_launch();
+#define PBX_LAUNCH(...)              \
+    do {                             \
+        pbx_count_launch();          \
+        hipLaunchKernelGGL(__VA_ARGS__); \
+    } while (0)
+
 #define PBX_HIP(call)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

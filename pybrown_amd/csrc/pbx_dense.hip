@@ -1076,7 +1076,7 @@ void pbx_status_sequence_queued(int* info_dev, hipStream_t s) {
     int k = g_inject.load();
     while (k > 0 && !g_inject.compare_exchange_weak(k, k - 1)) {
     }
-    if (k == 1) hipLaunchKernelGGL(inject_status_kernel, dim3(1), dim3(64), 0, s, info_dev);
+    if (k == 1) PBX_LAUNCH(inject_status_kernel, dim3(1), dim3(64), 0, s, info_dev);
 }
 
 extern "C" int pbx_debug_inject_wait_failure(int k) {
@@ -1088,14 +1088,14 @@ extern "C" int pbx_debug_inject_wait_failure(int k) {
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s) {
     if (np == n) return PBX_OK;
     size_t total = (size_t)np * np;
-    hipLaunchKernelGGL(fill_pad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, n, np, ld);
+    PBX_LAUNCH(fill_pad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, n, np, ld);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s) {
     int T = np / T64;
-    hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, src, dst, np, ld, nullptr);
+    PBX_LAUNCH(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, src, dst, np, ld, nullptr);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -1115,7 +1115,7 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // workgroups per CU.
     const bool own_cu = T <= g_own_cu_tiles;
     const size_t dyn_lds = own_cu ? 24576 : 0;
-    if (!la_ready) hipLaunchKernelGGL(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
+    if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
     for (int c0 = 0, P = 0; c0 < T;) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
@@ -1133,11 +1133,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
             while ((grid - 1) - (grid - 1) / 8 < W) ++grid;
         }
         if (own_cu)
-            hipLaunchKernelGGL((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
+            PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
                                c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
                                g_chain_jobs);
         else
-            hipLaunchKernelGGL((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
+            PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
                                P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
                                g_chain_jobs);
         P = nc;
@@ -1186,11 +1186,11 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
     }
     if (g_potrf_variant >= 4) {
         int* la = (int*)(work + (size_t)np * T64);
-        hipLaunchKernelGGL(potrf_diag_kernel<5>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, la);
-        if (T > 1) hipLaunchKernelGGL(trsm_kernel, dim3(T - 1), dim3(256), 0, s, A, ld, 0, work);
+        PBX_LAUNCH(potrf_diag_kernel<5>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, la);
+        if (T > 1) PBX_LAUNCH(trsm_kernel, dim3(T - 1), dim3(256), 0, s, A, ld, 0, work);
         for (int k = 0; k + 1 < T; ++k) {
             int cnt = (T - 1 - k) * (T - k) / 2;
-            hipLaunchKernelGGL(syrk_trsm_kernel<5>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev, la);
+            PBX_LAUNCH(syrk_trsm_kernel<5>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev, la);
         }
         pbx_status_sequence_queued(info_dev, s);
         PBX_HIP(hipGetLastError());
@@ -1198,16 +1198,16 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
     }
     const bool v2 = g_potrf_variant == 2;
     if (v2)
-        hipLaunchKernelGGL(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
+        PBX_LAUNCH(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
     else
-        hipLaunchKernelGGL(potrf_diag_kernel<1>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
+        PBX_LAUNCH(potrf_diag_kernel<1>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
     for (int k = 0; k + 1 < T; ++k) {
-        hipLaunchKernelGGL(trsm_kernel, dim3(T - 1 - k), dim3(256), 0, s, A, ld, k, work);
+        PBX_LAUNCH(trsm_kernel, dim3(T - 1 - k), dim3(256), 0, s, A, ld, k, work);
         int cnt = (T - 1 - k) * (T - k) / 2;
         if (v2)
-            hipLaunchKernelGGL(syrk_kernel<2>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
+            PBX_LAUNCH(syrk_kernel<2>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
         else
-            hipLaunchKernelGGL(syrk_kernel<1>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
+            PBX_LAUNCH(syrk_kernel<1>, dim3(cnt), dim3(256), 0, s, A, ld, k, work, info_dev);
     }
     pbx_status_sequence_queued(info_dev, s);
     PBX_HIP(hipGetLastError());
@@ -1226,7 +1226,7 @@ int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int
     const int T = np / T64;
     const bool two_col = g_potrf_variant == 6 || mixed;
     int* la = two_col ? (int*)(work + (size_t)np * T64) : nullptr;
-    hipLaunchKernelGGL(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, np, ld, la);
+    PBX_LAUNCH(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, np, ld, la);
     if (!two_col) return pbx_potrf(B, np, ld, work, info_dev, s, mixed);
     potrf_two_col(B, np, ld, work, info_dev, s, mixed != 0, true);
     pbx_status_sequence_queued(info_dev, s);
@@ -1243,13 +1243,13 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
     const int T = np / T64;
     // every lower tile of X is written below before lauum reads it (the
     // upper tiles hold the recursion's partial products and are not read)
-    hipLaunchKernelGGL(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
+    PBX_LAUNCH(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
     for (int b = 1; b < T; b *= 2) {
         const int npairs = (T - b + 2 * b - 1) / (2 * b);
-        hipLaunchKernelGGL(trtri_level_kernel<0>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
-        hipLaunchKernelGGL(trtri_level_kernel<1>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+        PBX_LAUNCH(trtri_level_kernel<0>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+        PBX_LAUNCH(trtri_level_kernel<1>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
     }
-    hipLaunchKernelGGL(lauum_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+    PBX_LAUNCH(lauum_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -1260,7 +1260,7 @@ int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld,
                         hipStream_t s) {
     const int T = np / T64;
     PBX_HIP(hipMemsetAsync(acc, 0, 2 * sizeof(double), s));
-    hipLaunchKernelGGL(residual_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, ld, n, acc);
+    PBX_LAUNCH(residual_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, ld, n, acc);
     PBX_HIP(hipGetLastError());
     double h[2];
     PBX_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1274,7 +1274,7 @@ int pbx_gemm_nt_test(const double* A, const double* B, double* C, int M, int N, 
         pbx_set_error(PBX_ERR_ARG, "gemm_nt: dimensions must be multiples of 64");
         return PBX_ERR_ARG;
     }
-    hipLaunchKernelGGL(gemm_nt_kernel, dim3(N / 64, M / 64), dim3(256), 0, s, A, B, C, M, N, K);
+    PBX_LAUNCH(gemm_nt_kernel, dim3(N / 64, M / 64), dim3(256), 0, s, A, B, C, M, N, K);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

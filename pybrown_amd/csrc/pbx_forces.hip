@@ -198,8 +198,8 @@ __global__ __launch_bounds__(256) void energy_sum_kernel(const double* e_part, i
 
 int pbx_launch_forces(const PbxForceArgs& a, const double* pos, double* F, double* e_part, double* E,
                       hipStream_t s) {
-    hipLaunchKernelGGL(forces_kernel, dim3((a.N + BEADS - 1) / BEADS), dim3(256), 0, s, a, pos, F, e_part);
-    hipLaunchKernelGGL(energy_sum_kernel, dim3(1), dim3(256), 0, s, e_part, a.N, E);
+    PBX_LAUNCH(forces_kernel, dim3((a.N + BEADS - 1) / BEADS), dim3(256), 0, s, a, pos, F, e_part);
+    PBX_LAUNCH(energy_sum_kernel, dim3(1), dim3(256), 0, s, e_part, a.N, E);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -304,17 +304,17 @@ int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a
     dim3 grid((N + 3) / 4), block(256);
     if (xyz) {
         GeomXYZ g{xyz, L};
-        hipLaunchKernelGGL(lub_rows_kernel<GeomXYZ>, grid, block, 0, s, g, a, N, cutoff, cich, mul, div, C, ld);
+        PBX_LAUNCH(lub_rows_kernel<GeomXYZ>, grid, block, 0, s, g, a, N, cutoff, cich, mul, div, C, ld);
     } else {
         GeomPacked g{P, N};
-        hipLaunchKernelGGL(lub_rows_kernel<GeomPacked>, grid, block, 0, s, g, a, N, cutoff, cich, mul, div, C, ld);
+        PBX_LAUNCH(lub_rows_kernel<GeomPacked>, grid, block, 0, s, g, a, N, cutoff, cich, mul, div, C, ld);
     }
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_jo_2b(double ai, double aj, double rx, double ry, double rz, double* out18, hipStream_t s) {
-    hipLaunchKernelGGL(jo_2b_kernel, dim3(1), dim3(64), 0, s, ai, aj, rx, ry, rz, out18);
+    PBX_LAUNCH(jo_2b_kernel, dim3(1), dim3(64), 0, s, ai, aj, rx, ry, rz, out18);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

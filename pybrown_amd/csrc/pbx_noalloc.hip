@@ -110,11 +110,11 @@ extern "C" int pbx_cholesky_noalloc(const double* D_dev, int n, int ld_D, double
     int* la = (int*)(work + (size_t)np * 64);
     int* status = (int*)(work + kWorkDoubles);
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(noalloc_stage_kernel, dim3(T, T), dim3(256), 0, s, D_dev, n, ld_D, B_dev, np, ld_B, la, status);
+    PBX_LAUNCH(noalloc_stage_kernel, dim3(T, T), dim3(256), 0, s, D_dev, n, ld_D, B_dev, np, ld_B, la, status);
     PBX_HIP(hipGetLastError());
     rc = pbx_potrf_two_col_ready(B_dev, np, ld_B, work, status, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(noalloc_zero_upper_kernel, dim3(T, T), dim3(256), 0, s, B_dev, ld_B);
+    PBX_LAUNCH(noalloc_zero_upper_kernel, dim3(T, T), dim3(256), 0, s, B_dev, ld_B);
     PBX_HIP(hipGetLastError());
     int info = 0;
     PBX_HIP(hipMemcpyAsync(&info, status, sizeof(int), hipMemcpyDeviceToHost, s));

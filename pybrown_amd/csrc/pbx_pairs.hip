@@ -425,7 +425,7 @@ void launch_smith(Geom g, const double* a, double L, double alpha, int m_max, in
                   int ld, int tiles, hipStream_t s) {
 #define PBX_SMITH_CASE(MMV, NMV)                                                                              \
     if (m_max == MMV && n_max == NMV) {                                                                     \
-        hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, Geom, MMV, NMV>), dim3(tiles), dim3(256), 0, s, g, a, \
+        PBX_LAUNCH((assemble_tile_kernel<K_SMITH, Geom, MMV, NMV>), dim3(tiles), dim3(256), 0, s, g, a, \
                            L, alpha, m_max, n_max, scale, M, ld);                                           \
         return;                                                                                              \
     }
@@ -436,7 +436,7 @@ void launch_smith(Geom g, const double* a, double L, double alpha, int m_max, in
     PBX_SMITH_ROW(3)
 #undef PBX_SMITH_ROW
 #undef PBX_SMITH_CASE
-    hipLaunchKernelGGL((assemble_tile_kernel<K_SMITH, Geom, -1, -1>), dim3(tiles), dim3(256), 0, s, g, a, L, alpha,
+    PBX_LAUNCH((assemble_tile_kernel<K_SMITH, Geom, -1, -1>), dim3(tiles), dim3(256), 0, s, g, a, L, alpha,
                        m_max, n_max, scale, M, ld);
 }
 
@@ -455,7 +455,7 @@ int pbx_launch_rpy(const double* xyz, const double* a, int N, double L, double s
     if (N <= 0) return PBX_OK;
     PBX_CHECK_BOX(L);
     GeomXYZ g{xyz, L, N};
-    hipLaunchKernelGGL((assemble_tile_kernel<K_RPY, GeomXYZ>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, L, 0.0, 0,
+    PBX_LAUNCH((assemble_tile_kernel<K_RPY, GeomXYZ>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, L, 0.0, 0,
                        0, scale, M, ld);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
@@ -478,7 +478,7 @@ int pbx_launch_smith(const double* xyz, const double* a, int N, double L, double
 int pbx_launch_rpy_packed(const double* a, const double* P, int N, double* M, int ld, hipStream_t s) {
     if (N <= 0) return PBX_OK;
     GeomPacked g{P, N};
-    hipLaunchKernelGGL((assemble_tile_kernel<K_RPY, GeomPacked>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, 0.0,
+    PBX_LAUNCH((assemble_tile_kernel<K_RPY, GeomPacked>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, 0.0,
                        0.0, 0, 0, 1.0, M, ld);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
@@ -502,7 +502,7 @@ int pbx_launch_pointers(const double* xyz, int N, double L, double* P3, hipStrea
     if (N <= 1) return PBX_OK;
     PBX_CHECK_BOX(L);
     size_t total = (size_t)N * N;
-    hipLaunchKernelGGL(pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xyz, N, L, P3);
+    PBX_LAUNCH(pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xyz, N, L, P3);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -512,7 +512,7 @@ int pbx_launch_immobile_pointers(const double* xm, const double* xi, int nm, int
     if (nm <= 0 || ni <= 0) return PBX_OK;
     PBX_CHECK_BOX(L);
     size_t total = (size_t)nm * ni;
-    hipLaunchKernelGGL(immobile_pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xm, xi, nm, ni,
+    PBX_LAUNCH(immobile_pointers_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, xm, xi, nm, ni,
                        L, P3);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
@@ -526,7 +526,7 @@ int pbx_launch_overlaps(const double* xyz, const double* a, int Nt, int Nm, doub
         return PBX_ERR_ARG;
     }
     PBX_CHECK_BOX(L);
-    hipLaunchKernelGGL(overlaps_kernel, dim3(pbx_pair_tiles(Nm, Nt)), dim3(256), 0, s, xyz, a, Nt, Nm, L, thr, mask,
+    PBX_LAUNCH(overlaps_kernel, dim3(pbx_pair_tiles(Nm, Nt)), dim3(256), 0, s, xyz, a, Nt, Nm, L, thr, mask,
                        flag);
     PBX_HIP(hipGetLastError());
     return PBX_OK;

@@ -506,10 +506,10 @@ int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, d
     }
     f.on = flux_on;
     if (diag_only)
-        hipLaunchKernelGGL(stoch_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, B, xi, sq2dt, mult, pos, disp,
+        PBX_LAUNCH(stoch_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, B, xi, sq2dt, mult, pos, disp,
                            N, f, label, flux);
     else
-        hipLaunchKernelGGL(stoch_kernel, dim3(N), dim3(192), 0, s, B, ld, xi, sq2dt, mult, pos, disp, f, label,
+        PBX_LAUNCH(stoch_kernel, dim3(N), dim3(192), 0, s, B, ld, xi, sq2dt, mult, pos, disp, f, label,
                            flux);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
@@ -525,10 +525,10 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
     }
     f.on = flux_on;
     if (diag_only)
-        hipLaunchKernelGGL(det_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, F, k, mult, N, pos, f, label,
+        PBX_LAUNCH(det_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, F, k, mult, N, pos, f, label,
                            flux);
     else
-        hipLaunchKernelGGL(det_kernel, dim3(N), dim3(192), 0, s, D, ld, F, k, mult, 3 * N, pos, f, label, flux);
+        PBX_LAUNCH(det_kernel, dim3(N), dim3(192), 0, s, D, ld, F, k, mult, 3 * N, pos, f, label, flux);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -562,25 +562,25 @@ __global__ void div_finish_kernel(const double* div, int n, double eps, double d
 }
 
 int pbx_launch_div_shift(double* pos, int idx, double delta, hipStream_t s) {
-    hipLaunchKernelGGL(div_shift_kernel, dim3(1), dim3(64), 0, s, pos, idx, delta);
+    PBX_LAUNCH(div_shift_kernel, dim3(1), dim3(64), 0, s, pos, idx, delta);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_div_init(const double* D, int n, int ld, double* div, hipStream_t s) {
-    hipLaunchKernelGGL(div_init_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, n, ld, div);
+    PBX_LAUNCH(div_init_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, n, ld, div);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_div_add_row(const double* D, int row, int n, int ld, double* div, hipStream_t s) {
-    hipLaunchKernelGGL(div_add_row_kernel, dim3((n + 255) / 256), dim3(256), 0, s, D, row, n, ld, div);
+    PBX_LAUNCH(div_add_row_kernel, dim3((n + 255) / 256), dim3(256), 0, s, D, row, n, ld, div);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_div_finish(const double* div, int n, double eps, double dt, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(div_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, div, n, eps, dt, out);
+    PBX_LAUNCH(div_finish_kernel, dim3((n + 255) / 256), dim3(256), 0, s, div, n, eps, dt, out);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -593,7 +593,7 @@ int pbx_launch_translate(const double* v, double sign, double* pos, int N, const
         f.p[c] = plane_p ? plane_p[c] : 0.0;
     }
     f.on = flux_on;
-    hipLaunchKernelGGL(translate_kernel, dim3((N + 255) / 256), dim3(256), 0, s, v, sign, N, pos, f, label, flux);
+    PBX_LAUNCH(translate_kernel, dim3((N + 255) / 256), dim3(256), 0, s, v, sign, N, pos, f, label, flux);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -601,27 +601,27 @@ int pbx_launch_translate(const double* v, double sign, double* pos, int N, const
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s) {
     PBX_CHECK_BOX(L);
     int n3 = 3 * N;
-    hipLaunchKernelGGL(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L);
+    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_lower_gemv(const double* Lm, int n, int ld, const double* x, double* y, double scale, hipStream_t s) {
-    hipLaunchKernelGGL(lower_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, Lm, n, ld, x, y, scale);
+    PBX_LAUNCH(lower_gemv_kernel, dim3((n + 3) / 4), dim3(256), 0, s, Lm, n, ld, x, y, scale);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const double* x, const double* z, double alpha,
                      double* y, hipStream_t s) {
-    hipLaunchKernelGGL(drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, Lo, n, ld, x, z, alpha, y);
+    PBX_LAUNCH(drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, Lo, n, ld, x, z, alpha, y);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
                          hipStream_t s) {
-    hipLaunchKernelGGL(det_drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D1, D0, n, ld, F, beta, y);
+    PBX_LAUNCH(det_drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D1, D0, n, ld, F, beta, y);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -634,7 +634,7 @@ int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const 
         pbx_set_error(PBX_ERR_ARG, "trsv_lt: np=%d epoch=%d", np, epoch);
         return PBX_ERR_ARG;
     }
-    hipLaunchKernelGGL(trsv_lt_rows_kernel, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, flags, epoch, info);
+    PBX_LAUNCH(trsv_lt_rows_kernel, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, flags, epoch, info);
     pbx_status_sequence_queued(info, s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
@@ -679,19 +679,19 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
         const int T = np / BT;
         dim3 grid(T, (T + S - 1) / S);
         if (KC == 16)
-            hipLaunchKernelGGL(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
+            PBX_LAUNCH(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
                                K, S, P);
         else
-            hipLaunchKernelGGL(stoch_batch_kernel<32>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
+            PBX_LAUNCH(stoch_batch_kernel<32>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
                                K, S, P);
         PBX_HIP(hipGetLastError());
     }
     double* delta = cand + (size_t)(K + 1) * n;
-    hipLaunchKernelGGL(cand_kernel, dim3((N + CB - 1) / CB), dim3(32 * CB), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
+    PBX_LAUNCH(cand_kernel, dim3((N + CB - 1) / CB), dim3(32 * CB), 0, s, pos, P, np, KC, S, N, K, sq2dt, Bd, ring,
                        ring_pos, ring_att, cand, delta, cflag);
     PBX_HIP(hipGetLastError());
     if (Nt > 1) {
-        hipLaunchKernelGGL(overlaps_batch_kernel, dim3(pbx_pair_tiles(N, Nt)), dim3(256), 0, s, cand, pos, delta, a, N,
+        PBX_LAUNCH(overlaps_batch_kernel, dim3(pbx_pair_tiles(N, Nt)), dim3(256), 0, s, cand, pos, delta, a, N,
                            Nt, K, L, thr, mask, cflag);
         PBX_HIP(hipGetLastError());
     }
@@ -707,7 +707,7 @@ int pbx_launch_attempt_select(const double* cand, int N, int K, const int* cflag
         f.p[c] = plane_p ? plane_p[c] : 0.0;
     }
     f.on = flux_on;
-    hipLaunchKernelGGL(select_kernel, dim3((N + 255) / 256), dim3(256), 0, s, cand, N, K, cflag, pos, f, label, flux,
+    PBX_LAUNCH(select_kernel, dim3((N + 255) / 256), dim3(256), 0, s, cand, N, K, cflag, pos, f, label, flux,
                        sel);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
