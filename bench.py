@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--request-steps", type=int, default=200, help="c5: steps per request")
     ap.add_argument("--graphs", action="store_true",
                     help="c5: replay the factorisation and SPD-inverse launch sequences from HIP graphs")
+    ap.add_argument("--no-c1-leg", action="store_true", help="c2: skip the small C1 leg")
+    ap.add_argument("--no-served-leg", action="store_true", help="c2: skip the served request-stream sample")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -111,6 +113,9 @@ def main():
     if args.workload == "c5":
         serve_bench(args)
         return
+    # the served-sample leg runs 16 simulations at once: one hardware queue
+    # each (read when the HIP runtime starts; one stream alone uses one)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
 
     import ctypes
 
@@ -183,8 +188,10 @@ def main():
         "kernels_per_step": kernels_per_step,
         "roofline": roofline(args.workload, N, stage, args.precision),
     }
-    if args.workload == "c2":
+    if args.workload == "c2" and not args.no_c1_leg:
         out["c1_leg"] = c1_leg()
+    if args.workload == "c2" and not args.no_served_leg:
+        out["served_sample"] = served_legs(64, 100, 16, log=lambda m: print(m, file=sys.stderr))
     if accuracy is not None:
         out["accuracy"] = accuracy
     if not args.no_cpu:
@@ -248,52 +255,83 @@ def mixed_accuracy(box, xyz, a, inp, dt, steps=3):
         mix.close()
 
 
+def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None):
+    """The served request stream on the first `requests` requests (request
+    k: seed k; pybrown_amd.serving), `steps` steps each:
+      plain   one at a time on the plain FP64 path (Box.propagate per step,
+              one synchronised step at a time, as BD.py drives a Box);
+      run     one at a time, all steps inside the library (Box.run);
+      burst   served on `slots` concurrent simulations, all arrived at the
+              start (throughput X);
+      poisson served with Poisson arrivals at 1.5 X (the reported sims/s and
+              p50/p99 time to trajectory).
+    Served results are checked against the one-at-a-time ones."""
+    import functools
+
+    import numpy as np
+
+    from pybrown_amd.serving import Server, make_requests_parallel, run_one, run_plain, stream_params, summary
+    log = log or (lambda *a: None)
+    params = stream_params(requests)
+    t = time.perf_counter()
+    reqs = make_requests_parallel(params, steps=steps, workers=min(16, os.cpu_count() or 1))
+    log("c5: %d requests placed in %.1f s (N %d..%d, %s)" % (
+        len(reqs), time.perf_counter() - t, min(r.N for r in reqs), max(r.N for r in reqs),
+        {m: sum(r.mode == m for r in reqs) for m in ("rpy_smith", "rpy", "rpy_smith_lub")}))
+    budget = 250e9
+    runner = functools.partial(run_one, graphs=graphs, precision=precision)
+    small = sorted(reqs, key=lambda r: r.N)[:2]
+    Server(slots=2, hbm_budget=budget, runner=runner).run(small, realtime=False)  # code objects, allocator
+    plain = Server(slots=1, hbm_budget=budget, runner=run_plain).run(reqs, realtime=False)
+    log("c5 plain one at a time: %s" % summary(plain))
+    one = Server(slots=1, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
+    log("c5 run one at a time: %s" % summary(one))
+    burst_res = Server(slots=slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
+    burst = summary(burst_res)
+    log("c5 served, all at start: %s" % burst)
+    rate = 1.5 * burst["sims_per_s"]
+    for r, p in zip(reqs, stream_params(requests, rate=rate)):
+        r.arrival = p["arrival"]
+    res = Server(slots=slots, hbm_budget=budget, runner=runner).run(reqs, realtime=True)
+    served = summary(res)
+    log("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served))
+    failed = [(r.rid, r.N, r.mode, r.status) for r in res + burst_res if r.status != "ok"]
+    if failed:
+        log("c5 failed requests: %s" % failed)
+    # every served result against the one-at-a-time one (bit for bit in FP64)
+    same = all(a.status == b.status == c.status == "ok" and np.array_equal(a.xyz, b.xyz) and
+               np.array_equal(a.xyz, c.xyz) and np.array_equal(a.flux, b.flux) and np.array_equal(a.flux, c.flux)
+               for a, b, c in zip(one, burst_res, res))
+    plain_same = all(a.status == p_.status == "ok" and np.array_equal(a.xyz, p_.xyz) and np.array_equal(a.flux, p_.flux)
+                     for a, p_ in zip(one, plain))
+    base = summary(plain)
+    return {
+        "requests": requests, "steps_per_request": steps, "slots": slots, "precision": precision,
+        "sims_per_s": served["sims_per_s"], "p50_s": served["p50_s"], "p99_s": served["p99_s"],
+        "completed": served["completed"], "failed": served["failed"], "arrival_rate_per_s": rate,
+        "plain_one_at_a_time_sims_per_s": base["sims_per_s"],
+        "run_one_at_a_time_sims_per_s": summary(one)["sims_per_s"],
+        "burst_sims_per_s": burst["sims_per_s"],
+        "speedup_vs_plain_one_at_a_time": served["sims_per_s"] / base["sims_per_s"],
+        "served_equal_to_one_at_a_time": bool(same),
+        "run_equal_to_plain": bool(plain_same),
+        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
+    }
+
+
 def serve_bench(args):
-    """Served request stream (c5): the first --requests requests of the
-    stream (pybrown_amd.serving), run (1) one at a time, (2) served with all
-    arrived at the start (throughput X) and (3) served with Poisson arrivals
-    at rate 1.5 X, which gives the reported sims/s and p50/p99 time to
-    trajectory."""
+    """--workload c5: the served request stream (served_legs) as the bench
+    line."""
     # hardware queues the HIP runtime spreads the simulations' streams over;
     # read once when the runtime starts, so set before anything touches it
     os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or str(min(max(args.slots, 4), 32))
-    import functools
-    from pybrown_amd.serving import Server, make_requests_parallel, run_one, stream_params, summary
-    params = stream_params(args.requests)
-    t = time.perf_counter()
-    reqs = make_requests_parallel(params, steps=args.request_steps, workers=min(16, os.cpu_count() or 1))
-    print("c5: %d requests placed in %.1f s (N %d..%d, %s)" % (
-        len(reqs), time.perf_counter() - t, min(r.N for r in reqs), max(r.N for r in reqs),
-        {m: sum(r.mode == m for r in reqs) for m in ("rpy_smith", "rpy", "rpy_smith_lub")}), file=sys.stderr)
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    budget = 250e9
-    runner = functools.partial(run_one, graphs=args.graphs)
-    small = sorted(reqs, key=lambda r: r.N)[:2]
-    Server(slots=2, hbm_budget=budget, runner=runner).run(small, realtime=False)  # warm-up: code objects, allocator
-    one = Server(slots=1, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
-    base = summary(one)
-    print("c5 one at a time: %s" % base, file=sys.stderr)
-    for r in sorted(one, key=lambda r: r.start - r.end)[:8]:
-        print("   request %4d  N %5d  %-14s service %.3f s" % (r.rid, r.N, r.mode, r.end - r.start), file=sys.stderr)
-    burst_res = Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
-    burst = summary(burst_res)
-    print("c5 served, all at start: %s" % burst, file=sys.stderr)
-    for r in burst_res:
-        if r.status != "ok":
-            print("c5 burst: request %d (N %d, %s) failed: %s" % (r.rid, r.N, r.mode, r.status), file=sys.stderr)
-    rate = 1.5 * burst["sims_per_s"]
-    for r, p in zip(reqs, stream_params(args.requests, rate=rate)):
-        r.arrival = p["arrival"]
-    res = Server(slots=args.slots, hbm_budget=budget, runner=runner).run(reqs, realtime=True)
-    served = summary(res)
-    print("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served), file=sys.stderr)
-    failed = [(r.rid, r.N, r.mode, r.status) for r in res if r.status != "ok"]
-    if failed:
-        print("c5 failed requests: %s" % failed, file=sys.stderr)
+    leg = served_legs(args.requests, args.request_steps, args.slots, args.precision, args.graphs,
+                      log=lambda m: print(m, file=sys.stderr))
     out = {
         "metric": "served sims/s + p99 latency (request stream)",
-        "value": served["sims_per_s"],
+        "value": leg["sims_per_s"],
         "unit": "sims/s",
         "n_gpus": 1,
         "steps": args.request_steps,
@@ -302,22 +340,14 @@ def serve_bench(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 + f32 (trailing Cholesky updates)",
         "data": "synthetic",
         "config": {"workload": "request stream: first %d requests, N log-uniform 32..3000, phi 0.02..0.30, "
                                "60%% rpy_smith / 25%% rpy (open box) ermak / 15%% rpy_smith_lub midpoint, dt=1ps, "
                                "%d steps each, flux x=0, Poisson arrivals at 1.5x the burst throughput"
                                % (args.requests, args.request_steps),
-                   "requests": args.requests, "slots": args.slots, "hbm_budget_bytes": budget,
-                   "arrival_rate_per_s": rate, "gpu_max_hw_queues": os.environ["GPU_MAX_HW_QUEUES"],
-                   "graphs": args.graphs},
-        "completed": served["completed"],
-        "failed": served["failed"],
-        "p50_s": served["p50_s"],
-        "p99_s": served["p99_s"],
-        "one_at_a_time_sims_per_s": base["sims_per_s"],
-        "burst_sims_per_s": burst["sims_per_s"],
-        "speedup_vs_one_at_a_time": served["sims_per_s"] / base["sims_per_s"],
+                   "slots": args.slots, "hbm_budget_bytes": 250e9, "graphs": args.graphs},
+        "served": leg,
         "roofline": None,
     }
     print(json.dumps(out))

@@ -436,6 +436,9 @@ struct PbxSim {
     int next_builds = -1;  // whether the next step builds D: 1, 0, or -1 unknown (then assumed)
     hipEvent_t done_ev = nullptr;
     double stage_ms[3] = {0, 0, 0};  // assembly (+lubrication), factorisation, propagation
+    // queued runs (pbx_sim_run without host decisions): per-step flux slots
+    long long* flux_log = nullptr;
+    long long flux_log_cap = 0;  // in counters
     long long timed_steps = 0;
 };
 
@@ -710,23 +713,25 @@ struct StepCtx {
     const double* pn;
     const double* pp;
     int flux_on;
+    long long* flux;  // per-label counters the step's moves add their crossings to
+    int queued;       // steps queued back to back: no per-step status copies
 };
 
 int stoch(PbxSim* s, const double* Bm, const double* xi, double mult, StepCtx& c) {
     int diag = s->cfg.hydrodynamics == PBX_NOHI;
     return pbx_launch_stoch(diag ? s->Bd : Bm, s->ld, diag, xi, c.sq2dt, mult, s->pos, s->disp, s->N, c.pn, c.pp,
-                            c.flux_on, s->label, s->flux, s->stream);
+                            c.flux_on, s->label, c.flux, s->stream);
 }
 
 int det(PbxSim* s, const double* Dm, double dt, double mult, StepCtx& c) {
     if (!s->F_nonzero) return PBX_OK;  // translating by D.0 leaves positions unchanged
     int diag = s->cfg.hydrodynamics == PBX_NOHI;
     return pbx_launch_det(diag ? s->Dd : Dm, s->ld, diag, s->F, dt / s->kBT, mult, s->pos, s->N, c.pn, c.pp,
-                          c.flux_on, s->label, s->flux, s->stream);
+                          c.flux_on, s->label, c.flux, s->stream);
 }
 
 int undo(PbxSim* s, StepCtx& c) {
-    return pbx_launch_translate(s->disp, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
+    return pbx_launch_translate(s->disp, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream);
 }
 
 // box.py:560-610 (applied in box.py:211-215): with lubrication, the
@@ -751,7 +756,7 @@ int divergence_step(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         TRY(pbx_launch_div_shift(s->pos, k, -eps, s->stream));
     }
     TRY(pbx_launch_div_finish(s->divv, s->n, eps, dt, s->drift, s->stream));
-    return pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream);
+    return pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream);
 }
 
 // box.py:202-250.  With overlap checks the attempts run in batches of K:
@@ -763,7 +768,7 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     if (s->cfg.divergence_term && is_lub(s)) {
         TRY(divergence_step(s, dt, bDff, bDnf, c));
         // the 3N rebuilds factorised after the step's status copy was queued
-        TRY(post_info(s));
+        if (!c.queued) TRY(post_info(s));
     }
     if (!s->cfg.check_overlaps) {
         const double* xi;
@@ -792,7 +797,7 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
                                      s->cflag, s->a, s->cfg.box_length, s->cfg.overlap_treshold, s->ovl_mask,
                                      s->stream));
         TRY(pbx_launch_attempt_select(s->cand, s->N, (int)K, s->cflag, s->pos, c.pn, c.pp, c.flux_on, s->label,
-                                      s->flux, s->flag, s->stream));
+                                      c.flux, s->flag, s->stream));
         PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
         PBX_HIP(hipStreamSynchronize(s->stream));
         TRY(take_info(s));
@@ -853,20 +858,20 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
                                s->info, s->stream));
         // status of the midpoint inverses and of the solve's bounded waits,
         // read at this attempt's overlap check or at the step's end
-        TRY(post_info(s));
+        if (!c.queued) TRY(post_info(s));
         TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
         if (s->F_nonzero)
             TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));
         TRY(det(s, s->D, dt, 1.0 - 1.0 / m, c));
         TRY(stoch(s, s->B, xi, 1.0 - 1.0 / m, c));
-        TRY(pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream));
+        TRY(pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream));
         if (!cf.check_overlaps) break;
         int hit = 0;
         TRY(overlap_now(s, &hit));
         if (!hit) break;
         TRY(det(s, s->D, dt, -(1.0 - 1.0 / m), c));
         TRY(stoch(s, s->B, xi, -1.0, c));
-        TRY(pbx_launch_translate(s->drift, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, s->flux, s->stream));
+        TRY(pbx_launch_translate(s->drift, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream));
         if (attempts >= cf.max_move_attempts) {
             s->last_attempts = attempts;
             pbx_set_error(PBX_ERR_MOVES, "maximal number of move attepts exceeded");
@@ -1328,6 +1333,8 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     c.pn = cf.flux_normal;
     c.pp = cf.flux_plane_point;
     c.flux_on = cf.measure_flux;
+    c.flux = s->flux;
+    c.queued = 0;
     if (cf.measure_flux) {
         int nl = cf.n_labels > 0 ? cf.n_labels : 1;
         PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
@@ -1415,6 +1422,113 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     return rc;
 }
 
+namespace {
+
+// One Box.propagate queued on the stream with no host synchronisation: the
+// matrix builds and factorisation its flags ask for, then the Ermak or
+// midpoint move with no overlap test, then the wrap.  Crossings add into
+// flux_slot (zero beforehand).  The same kernels as pbx_sim_propagate with
+// check_overlaps off, so the same bits.
+int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* flux_slot) {
+    const pbx_config& cf = s->cfg;
+    StepCtx c;
+    c.sq2dt = sqrt(2 * dt);
+    c.pn = cf.flux_normal;
+    c.pp = cf.flux_plane_point;
+    c.flux_on = cf.measure_flux;
+    c.flux = flux_slot;
+    c.queued = 1;
+    if (cf.hydrodynamics != PBX_NOHI) {
+        if (bDff) {
+            TRY(build_Dff(s, s->D, s->Rff));
+            if (is_lub(s)) s->have_Rff = 1; else s->have_D = 1;
+        }
+        if (is_lub(s) && bDnf) {
+            if (!s->have_Rff) {
+                pbx_set_error(PBX_ERR_STATE, "lubrication step before the far-field matrix was built");
+                return PBX_ERR_STATE;
+            }
+            TRY(build_Dtot(s, s->Rff, s->D));
+            s->have_D = 1;
+        }
+        if (chol) {
+            if (!s->have_D) {
+                pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
+                return PBX_ERR_STATE;
+            }
+            TRY(factor(s));
+            s->have_B = 1;
+        }
+        if (!s->have_B) {
+            pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
+            return PBX_ERR_STATE;
+        }
+    }
+    if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
+    if (cf.scheme == PBX_ERMAK)
+        TRY(ermak(s, dt, bDff, bDnf, c));
+    else
+        TRY(midpoint(s, dt, bDff, bDnf, c));
+    return pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream);
+}
+
+// Steps j0 .. j0+steps-1 queued back to back (pbx_sim_run when no step
+// needs a host decision): one flux slot per step, zeroed once; the status
+// word is copied every 32 steps and read one copy later, so the host stays up
+// to 64 steps ahead of the device and a failure is reported within 64 steps;
+// positions, flux and energy come back once at the end.
+int run_queued(PbxSim* s, double dt, long long j0, long long steps, int diff_freq, int lub_freq, int chol_freq,
+               long long* flux_out) {
+    const int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    const long long need = steps * nl;
+    if (need > s->flux_log_cap) {
+        if (s->flux_log) {
+            PBX_HIP(hipStreamSynchronize(s->stream));
+            PBX_HIP(hipFree(s->flux_log));
+            s->owned.erase(std::find(s->owned.begin(), s->owned.end(), (void*)s->flux_log));
+            s->device_bytes -= sizeof(long long) * (size_t)s->flux_log_cap;
+            s->flux_log = nullptr;
+        }
+        TRY(sim_alloc(s, (void**)&s->flux_log, sizeof(long long) * (size_t)need));
+        s->flux_log_cap = need;
+    }
+    PBX_HIP(hipMemsetAsync(s->flux_log, 0, sizeof(long long) * (size_t)need, s->stream));
+    s->spec_ok = 0;  // D is assembled at the start of each step here
+    PBX_HIP(hipEventRecord(s->done_ev, s->stream));
+    bool posted = false;
+    for (long long t = 0; t < steps; ++t) {
+        const long long j = j0 + t;
+        TRY(queued_step(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0, s->flux_log + t * nl));
+        if ((t + 1) % 32 == 0) {
+            if (posted) {  // the copy queued 32 steps ago: read it
+                PBX_HIP(hipEventSynchronize(s->done_ev));
+                TRY(take_info(s));
+            }
+            TRY(post_info(s));
+            PBX_HIP(hipEventRecord(s->done_ev, s->stream));
+            posted = true;
+        }
+    }
+    TRY(post_info(s));
+    PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+    if (s->forces_on) PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    if (steps > 0) {  // the last step's counters, as pbx_sim_get_flux and snapshots read them
+        PBX_HIP(hipMemcpyAsync(s->flux, s->flux_log + (steps - 1) * nl, sizeof(long long) * nl,
+                               hipMemcpyDeviceToDevice, s->stream));
+        PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux_log + (steps - 1) * nl, sizeof(long long) * nl,
+                               hipMemcpyDeviceToHost, s->stream));
+    }
+    if (flux_out && s->cfg.measure_flux)
+        PBX_HIP(hipMemcpyAsync(flux_out, s->flux_log, sizeof(long long) * (size_t)need, hipMemcpyDeviceToHost,
+                               s->stream));
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    if (flux_out && !s->cfg.measure_flux) memset(flux_out, 0, sizeof(long long) * (size_t)need);
+    s->last_attempts = 1;
+    return take_info(s);
+}
+
+}  // namespace
+
 // Box.propagate plus the read-back BD.py needs after every step, in the
 // step's final synchronisation: positions (N x 3) and, when flux is
 // measured, the per-label counters (either pointer may be NULL).
@@ -1444,6 +1558,10 @@ extern "C" int pbx_sim_run(pbx_sim* s, double dt, long long j0, long long steps,
         return PBX_ERR_ARG;
     }
     const int nl = s->cfg.n_labels > 0 ? s->cfg.n_labels : 1;
+    // no step needs a host decision (no overlap test, no divergence term, no
+    // frames to hand back): queue them all
+    if (!xyz_out && !s->cfg.check_overlaps && !(s->cfg.divergence_term && is_lub(s)))
+        return run_queued(s, dt, j0, steps, diff_freq, lub_freq, chol_freq, flux_out);
     double* frame = xyz_out;
     for (long long t = 0; t < steps; ++t) {
         const long long j = j0 + t;

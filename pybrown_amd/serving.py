@@ -182,18 +182,39 @@ class Result:
         return None if self.end is None else self.end - self.arrival
 
 
-def run_one(req, graphs=False):
-    """Create the simulation on the device, run all steps, free it.
-    Returns (per-step flux (steps, labels), final positions (N, 3), device
-    bytes held).  graphs: replay the factorisation and SPD-inverse launch
-    sequences from HIP graphs (Box(..., graphs=True))."""
+def run_one(req, graphs=False, precision="fp64"):
+    """Create the simulation on the device, run all steps inside the library
+    (Box.run: steps that need no host decision are queued back to back), free
+    it.  Returns (per-step flux (steps, labels), final positions (N, 3),
+    device bytes held).  graphs: replay the factorisation and SPD-inverse
+    launch sequences from HIP graphs; precision "mixed": the noise factor's
+    trailing updates on FP32 matrix cores."""
     from pybrown_amd.bead import Bead
     from pybrown_amd.box import Box
     beads = [Bead(req.xyz[i].copy(), float(req.radii[i]), label="XXX") for i in range(req.N)]
-    box = Box(beads, req.inp, graphs=graphs)
+    box = Box(beads, req.inp, graphs=graphs, precision=precision)
     try:
         held = box.device_bytes
         flux, _ = box.run(req.dt, req.steps)
+        return flux, box._pos.copy(), held
+    finally:
+        box.close()
+
+
+def run_plain(req):
+    """The plain one-at-a-time FP64 path: the simulation stepped from Python
+    with Box.propagate, one synchronised step at a time, as BD.py:164-185
+    drives the reference's Box.  Same results as run_one."""
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    beads = [Bead(req.xyz[i].copy(), float(req.radii[i]), label="XXX") for i in range(req.N)]
+    box = Box(beads, req.inp)
+    try:
+        held = box.device_bytes
+        flux = np.zeros((req.steps, 1), dtype=np.int64)
+        for j in range(req.steps):
+            box.propagate(req.dt)
+            flux[j, 0] = box.net_flux["XXX"]
         return flux, box._pos.copy(), held
     finally:
         box.close()
