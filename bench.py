@@ -94,8 +94,11 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--potrf", type=int, default=0, help="Cholesky variant (1, 2, 4, 6; 0 = library default)")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="noise factor: FP64, or FP32 trailing updates with FP64 diagonal tiles")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed_refined", "mixed_factor"],
+                    help="dense algebra: FP64; mixed = FP32 matrix-core bulk of the Cholesky and of the SPD "
+                         "inverses (FP64 diagonal tiles); mixed_refined = mixed plus one Newton-Schulz step per "
+                         "inverse; mixed_factor = the Cholesky's bulk only (c5: the served mode, lubrication "
+                         "requests as mixed_factor)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (HBM counter calibration)")
     ap.add_argument("--requests", type=int, default=64, help="c5: requests taken from the head of the stream")
@@ -166,9 +169,9 @@ def main():
           % (stage[0], stage[1], stage[2], nt, ms), file=sys.stderr)
 
     accuracy = None
-    if args.precision == "mixed":
-        accuracy = mixed_accuracy(box, xyz, a, inp, dt)
-        print("mixed factor: %s" % accuracy, file=sys.stderr)
+    if args.precision != "fp64":
+        accuracy = mixed_accuracy(box, xyz, a, inp, dt, precision=args.precision)
+        print("mixed precision errors: %s" % accuracy, file=sys.stderr)
     out = {
         "metric": METRIC[args.workload],
         "value": 1e3 / ms,
@@ -180,7 +183,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 + f32 (trailing Cholesky updates)",
+        "dtype": DTYPE_TEXT[args.precision],
         "data": "synthetic",
         "config": {"workload": WORKLOAD_TEXT[args.workload], "N": N, "box_length": round(inp["box_length"], 4),
                    "precision": args.precision},
@@ -230,42 +233,67 @@ def c1_leg(warmup=20, steps=500):
         box.close()
 
 
-def mixed_accuracy(box, xyz, a, inp, dt, steps=3):
-    """Errors of the mixed-precision engine against an FP64 twin: the
-    residual ||B B^T - D||_F / ||D||_F of both factors (on the device) and the
-    relative error of one step's displacement from identical positions and
-    noise, the largest over `steps` steps."""
+def mixed_accuracy(box, xyz, a, inp, dt, steps=3, precision="mixed"):
+    """Errors of the mixed-precision engine against an FP64 twin stepped from
+    the same positions with the same noise, each the largest over `steps`
+    steps: the noise covariance residual ||B B^T - D||_F / ||D||_F of both
+    factors (on the device); the max-norm relative error of the diffusion
+    tensor D and (lubrication) of the far-field resistance Rff = inv(Mff);
+    the step displacement (2-norm relative); and the per-step diffusion
+    coefficient <|d|^2> / (6 dt).  The mobility is assembled in FP64 on
+    both paths (identical)."""
     from pybrown_amd.bead import Bead
     from pybrown_amd.box import Box
     ref = Box([Bead(p, r) for p, r in zip(box._pos.copy(), a)], dict(inp, seed=inp["seed"] + 1))
-    mix = Box([Bead(p, r) for p, r in zip(box._pos.copy(), a)], dict(inp, seed=inp["seed"] + 1), precision="mixed")
+    mix = Box([Bead(p, r) for p, r in zip(box._pos.copy(), a)], dict(inp, seed=inp["seed"] + 1), precision=precision)
+    L = inp["box_length"]
+    lub = inp["hydrodynamics"].endswith("_lub")
     try:
-        errs = []
+        errs, derrs, cerrs, rerrs = [], [], [], []
         for _ in range(steps):
             start = ref._pos.copy()
             mix._pos[:] = start
             ref.propagate(dt)
             mix.propagate(dt)
             d_ref = ref._pos - start
-            errs.append(float(np.linalg.norm(mix._pos - start - d_ref) / np.linalg.norm(d_ref)))
-        return {"factor_residual": mix.factor_residual(), "fp64_factor_residual": ref.factor_residual(),
-                "step_displacement_rel_error": max(errs), "steps_compared": steps}
+            d_ref -= L * np.round(d_ref / L)
+            d_mix = mix._pos - start
+            d_mix -= L * np.round(d_mix / L)
+            errs.append(float(np.linalg.norm(d_mix - d_ref) / np.linalg.norm(d_ref)))
+            c_ref = (d_ref ** 2).sum(axis=1).mean() / (6 * dt)
+            c_mix = (d_mix ** 2).sum(axis=1).mean() / (6 * dt)
+            cerrs.append(float(abs(c_mix - c_ref) / c_ref))
+            D_ref = ref.D
+            derrs.append(float(np.abs(mix.D - D_ref).max() / np.abs(D_ref).max()))
+            if lub:
+                R_ref = ref.Rff
+                rerrs.append(float(np.abs(mix.Rff - R_ref).max() / np.abs(R_ref).max()))
+        out = {"precision": precision, "factor_residual": mix.factor_residual(),
+               "fp64_factor_residual": ref.factor_residual(), "D_rel_error": max(derrs),
+               "step_displacement_rel_error": max(errs), "diffusion_coefficient_rel_error": max(cerrs),
+               "steps_compared": steps}
+        if lub:
+            out["Rff_rel_error"] = max(rerrs)
+        return out
     finally:
         ref.close()
         mix.close()
 
 
-def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None):
+def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None, order="cost"):
     """The served request stream on the first `requests` requests (request
     k: seed k; pybrown_amd.serving), `steps` steps each:
       plain   one at a time on the plain FP64 path (Box.propagate per step,
               one synchronised step at a time, as BD.py drives a Box);
-      run     one at a time, all steps inside the library (Box.run);
+      run     one at a time at the served precision, all steps inside the
+              library (Box.run);
       burst   served on `slots` concurrent simulations, all arrived at the
-              start (throughput X);
+              start, admitted in `order` (throughput X);
       poisson served with Poisson arrivals at 1.5 X (the reported sims/s and
               p50/p99 time to trajectory).
-    Served results are checked against the one-at-a-time ones."""
+    Served results are checked against the one-at-a-time ones (bit for bit)
+    and, in a mixed precision mode, against the plain FP64 ones (relative
+    error of each request's final displacement)."""
     import functools
 
     import numpy as np
@@ -284,15 +312,16 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
     Server(slots=2, hbm_budget=budget, runner=runner).run(small, realtime=False)  # code objects, allocator
     plain = Server(slots=1, hbm_budget=budget, runner=run_plain).run(reqs, realtime=False)
     log("c5 plain one at a time: %s" % summary(plain))
-    one = Server(slots=1, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
+    one = Server(slots=1, hbm_budget=budget, runner=runner, precision=precision).run(reqs, realtime=False)
     log("c5 run one at a time: %s" % summary(one))
-    burst_res = Server(slots=slots, hbm_budget=budget, runner=runner).run(reqs, realtime=False)
+    burst_res = Server(slots=slots, hbm_budget=budget, runner=runner, precision=precision,
+                       order=order).run(reqs, realtime=False)
     burst = summary(burst_res)
-    log("c5 served, all at start: %s" % burst)
+    log("c5 served, all at start (%s order): %s" % (order, burst))
     rate = 1.5 * burst["sims_per_s"]
     for r, p in zip(reqs, stream_params(requests, rate=rate)):
         r.arrival = p["arrival"]
-    res = Server(slots=slots, hbm_budget=budget, runner=runner).run(reqs, realtime=True)
+    res = Server(slots=slots, hbm_budget=budget, runner=runner, precision=precision).run(reqs, realtime=True)
     served = summary(res)
     log("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served))
     failed = [(r.rid, r.N, r.mode, r.status) for r in res + burst_res if r.status != "ok"]
@@ -305,6 +334,27 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
     plain_same = all(a.status == p_.status == "ok" and np.array_equal(a.xyz, p_.xyz) and np.array_equal(a.flux, p_.flux)
                      for a, p_ in zip(one, plain))
     base = summary(plain)
+    mixed_err = None
+    if precision != "fp64":
+        # final displacement of each request (min image) against plain FP64;
+        # a request whose overlap test decides one attempt differently takes
+        # another trajectory from there (the largest errors name them)
+        errs, worst = [], []
+        for q, a, p_ in zip(reqs, res, plain):
+            if a.status == p_.status == "ok":
+                L = q.inp["box_length"]
+                d64 = p_.xyz - q.xyz
+                d64 -= L * np.round(d64 / L)
+                dmx = a.xyz - q.xyz
+                dmx -= L * np.round(dmx / L)
+                errs.append(float(np.linalg.norm(dmx - d64) / np.linalg.norm(d64)))
+                worst.append((errs[-1], q.rid, q.N, q.mode))
+        worst.sort(reverse=True)
+        mixed_err = {"final_displacement_rel_error_max": max(errs), "median": float(np.median(errs)),
+                     "largest": [{"rid": r, "N": n_, "mode": m, "error": e} for e, r, n_, m in worst[:3]],
+                     "flux_equal": sum(bool(np.array_equal(a.flux, p_.flux)) for a, p_ in zip(res, plain)),
+                     "compared": len(errs)}
+        log("c5 served %s against plain FP64: %s" % (precision, mixed_err))
     return {
         "requests": requests, "steps_per_request": steps, "slots": slots, "precision": precision,
         "sims_per_s": served["sims_per_s"], "p50_s": served["p50_s"], "p99_s": served["p99_s"],
@@ -315,6 +365,8 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
         "speedup_vs_plain_one_at_a_time": served["sims_per_s"] / base["sims_per_s"],
         "served_equal_to_one_at_a_time": bool(same),
         "run_equal_to_plain": bool(plain_same),
+        "burst_order": order,
+        "against_plain_fp64": mixed_err,
         "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES"),
     }
 
@@ -340,7 +392,7 @@ def serve_bench(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 + f32 (trailing Cholesky updates)",
+        "dtype": DTYPE_TEXT[args.precision],
         "data": "synthetic",
         "config": {"workload": "request stream: first %d requests, N log-uniform 32..3000, phi 0.02..0.30, "
                                "60%% rpy_smith / 25%% rpy (open box) ermak / 15%% rpy_smith_lub midpoint, dt=1ps, "
@@ -355,6 +407,11 @@ def serve_bench(args):
 
 METRIC = {"c2": "steps/s at N=1000 RPY-Ewald", "c1": "steps/s at N=100 RPY", "c3": "steps/s at N=500 RPY-Ewald+JO midpoint",
           "c4": "steps/s at N=4000 RPY-Ewald"}
+DTYPE_TEXT = {"fp64": "f64",
+              "mixed": "f64 + f32 (matrix-core bulk of the Cholesky and of the SPD inverses)",
+              "mixed_refined": "f64 + f32 (matrix-core bulk; SPD inverses refined by one FP64-residual "
+                               "Newton-Schulz step)",
+              "mixed_factor": "f64 + f32 (matrix-core bulk of the Cholesky; SPD inverses f64)"}
 WORKLOAD_TEXT = {
     "c1": "rpy N=100 a=5 L=150 ermak dt=1ps flux x=0",
     "c2": "rpy_smith N=1000 a=10 phi=0.05 L=437.6 alpha=sqrt(pi) m=n=2 ermak dt=1ps flux x=0",

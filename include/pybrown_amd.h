@@ -72,6 +72,13 @@ int pbx_cholesky_mixed(const double* A, int n, double* L);
 
 /* pyBrown/box.py:514,524,537 np.linalg.inv of an SPD matrix, times scale. */
 int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
+/* As pbx_spd_inverse_host, with the factorisation's trailing updates and the
+ * tile products of the triangular inverse and of inv(L)^T inv(L) on FP32
+ * matrix cores (diagonal tiles, their inverses and all stored values FP64).
+ * refine = 1 adds one Newton-Schulz step X + X (I - A X) whose residual is
+ * formed in FP64.  No reference counterpart: the mixed-precision resistance
+ * inversion measured against the FP64 one. */
+int pbx_spd_inverse_mixed(const double* A, int n, double scale, int refine, double* X);
 
 /* pbx_cholesky for a matrix already on the device, allocating nothing.  No
  * counterpart in the reference: it is for callers that cannot allocate
@@ -198,6 +205,11 @@ int pbx_sim_set_attempt_batch(pbx_sim* sim, int K);
  * No reference counterpart: it cuts host launch overhead where many small
  * simulations share the GPU (the serving loop). */
 int pbx_sim_set_graphs(pbx_sim* sim, int on);
+/* Lubrication modes: precision of the two SPD inverses behind each D
+ * (box.py:514,524,537): 0 = FP64 (default), 1 = FP32 matrix-core bulk as in
+ * pbx_spd_inverse_mixed, 2 = that plus one Newton-Schulz step (one more
+ * 3N x 3N matrix of device memory).  No reference counterpart. */
+int pbx_sim_set_inverse_precision(pbx_sim* sim, int precision);
 
 /* Look-ahead assembly (on by default).  In the RPY and Smith-Ewald modes
  * (not with lubrication, not with HIP graphs on), a step that built D ends

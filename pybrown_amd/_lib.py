@@ -79,6 +79,7 @@ _sig("check_overlaps", _i, [_dp, _dp, _i, _d, _d, _ip])
 _sig("pbx_cholesky", _i, [_dp, _i, _dp])
 _sig("pbx_cholesky_mixed", _i, [_dp, _i, _dp])
 _sig("pbx_spd_inverse_host", _i, [_dp, _i, _d, _dp])
+_sig("pbx_spd_inverse_mixed", _i, [_dp, _i, _d, _i, _dp])
 _sig("pbx_cholesky_noalloc", _i, [ctypes.c_void_p, _i, _i, ctypes.c_void_p, _i, ctypes.c_void_p])
 _sig("pbx_gemm_nt", _i, [_dp, _dp, _dp, _i, _i, _i])
 _sig("pbx_set_potrf_variant", _i, [_i])
@@ -93,6 +94,7 @@ _sig("pbx_sim_set_noise_source", _i, [ctypes.c_void_p, XI_FILL, ctypes.c_void_p]
 _sig("pbx_sim_set_seed", _i, [ctypes.c_void_p, ctypes.c_uint])
 _sig("pbx_sim_set_attempt_batch", _i, [ctypes.c_void_p, _i])
 _sig("pbx_sim_set_graphs", _i, [ctypes.c_void_p, _i])
+_sig("pbx_sim_set_inverse_precision", _i, [ctypes.c_void_p, _i])
 _sig("pbx_sim_set_lookahead", _i, [ctypes.c_void_p, _i])
 _sig("pbx_sim_snapshot_size", ctypes.c_longlong, [ctypes.c_void_p])
 _sig("pbx_sim_snapshot", _i, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong])
@@ -125,7 +127,7 @@ _sig("pbx_version", ctypes.c_char_p, [])
 EXPORTED = [
     "RPY_M_matrix", "RPY_Smith_M_matrix", "JO_2B_R_matrix", "JO_R_lubrication_correction_matrix",
     "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_cholesky_mixed",
-    "pbx_spd_inverse_host", "pbx_gemm_nt", "pbx_cholesky_noalloc",
+    "pbx_spd_inverse_host", "pbx_spd_inverse_mixed", "pbx_sim_set_inverse_precision", "pbx_gemm_nt", "pbx_cholesky_noalloc",
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs",

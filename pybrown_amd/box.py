@@ -59,6 +59,15 @@ _UNSUPPORTED = ("walls", "measure_concentration", "reactions")
 UNIT_FACTORS = {"joule": 1.0, "kcal/mol": 10 ** (-21) * 6.9477, "eV": 10 ** (-19) * 1.602176634}
 
 
+# Precision of the dense algebra: "fp64" throughout; "mixed": the noise
+# factor's trailing updates and (lubrication modes) the SPD inverses' bulk on
+# FP32 matrix cores, diagonal tiles FP64; "mixed_refined": as "mixed", each
+# SPD inverse followed by one Newton-Schulz step with an FP64 residual;
+# "mixed_factor": the noise factor as "mixed", the SPD inverses in FP64.
+PRECISIONS = ("fp64", "mixed", "mixed_refined", "mixed_factor")
+_INVERSE_PRECISION = {"fp64": 0, "mixed": 1, "mixed_refined": 2, "mixed_factor": 0}
+
+
 class Box:
     """Simulation box (pyBrown/box.py:30-109)."""
 
@@ -79,8 +88,8 @@ class Box:
         if inp["propagation_scheme"] not in _lib.SCHEME_CODES:
             raise ValueError("unknown propagation_scheme %r" % inp["propagation_scheme"])
 
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of %s" % ", ".join(repr(p) for p in PRECISIONS))
         self.precision = precision
         self._init_args = (xi_pool_attempts, noise, attempt_batch, precision, bool(graphs))
         self.beads = beads
@@ -157,7 +166,7 @@ class Box:
         cfg.n_labels = max(1, len(self.mobile_labels))
         cfg.max_move_attempts = int(inp["max_move_attempts"])
         cfg.xi_pool_attempts = int(xi_pool_attempts)
-        cfg.factor_precision = 1 if precision == "mixed" else 0
+        cfg.factor_precision = 0 if precision == "fp64" else 1
         cfg.divergence_term = 1 if inp["divergence_term"] else 0
         # bead.py:401-413 connection matrix: bonded pairs skip the overlap test
         self._exempt = _overlap_exemptions(self._dev_beads)
@@ -207,6 +216,8 @@ class Box:
         _lib.check_rc(lib.pbx_sim_set_attempt_batch(self._sim, int(attempt_batch)), "set_attempt_batch")
         # the factorisation and SPD-inverse launch sequences as HIP graphs
         _lib.check_rc(lib.pbx_sim_set_graphs(self._sim, 1 if graphs else 0), "set_graphs")
+        _lib.check_rc(lib.pbx_sim_set_inverse_precision(self._sim, _INVERSE_PRECISION[precision]),
+                      "set_inverse_precision")
         self._set_forces()
         self._synced = self._pos.copy()
         self._counts = np.zeros(max(1, len(self.mobile_labels)), dtype=np.int64)

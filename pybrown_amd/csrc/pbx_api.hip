@@ -45,14 +45,14 @@ int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const dou
                      double* y, hipStream_t s);
 int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, const double* F, double beta, double* y,
                          hipStream_t s);
+int pbx_launch_drift_diff(const double* D1, const double* D0, int n, int ld, const double* x, double alpha, double* y,
+                          hipStream_t s);
 // flags: np / 64 + 1 ints (row flags, then the row ticket counter), zero
 // before the first call; epoch: 1, 2, ... per call
 int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
                        int epoch, int* info, hipStream_t s);
 int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
                              int cich, double mul, double div, double* C, int ld, hipStream_t s);
-int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
-                    double scale);
 
 // ---------------------------------------------------------------------------
 // error state
@@ -290,21 +290,22 @@ extern "C" int pbx_cholesky(const double* A, int n, double* L) { return cholesky
 // tiles factored and inverted in FP64.
 extern "C" int pbx_cholesky_mixed(const double* A, int n, double* L) { return cholesky_host(A, n, L, 1); }
 
-// X = scale * inv(A) for SPD A (box.py:514,524,537 np.linalg.inv call sites)
-extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double* X) {
+static int spd_inverse_host(const double* A, int n, double scale, int precision, double* X) {
     pbx_clear_error();
-    if (n <= 0) return PBX_ERR_ARG;
+    if (n <= 0 || precision < 0 || precision > 2) return PBX_ERR_ARG;
     int np = (int)pbx_round_up(n, 64), ld = np;
-    DevBuf dA, dX, dW, dI;
+    DevBuf dA, dX, dW, dI, dC;
     TRY(dA.alloc(sizeof(double) * (size_t)np * ld));
     TRY(dX.alloc(sizeof(double) * (size_t)np * ld));
+    if (precision == 2) TRY(dC.alloc(sizeof(double) * (size_t)np * ld));
     TRY(dW.alloc(sizeof(double) * pbx_dense_work_doubles(np)));
     TRY(dI.alloc(sizeof(int)));
     PBX_HIP(hipMemset(dI.p, 0, sizeof(int)));
     PBX_HIP(hipMemcpy2D(dA.p, sizeof(double) * ld, A, sizeof(double) * n, sizeof(double) * n, n,
                         hipMemcpyHostToDevice));
     TRY(pbx_fill_pad(dA.as<double>(), n, np, ld, 0));
-    TRY(pbx_spd_inverse(dA.as<double>(), np, ld, dW.as<double>(), dX.as<double>(), dI.as<int>(), 0, scale));
+    TRY(pbx_spd_inverse(dA.as<double>(), np, ld, dW.as<double>(), dX.as<double>(), dI.as<int>(), 0, scale,
+                        precision, precision == 2 ? dC.as<double>() : nullptr));
     int info = 0;
     PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
     PBX_HIP(hipMemcpy2D(X, sizeof(double) * n, dA.p, sizeof(double) * ld, sizeof(double) * n, n,
@@ -315,6 +316,22 @@ extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double
     }
     if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
     return info;
+}
+
+// X = scale * inv(A) for SPD A (box.py:514,524,537 np.linalg.inv call sites)
+extern "C" int pbx_spd_inverse_host(const double* A, int n, double scale, double* X) {
+    return spd_inverse_host(A, n, scale, 0, X);
+}
+
+// The same with the factorisation's trailing updates and the inverse's tile
+// products on FP32 matrix cores; refine = 1 adds one Newton-Schulz step with
+// an FP64 residual.
+extern "C" int pbx_spd_inverse_mixed(const double* A, int n, double scale, int refine, double* X) {
+    if (refine != 0 && refine != 1) {
+        pbx_set_error(PBX_ERR_ARG, "spd_inverse_mixed: refine must be 0 or 1");
+        return PBX_ERR_ARG;
+    }
+    return spd_inverse_host(A, n, scale, 1 + refine, X);
 }
 
 // C = A B^T, all dimensions multiples of 64 (checks the MFMA tile layout)
@@ -347,6 +364,11 @@ struct PbxSim {
     int *flag = nullptr, *info = nullptr;
     double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
     double *Wchol = nullptr, *Winv = nullptr;
+    // lubrication modes: precision of the two SPD inverses per D (0 FP64,
+    // 1 FP32 matrix-core bulk, 2 that plus one Newton-Schulz step, whose
+    // copy of the input matrix is Xref)
+    int inv_precision = 0;
+    double* Xref = nullptr;
     // launch sequences with fixed arguments (the factorisation and the SPD
     // inverses) replayed from HIP graphs when `graphs` is on: captured on
     // first use per (operation, output buffer, Cholesky variant)
@@ -572,7 +594,10 @@ int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
     // Mff = M 1e19 / eta ; Rff = inv(Mff)
     TRY(assemble_M(s, RffOut, 1e19 / c.viscosity));
     return run_graphed(s, GRAPH_INV_RFF, RffOut,
-                       [&]() { return pbx_spd_inverse(RffOut, s->np, s->ld, s->Winv, s->X, s->info, s->stream, 1.0); });
+                       [&]() {
+                           return pbx_spd_inverse(RffOut, s->np, s->ld, s->Winv, s->X, s->info, s->stream, 1.0,
+                                                  s->inv_precision, s->Xref);
+                       });
 }
 
 // box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R)
@@ -582,7 +607,8 @@ int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
     TRY(pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff, c.cichocki_correction,
                                  c.viscosity, 1e19, Dout, s->ld, s->stream));
     return run_graphed(s, GRAPH_INV_R, Dout, [&]() {
-        TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT));
+        TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT, s->inv_precision,
+                            s->Xref));
         return pbx_fill_pad(Dout, s->n, s->np, s->ld, s->stream);
     });
 }
@@ -859,7 +885,11 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         // status of the midpoint inverses and of the solve's bounded waits,
         // read at this attempt's overlap check or at the step's end
         if (!c.queued) TRY(post_info(s));
-        TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
+        if (cf.factor_precision)
+            // FP32-bulk factor: the same drift as (D_mid - D0) B0^{-T} xi
+            TRY(pbx_launch_drift_diff(Dm, s->D, s->n, s->ld, s->w, m / 2 * sqrt(2 * dt), s->drift, s->stream));
+        else
+            TRY(pbx_launch_drift(Dm, s->B, s->n, s->ld, s->w, s->xipad, m / 2 * sqrt(2 * dt), s->drift, s->stream));
         if (s->F_nonzero)
             TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));
         TRY(det(s, s->D, dt, 1.0 - 1.0 / m, c));
@@ -1321,6 +1351,23 @@ extern "C" int pbx_sim_set_graphs(pbx_sim* s, int on) {
         drop_graphs(s);
     }
     s->graphs = on;
+    return PBX_OK;
+}
+
+extern "C" int pbx_sim_set_inverse_precision(pbx_sim* s, int precision) {
+    pbx_clear_error();
+    if (!s || precision < 0 || precision > 2) {
+        pbx_set_error(PBX_ERR_ARG, "set_inverse_precision: precision must be 0, 1 or 2");
+        return PBX_ERR_ARG;
+    }
+    if (precision == 2 && is_lub(s) && !s->Xref) TRY(sim_alloc(s, (void**)&s->Xref, sizeof(double) * (size_t)s->np * s->ld));
+    if (precision != s->inv_precision) {
+        // captured inverse sequences hold the previous precision's kernels
+        PBX_HIP(hipStreamSynchronize(s->stream));
+        drop_graphs(s);
+        s->spec_ok = 0;
+    }
+    s->inv_precision = precision;
     return PBX_OK;
 }
 

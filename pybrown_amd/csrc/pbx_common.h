@@ -177,8 +177,10 @@ int pbx_launch_forces(const PbxForceArgs& a, const double* pos, double* F, doubl
 
 int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld, double* acc, double* rel,
                         hipStream_t s);
+// A := scale * inv(A); precision 0 FP64, 1 FP32 matrix-core bulk, 2 as 1 plus
+// one Newton-Schulz step with an FP64 residual (C: np x np scratch, else null)
 int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
-                    double scale);
+                    double scale, int precision = 0, double* C = nullptr);
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
 int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,

@@ -866,10 +866,55 @@ __device__ __forceinline__ void chain_product(int k0, int k1, FA tileA, FB tileB
     }
 }
 
-template <int PHASE>
+// The same on the FP32 matrix cores (mixed-precision inverse): operands
+// rounded to FP32 as they are staged, each 64-deep slab accumulated in FP32
+// and added into the FP64 sum (FP32 accumulator layout).
+template <class FA, class FB>
+__device__ __forceinline__ void chain_product_f32(int k0, int k1, FA tileA, FB tileB, int lda, int ldb, float* sA,
+                                                  float* sB, v4d sum[2][2]) {
+    if (k0 > k1) return;
+    v2d fa[8], fb[8];
+    stage_fetch(fa, tileA(k0), lda);
+    stage_fetch(fb, tileB(k0), ldb);
+    for (int k = k0; k <= k1; ++k) {
+        if (k > k0) __syncthreads();
+        stage_put_f32t(sA, fa);  // op(A) = tileA
+        stage_put_f32n(sB, fb);  // op(B) = tileB
+        __syncthreads();
+        if (k < k1) {
+            stage_fetch(fa, tileA(k + 1), lda);
+            stage_fetch(fb, tileB(k + 1), ldb);
+        }
+        v4f acc[2][2];
+        acc32_zero(acc);
+        mma64_f32(sA, sB, acc);
+        acc32_add_to(acc, sum);
+    }
+}
+
+// One level's tile products, in FP64 or (MIXED) on the FP32 matrix cores;
+// results are stored in FP64 either way.
+template <bool MIXED, class FA, class FB, class FO>
+__device__ __forceinline__ void level_product(int k0, int k1, FA tileA, FB tileB, int lda, int ldb, FO out) {
+    if constexpr (MIXED) {
+        __shared__ float fA[T64 * SPF];
+        __shared__ float fB[T64 * SPF];
+        v4d sum[2][2];
+        acc_zero(sum);
+        chain_product_f32(k0, k1, tileA, tileB, lda, ldb, fA, fB, sum);
+        acc32d_for_each(sum, out);
+    } else {
+        __shared__ double sA[T64 * SP];
+        __shared__ double sB[T64 * SP];
+        v4d acc[2][2];
+        acc_zero(acc);
+        chain_product(k0, k1, tileA, tileB, lda, ldb, sA, sB, acc);
+        acc_for_each(acc, out);
+    }
+}
+
+template <int PHASE, bool MIXED>
 __global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int ld, double* X, int ldx, int T, int b) {
-    __shared__ double sA[T64 * SP];
-    __shared__ double sB[T64 * SP];
     const int pair = blockIdx.x / (b * b), rem = blockIdx.x - pair * b * b;
     const int p = rem / b, q = rem - p * b;
     const int s = pair * 2 * b;
@@ -877,20 +922,18 @@ __global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int l
     if (p >= h2) return;  // whole workgroup: B2 is shorter than b
     auto Lt = [&](int i, int j) { return A + (size_t)i * T64 * ld + (size_t)j * T64; };
     auto Xt = [&](int i, int j) { return X + (size_t)i * T64 * ldx + (size_t)j * T64; };
-    v4d acc[2][2];
-    acc_zero(acc);
     if (PHASE == 0) {
         // T1(p, q) = sum_{k=q}^{b-1} L(s+b+p, s+k) X(s+k, s+q)
-        chain_product(q, b - 1, [&](int k) { return Lt(s + b + p, s + k); },
-                      [&](int k) { return (const double*)Xt(s + k, s + q); }, ld, ldx, sA, sB, acc);
         double* out = Xt(s + q, s + b + p);
-        acc_for_each(acc, [&](int r, int c, double v) { out[(size_t)r * ldx + c] = v; });
+        level_product<MIXED>(q, b - 1, [&](int k) { return Lt(s + b + p, s + k); },
+                             [&](int k) { return (const double*)Xt(s + k, s + q); }, ld, ldx,
+                             [&](int r, int c, double v) { out[(size_t)r * ldx + c] = v; });
     } else {
         // X(s+b+p, s+q) = -sum_{k=0}^{p} X(s+b+p, s+b+k) T1(k, q)
-        chain_product(0, p, [&](int k) { return (const double*)Xt(s + b + p, s + b + k); },
-                      [&](int k) { return (const double*)Xt(s + q, s + b + k); }, ldx, ldx, sA, sB, acc);
         double* out = Xt(s + b + p, s + q);
-        acc_for_each(acc, [&](int r, int c, double v) { out[(size_t)r * ldx + c] = -v; });
+        level_product<MIXED>(0, p, [&](int k) { return (const double*)Xt(s + b + p, s + b + k); },
+                             [&](int k) { return (const double*)Xt(s + q, s + b + k); }, ldx, ldx,
+                             [&](int r, int c, double v) { out[(size_t)r * ldx + c] = -v; });
     }
 }
 
@@ -953,31 +996,97 @@ __global__ __launch_bounds__(256) void residual_kernel(const double* D, const do
 }
 
 // Y = scale * X^T X, lower tiles computed, both triangles written:
-// Y_ij = sum_{m >= i} X_mi^T X_mj for i >= j.
+// Y_ij = sum_{m >= i} X_mi^T X_mj for i >= j.  MIXED: products on the FP32
+// matrix cores (operands rounded to FP32, FP32 accumulation within each
+// 64-deep slab, FP64 across slabs).
+template <bool MIXED>
 __global__ __launch_bounds__(256) void lauum_kernel(const double* X, int ldx, double* Y, int ld, int T,
                                                     double scale) {
-    __shared__ double sA[T64 * SP];
-    __shared__ double sB[T64 * SP];
     int i, j;
     tri_decode(blockIdx.x, i, j);
-    v4d acc[2][2];
-    acc_zero(acc);
-    for (int m = i; m < T; ++m) {
-        stage<false>(sA, X + (size_t)m * T64 * ldx + (size_t)i * T64, ldx);  // op(A)(r,k) = X_mi[k][r]
-        stage<false>(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);  // op(B)(k,c) = X_mj[k][c]
-        __syncthreads();
-        mma64(sA, sB, acc);
-        __syncthreads();
-    }
     double* Yij = Y + (size_t)i * T64 * ld + (size_t)j * T64;
     double* Yji = Y + (size_t)j * T64 * ld + (size_t)i * T64;
-    acc_for_each(acc, [&](int r, int c, double v) {
+    auto put = [&](int r, int c, double v) {
         double s = v * scale;
         if (i != j || c <= r) {
             Yij[(size_t)r * ld + c] = s;
             Yji[(size_t)c * ld + r] = s;
         }
-    });
+    };
+    if constexpr (MIXED) {
+        __shared__ float fA[T64 * SPF];
+        __shared__ float fB[T64 * SPF];
+        v4d sum[2][2];
+        acc_zero(sum);
+        for (int m = i; m < T; ++m) {
+            v2d va[8], vb[8];
+            stage_fetch(va, X + (size_t)m * T64 * ldx + (size_t)i * T64, ldx);
+            stage_fetch(vb, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);
+            stage_put_f32n(fA, va);  // op(A)(r,k) = X_mi[k][r]
+            stage_put_f32n(fB, vb);  // op(B)(k,c) = X_mj[k][c]
+            __syncthreads();
+            v4f acc[2][2];
+            acc32_zero(acc);
+            mma64_f32(fA, fB, acc);
+            acc32_add_to(acc, sum);
+            __syncthreads();
+        }
+        acc32d_for_each(sum, put);
+    } else {
+        __shared__ double sA[T64 * SP];
+        __shared__ double sB[T64 * SP];
+        v4d acc[2][2];
+        acc_zero(acc);
+        for (int m = i; m < T; ++m) {
+            stage<false>(sA, X + (size_t)m * T64 * ldx + (size_t)i * T64, ldx);  // op(A)(r,k) = X_mi[k][r]
+            stage<false>(sB, X + (size_t)m * T64 * ldx + (size_t)j * T64, ldx);  // op(B)(k,c) = X_mj[k][c]
+            __syncthreads();
+            mma64(sA, sB, acc);
+            __syncthreads();
+        }
+        acc_for_each(acc, put);
+    }
+}
+
+// Newton-Schulz refinement of an SPD inverse, X1 = X0 + X0 (I - A X0), in
+// two launches over 64 x 64 tiles (A and X0 symmetric, both triangles held):
+//   ns_residual: E = I - A X0 in FP64 (the residual needs the full
+//                precision: its entries are of the size of X0's error);
+//   ns_update:   X1_ij = scale (X0_ij + sum_k X0_ik E_kj) for the lower
+//                tiles, both triangles written (X1 is neither X0 nor E).  E is small, so its product with X0 runs
+//                on the FP32 matrix cores when MIXED: the correction carries
+//                an FP32-relative error, times |E|.
+__global__ __launch_bounds__(256) void ns_residual_kernel(const double* A, const double* X0, int ld, int T,
+                                                          double* E) {
+    __shared__ double sA[T64 * SP];
+    __shared__ double sB[T64 * SP];
+    const int i = blockIdx.x / T, j = blockIdx.x - i * T;
+    v4d acc[2][2];
+    acc_zero(acc);
+    // (A X0)_ij = sum_k A_ik X0_kj
+    chain_product(0, T - 1, [&](int k) { return A + (size_t)i * T64 * ld + (size_t)k * T64; },
+                  [&](int k) { return X0 + (size_t)k * T64 * ld + (size_t)j * T64; }, ld, ld, sA, sB, acc);
+    double* Eij = E + (size_t)i * T64 * ld + (size_t)j * T64;
+    acc_for_each(acc, [&](int r, int c, double v) { Eij[(size_t)r * ld + c] = ((i == j && r == c) ? 1.0 : 0.0) - v; });
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(256) void ns_update_kernel(const double* X0, const double* E, int ld, int T,
+                                                        double* X1, double scale) {
+    int i, j;
+    tri_decode(blockIdx.x, i, j);
+    const double* X0ij = X0 + (size_t)i * T64 * ld + (size_t)j * T64;
+    double* Xij = X1 + (size_t)i * T64 * ld + (size_t)j * T64;
+    double* Xji = X1 + (size_t)j * T64 * ld + (size_t)i * T64;
+    level_product<MIXED>(0, T - 1, [&](int k) { return X0 + (size_t)i * T64 * ld + (size_t)k * T64; },
+                         [&](int k) { return E + (size_t)k * T64 * ld + (size_t)j * T64; }, ld, ld,
+                         [&](int r, int c, double v) {
+                             const double x = (X0ij[(size_t)r * ld + c] + v) * scale;
+                             if (i != j || c <= r) {
+                                 Xij[(size_t)r * ld + c] = x;
+                                 Xji[(size_t)c * ld + r] = x;
+                             }
+                         });
 }
 
 // ---------------------------------------------------------------------------
@@ -1234,22 +1343,50 @@ int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int
     return PBX_OK;
 }
 
-// A := scale * inv(A) for SPD A (both triangles written).  X is an np x np
-// scratch matrix (leading dimension ld).
-int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
-                    double scale) {
-    int rc = pbx_potrf(A, np, ld, work, info_dev, s);
-    if (rc) return rc;
-    const int T = np / T64;
+template <bool MIXED>
+static void trtri_lauum(double* A, int T, int ld, double* work, double* X, hipStream_t s, double scale) {
     // every lower tile of X is written below before lauum reads it (the
     // upper tiles hold the recursion's partial products and are not read)
     PBX_LAUNCH(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
     for (int b = 1; b < T; b *= 2) {
         const int npairs = (T - b + 2 * b - 1) / (2 * b);
-        PBX_LAUNCH(trtri_level_kernel<0>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
-        PBX_LAUNCH(trtri_level_kernel<1>, dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+        PBX_LAUNCH((trtri_level_kernel<0, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
+        PBX_LAUNCH((trtri_level_kernel<1, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
     }
-    PBX_LAUNCH(lauum_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+    PBX_LAUNCH(lauum_kernel<MIXED>, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+}
+
+// A := scale * inv(A) for SPD A (both triangles written).  X is an np x np
+// scratch matrix (leading dimension ld).  precision 0: FP64 throughout;
+// 1: the factorisation's trailing updates and the products of the triangular
+// inverse and of X^T X on the FP32 matrix cores (diagonal tiles, their
+// inverses and every stored value FP64); 2: as 1, then one Newton-Schulz step
+// with an FP64 residual (C: a further np x np matrix, overwritten).
+int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
+                    double scale, int precision, double* C) {
+    if (precision < 0 || precision > 2 || (precision == 2 && !C)) {
+        pbx_set_error(PBX_ERR_ARG, "spd_inverse: precision %d (2 needs a second scratch matrix)", precision);
+        return PBX_ERR_ARG;
+    }
+    const int T = np / T64;
+    if (precision == 2) {
+        // C = A; C := inv(C) in the mixed precision; X := I - A C;
+        // A := scale (C + C X)
+        PBX_HIP(hipMemcpyAsync(C, A, sizeof(double) * (size_t)np * ld, hipMemcpyDeviceToDevice, s));
+        const int rc = pbx_potrf(C, np, ld, work, info_dev, s, 1);
+        if (rc) return rc;
+        trtri_lauum<true>(C, T, ld, work, X, s, 1.0);
+        PBX_LAUNCH(ns_residual_kernel, dim3(T * T), dim3(256), 0, s, A, C, ld, T, X);
+        PBX_LAUNCH(ns_update_kernel<true>, dim3(T * (T + 1) / 2), dim3(256), 0, s, C, X, ld, T, A, scale);
+        PBX_HIP(hipGetLastError());
+        return PBX_OK;
+    }
+    const int rc = pbx_potrf(A, np, ld, work, info_dev, s, precision);
+    if (rc) return rc;
+    if (precision)
+        trtri_lauum<true>(A, T, ld, work, X, s, scale);
+    else
+        trtri_lauum<false>(A, T, ld, work, X, s, scale);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -157,6 +157,23 @@ __global__ __launch_bounds__(256) void drift_kernel(const double* __restrict__ D
     if ((threadIdx.x & 63) == 0) y[r] = alpha * (s1 - s2);
 }
 
+// y = alpha * (D1 - D0) x over full rows: the midpoint's stochastic drift in
+// the mixed-precision engine, alpha (D_mid B0^{-T} - B0) xi rewritten with
+// B0 xi = B0 B0^T B0^{-T} xi = D0 x (x = B0^{-T} xi).  Equal in exact
+// arithmetic; without the cancellation of two nearly equal products, which
+// would multiply the FP32-bulk factor's residual B0 B0^T - D0 by alpha.
+__global__ __launch_bounds__(256) void drift_diff_kernel(const double* __restrict__ D1, const double* __restrict__ D0,
+                                                         int n, int ld, const double* __restrict__ x, double alpha,
+                                                         double* y) {
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += (D1[(size_t)r * ld + c] - D0[(size_t)r * ld + c]) * x[c];
+    s = wave_sum(s);
+    if (lane == 0) y[r] = alpha * s;
+}
+
 // y += beta * (D1 - D0) F over full rows (midpoint deterministic drift)
 __global__ __launch_bounds__(256) void det_drift_kernel(const double* __restrict__ D1, const double* __restrict__ D0,
                                                         int n, int ld, const double* __restrict__ F, double beta,
@@ -615,6 +632,13 @@ int pbx_launch_lower_gemv(const double* Lm, int n, int ld, const double* x, doub
 int pbx_launch_drift(const double* D, const double* Lo, int n, int ld, const double* x, const double* z, double alpha,
                      double* y, hipStream_t s) {
     PBX_LAUNCH(drift_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D, Lo, n, ld, x, z, alpha, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_drift_diff(const double* D1, const double* D0, int n, int ld, const double* x, double alpha, double* y,
+                          hipStream_t s) {
+    PBX_LAUNCH(drift_diff_kernel, dim3((n + 3) / 4), dim3(256), 0, s, D1, D0, n, ld, x, alpha, y);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -146,6 +146,18 @@ __device__ __forceinline__ void stage_f32t(float* s, const double* X, int ld) {
     stage_put_f32t(s, v);
 }
 
+// s[row * SPF + col] = (float)X[row][col] (the tile as the B operand, or its
+// transpose as the A operand)
+__device__ __forceinline__ void stage_put_f32n(float* s, const v2d v[8]) {
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+        const int e = threadIdx.x + it * 256;
+        const int row = e >> 5, col = (e & 31) * 2;
+        s[row * SPF + col] = (float)v[it].x;
+        s[row * SPF + col + 1] = (float)v[it].y;
+    }
+}
+
 __device__ __forceinline__ void mma64_f32(const float* sA, const float* sB, v4f acc[2][2]) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r0 = (w >> 1) * 32, c0 = (w & 1) * 32;
@@ -173,6 +185,29 @@ __device__ __forceinline__ void acc32_zero(v4f acc[2][2]) {
 // Tile row of FP32 accumulator element (x, v) of this thread (columns as acc_col).
 __device__ __forceinline__ int acc32_row(int x, int v) {
     return ((threadIdx.x >> 7) * 32) + 16 * x + 4 * ((threadIdx.x & 63) >> 4) + v;
+}
+
+// FP64 sums of FP32 slab products, in the FP32 accumulator's layout: each
+// 64-deep slab accumulates in FP32 on the matrix cores and is then added in
+// FP64, so accumulation rounding grows with the slab depth, not with the
+// whole inner dimension.
+__device__ __forceinline__ void acc32_add_to(const v4f acc[2][2], v4d sum[2][2]) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sum[x][y][v] += (double)acc[x][y][v];
+}
+
+template <class F>
+__device__ __forceinline__ void acc32d_for_each(v4d sum[2][2], F f) {
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) f(acc32_row(x, v), acc_col(y), sum[x][y][v]);
 }
 
 __device__ __forceinline__ void acc32_load(const double* C, int ld, double out[16]) {

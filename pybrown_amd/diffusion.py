@@ -93,11 +93,20 @@ def cholesky(D, precision="fp64"):
     return L
 
 
-def spd_inverse(A, scale=1.0):
-    """scale * np.linalg.inv(A) for SPD A (box.py:514,524,537)."""
+def spd_inverse(A, scale=1.0, precision="fp64"):
+    """scale * np.linalg.inv(A) for SPD A (box.py:514,524,537).  precision
+    "mixed": the factorisation's trailing updates and the tile products of
+    the triangular inverse and of inv(L)^T inv(L) on the FP32 matrix cores
+    (diagonal tiles and stored values FP64); "mixed_refined": that, then one
+    Newton-Schulz step X + X (I - A X) with the residual in FP64."""
+    if precision not in ("fp64", "mixed", "mixed_refined"):
+        raise ValueError("precision must be 'fp64', 'mixed' or 'mixed_refined'")
     A = np.ascontiguousarray(A, dtype=np.float64)
     n = A.shape[0]
     X = np.zeros_like(A)
-    rc = lib.pbx_spd_inverse_host(ptr(A), n, float(scale), ptr(X))
+    if precision == "fp64":
+        rc = lib.pbx_spd_inverse_host(ptr(A), n, float(scale), ptr(X))
+    else:
+        rc = lib.pbx_spd_inverse_mixed(ptr(A), n, float(scale), 1 if precision == "mixed_refined" else 0, ptr(X))
     _lib.check_rc(rc, "spd_inverse")
     return X

@@ -133,11 +133,12 @@ def make_requests_parallel(params, steps=200, dt=1.0, workers=8):
         return list(ex.map(make_request, params, [steps] * len(params), [dt] * len(params)))
 
 
-def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attempts=64):
+def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attempts=64, precision="fp64"):
     """Upper bound of the device memory one simulation holds: the dense
     n x n matrices (D and B; without lubrication the look-ahead copy of D;
-    with lubrication the far-field resistance and an inverse scratch;
-    midpoint one more each), the factorisation work
+    with lubrication the far-field resistance and an inverse scratch, and
+    with refined inverses a copy of the matrix inverted; midpoint one more
+    each), the factorisation work
     areas (diagonal inverses and state), the noise ring (two chunks of
     xi_pool_attempts x 3N draws) and O(N) vectors."""
     n = 3 * N
@@ -149,6 +150,8 @@ def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attem
         mats, works = 0, 0
     else:
         mats = 2 + (1 if mid else 0) + ((2 + (1 if mid else 0)) if lub else 1)
+        if lub and precision == "mixed_refined":
+            mats += 1
         works = 2 if lub else 1
     work = 8 * (np_ * 64 + 8 + 64 * 64)
     ring = 8 * 2 * xi_pool_attempts * n
@@ -160,10 +163,12 @@ def footprint_bytes(N, mode, scheme="ermak", check_overlaps=False, xi_pool_attem
     return 8 * np_ * np_ * mats + works * work + ring + vecs + batch + (1 << 20)
 
 
-def request_footprint(req):
-    """footprint_bytes for a request's own keywords."""
+def request_footprint(req, precision="fp64"):
+    """footprint_bytes for a request's own keywords in a served precision
+    mode."""
     return footprint_bytes(req.N, req.mode, req.inp.get("propagation_scheme", "ermak"),
-                           bool(req.inp.get("check_overlaps", True)))
+                           bool(req.inp.get("check_overlaps", True)),
+                           precision=request_precision(req, precision))
 
 
 class Result:
@@ -182,17 +187,31 @@ class Result:
         return None if self.end is None else self.end - self.arrival
 
 
+def request_precision(req, precision):
+    """The Box precision a request runs at in a served precision mode:
+    "fp64" throughout, or "mixed" (FP32 matrix-core bulk of the noise
+    factor).  Lubrication requests keep their SPD inverses in FP64
+    ("mixed_factor"): their midpoint drift takes the difference of D at two
+    positions times m/2 = 50, so FP32-bulk inverses (D to a few 1e-6) put a
+    few 1e-5 of relative error into the displacement, and refining them
+    (one Newton-Schulz step, 2 n^3 FP64 flops per inverse against about n^3
+    for the inverse itself) makes those requests slower than FP64."""
+    if precision == "mixed" and req.mode in ("rpy_lub", "rpy_smith_lub"):
+        return "mixed_factor"
+    return precision
+
+
 def run_one(req, graphs=False, precision="fp64"):
     """Create the simulation on the device, run all steps inside the library
     (Box.run: steps that need no host decision are queued back to back), free
     it.  Returns (per-step flux (steps, labels), final positions (N, 3),
     device bytes held).  graphs: replay the factorisation and SPD-inverse
-    launch sequences from HIP graphs; precision "mixed": the noise factor's
-    trailing updates on FP32 matrix cores."""
+    launch sequences from HIP graphs; precision: the served precision mode
+    (request_precision)."""
     from pybrown_amd.bead import Bead
     from pybrown_amd.box import Box
     beads = [Bead(req.xyz[i].copy(), float(req.radii[i]), label="XXX") for i in range(req.N)]
-    box = Box(beads, req.inp, graphs=graphs, precision=precision)
+    box = Box(beads, req.inp, graphs=graphs, precision=request_precision(req, precision))
     try:
         held = box.device_bytes
         flux, _ = box.run(req.dt, req.steps)
@@ -220,25 +239,49 @@ def run_plain(req):
         box.close()
 
 
+def estimated_cost(req):
+    """Relative device work of a request, for ordering a batch: its steps
+    times the dense algebra of one step (n^3/3 for the noise factor; with
+    lubrication two SPD inverses of about n^3 each per D, and the midpoint's
+    second D), plus a per-step floor for the launch-bound small ones."""
+    n = 3.0 * req.N
+    lub = req.mode in ("rpy_lub", "rpy_smith_lub")
+    mid = req.inp.get("propagation_scheme", "ermak") == "midpoint"
+    per_step = n ** 3 / 3.0 + (2.0 * n ** 3 * (2 if mid else 1) if lub else 0.0)
+    return req.steps * (per_step + 2e9)
+
+
 class Server:
     """Continuous admission of a request stream onto `slots` concurrent
-    device-resident simulations within an HBM budget (bytes)."""
+    device-resident simulations within an HBM budget (bytes).  precision:
+    the served precision mode (request_precision; it also sets the
+    footprints).  order: "arrival" admits in arrival order; "cost" admits a
+    batch that has all arrived (realtime=False) largest estimated_cost
+    first, so the longest simulations start first and the short ones fill
+    the slots around them (longest-processing-time order)."""
 
-    def __init__(self, slots=8, hbm_budget=250e9, runner=run_one):
+    def __init__(self, slots=8, hbm_budget=250e9, runner=run_one, precision="fp64", order="arrival"):
         if slots < 1:
             raise ValueError("slots must be >= 1")
+        if order not in ("arrival", "cost"):
+            raise ValueError("order must be 'arrival' or 'cost'")
         self.slots = int(slots)
         self.budget = float(hbm_budget)
         self.runner = runner
+        self.precision = precision
+        self.order = order
 
     def run(self, requests, realtime=True):
         """Serve `requests` (arrival times in seconds from the start; with
         realtime=False all have arrived at the start).  Returns the results
         in request order.  A request larger than the whole budget fails with
         status 'rejected: footprint'."""
-        order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival if realtime else 0.0, i))
+        if realtime or self.order == "arrival":
+            order = sorted(range(len(requests)), key=lambda i: (requests[i].arrival if realtime else 0.0, i))
+        else:
+            order = sorted(range(len(requests)), key=lambda i: (-estimated_cost(requests[i]), i))
         results = [Result(r.rid, r.N, r.mode) for r in requests]
-        fps = [request_footprint(r) for r in requests]
+        fps = [request_footprint(r, self.precision) for r in requests]
         cv = threading.Condition()
         state = {"free": self.slots, "mem": 0.0, "done": 0}
         work = []
