@@ -62,3 +62,56 @@ def test_truncate(tmp_path):
     f.write_text("a\nb\nc\nd\n")
     io.truncate_output_file_during_restart(str(f), 2)
     assert f.read_text() == "a\nb\n"
+
+
+# ---------------------------------------------------------------------------
+# Output text against literal known answers in the reference's formats, so
+# the writers are pinned independently of themselves.  The expected strings
+# spell out what Python's `str.format` with '{}' fields produces for these
+# inputs, field by field, as output.py builds the lines:
+#   .xyz  output.py:63-68   'N', '<file> time [ps] <j*dt>', '<label> <x> <y> <z>'
+#   .enr  output.py:108-120 header 'time/ps energy/<unit>' at j = 0, then '<j*dt> <E>'
+#   .flx  output.py:122-140 header 'time/ps <labels>' at j = 0, else '<j*dt> <counts>'
+# The inputs include the float spellings '{}' gives: shortest round-trip
+# digits, exponent forms, -0.0, numpy scalars, and j*dt rounding
+# (3 * 0.1 = 0.30000000000000004).
+
+class _Bead:
+    def __init__(self, label, r):
+        self.label, self.r = label, r
+
+
+def test_xyz_text_known_answer():
+    import io as pyio
+    beads = [_Bead("A", np.array([1.0, -2.5, 1e-20])), _Bead("Bq", np.array([123456789.125, -0.0, 0.1])),
+             _Bead("C", [np.float64(3.0), 7, 2.5e16])]
+    f = pyio.StringIO()
+    io.write_to_xyz_file(f, "traj.xyz", 3, 0.1, beads)
+    assert f.getvalue() == ("3\n"
+                            "traj.xyz time [ps] 0.30000000000000004\n"
+                            "A 1.0 -2.5 1e-20\n"
+                            "Bq 123456789.125 -0.0 0.1\n"
+                            "C 3.0 7 2.5e+16\n")
+
+
+def test_flux_text_known_answer():
+    import io as pyio
+    f = pyio.StringIO()
+    flux = {"XXX": 0, "Y": -3}
+    io.write_to_flux_file(f, 0, 0.5, flux)
+    io.write_to_flux_file(f, 1, 0.5, {"XXX": 2, "Y": -3})
+    io.write_to_flux_file(f, 7, 0.1, {"XXX": np.int64(5), "Y": 0})
+    assert f.getvalue() == ("time/ps XXX Y\n"
+                            "0.5 2 -3\n"
+                            "0.7000000000000001 5 0\n")
+
+
+def test_enr_text_known_answer():
+    import io as pyio
+    from pybrown_amd.bd import write_to_enr_file
+    f = pyio.StringIO()
+    write_to_enr_file(f, 0, 0.1, 0.0, "kcal/mol")
+    write_to_enr_file(f, 3, 0.1, -1.25e-05, "kcal/mol")
+    assert f.getvalue() == ("time/ps energy/kcal/mol\n"
+                            "0.0 0.0\n"
+                            "0.30000000000000004 -1.25e-05\n")
