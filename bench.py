@@ -108,6 +108,9 @@ def main():
                     help="c5: replay the factorisation and SPD-inverse launch sequences from HIP graphs")
     ap.add_argument("--no-c1-leg", action="store_true", help="c2: skip the small C1 leg")
     ap.add_argument("--no-served-leg", action="store_true", help="c2: skip the served request-stream sample")
+    ap.add_argument("--tune", action="store_true",
+                    help="c5: time the factorisation's launch shapes per order first (pybrown_amd.autotune) and "
+                         "serve with the tuned table")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (got --gpus %d)"
@@ -318,6 +321,12 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
                        order=order).run(reqs, realtime=False)
     burst = summary(burst_res)
     log("c5 served, all at start (%s order): %s" % (order, burst))
+    # the requests that set the window: the longest one at a time, and when
+    # each of them started and ended in the served batch
+    longest = sorted(range(len(reqs)), key=lambda i: -(plain[i].end - plain[i].start))[:5]
+    log("c5 longest alone (rid N mode: alone s | served start-end s): %s" % "; ".join(
+        "%d %d %s: %.2f | %.2f-%.2f" % (reqs[i].rid, reqs[i].N, reqs[i].mode, plain[i].end - plain[i].start,
+                                        burst_res[i].start, burst_res[i].end) for i in longest))
     rate = 1.5 * burst["sims_per_s"]
     for r, p in zip(reqs, stream_params(requests, rate=rate)):
         r.arrival = p["arrival"]
@@ -379,8 +388,16 @@ def serve_bench(args):
     os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or str(min(max(args.slots, 4), 32))
     import torch
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    tuned = None
+    if args.tune:
+        from pybrown_amd import autotune
+        res = autotune.tune(grid=[t for t in autotune.DEFAULT_GRID if t <= 160], reps=3,
+                            log=lambda m: print("tune " + m, file=sys.stderr))
+        autotune.apply(res)
+        tuned = {"grid": res["grid"], "codes": res["codes"], "seconds": res["seconds"], "device": res["device"]}
     leg = served_legs(args.requests, args.request_steps, args.slots, args.precision, args.graphs,
                       log=lambda m: print(m, file=sys.stderr))
+    leg["potrf_table"] = tuned
     out = {
         "metric": "served sims/s + p99 latency (request stream)",
         "value": leg["sims_per_s"],

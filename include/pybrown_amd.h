@@ -115,6 +115,27 @@ int pbx_get_potrf_variant(void);
  * bit either way.  For tests; off by default.  No counterpart in the
  * reference (numpy.linalg.cholesky has no such choice). */
 int pbx_set_potrf_chain_jobs(int on);
+/* Variant 6 launch shape (the factor is the same to the bit for every
+ * setting): orders of up to `tiles` tile columns (default 64) run one
+ * workgroup per compute unit, larger ones two.  Process-wide.  No
+ * counterpart in the reference. */
+int pbx_set_potrf_own_cu_tiles(int tiles);
+int pbx_get_potrf_own_cu_tiles(void);
+/* Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
+ * codes[T] for T tile columns (T < len <= 1024): 0 = the settings above,
+ * 4 = variant 4, 61 = variant 6 one workgroup per CU, 62 = variant 6 two
+ * per CU.  len = 0 clears the table.  The mixed-precision factor keeps
+ * variant 6.  No counterpart in the reference. */
+int pbx_set_potrf_table(const int* codes, int len);
+/* The variant and own-CU choice a factorisation of T tile columns uses. */
+int pbx_get_potrf_shape(int T, int* variant, int* own_cu);
+/* Milliseconds per factorisation (the copy of D into B plus the
+ * factorisation, as pbx_cholesky runs them on the device) of a synthetic
+ * diagonally dominant SPD matrix of order n, with the launch shapes in force;
+ * one untimed run, then the mean of `reps`.  For the tuner. */
+int pbx_time_potrf(int n, int reps, int mixed, double* ms);
+/* "<name> (<architecture>)" of the current device, NUL-terminated in out. */
+int pbx_device_name(char* out, int len);
 /* Test hook: the k-th launch sequence from now that reports through a
  * simulation's or call's status word (a factorisation, an SPD inverse's
  * factorisation, the midpoint's B0^T solve) ends with that word set as if

@@ -334,6 +334,69 @@ extern "C" int pbx_spd_inverse_mixed(const double* A, int n, double scale, int r
     return spd_inverse_host(A, n, scale, 1 + refine, X);
 }
 
+// "<name> (<arch>)" of the current device, for records of tuned settings.
+extern "C" int pbx_device_name(char* out, int len) {
+    pbx_clear_error();
+    if (!out || len <= 0) return PBX_ERR_ARG;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    PBX_HIP(hipGetDevice(&dev));
+    PBX_HIP(hipGetDeviceProperties(&prop, dev));
+    snprintf(out, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    return PBX_OK;
+}
+
+// Milliseconds per factorisation (pbx_cholesky's device part: the copy of D
+// into B and the factorisation) of a synthetic SPD matrix of order n, with
+// the launch shapes in force, averaged over `reps` after one untimed run.
+extern "C" int pbx_time_potrf(int n, int reps, int mixed, double* ms) {
+    pbx_clear_error();
+    if (n <= 0 || reps <= 0 || !ms || (mixed != 0 && mixed != 1)) {
+        pbx_set_error(PBX_ERR_ARG, "time_potrf: n %d reps %d mixed %d", n, reps, mixed);
+        return PBX_ERR_ARG;
+    }
+    const int np = (int)pbx_round_up(n, 64), ld = np;
+    DevBuf dD, dB, dW, dI;
+    TRY(dD.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dB.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dW.alloc(sizeof(double) * pbx_dense_work_doubles(np)));
+    TRY(dI.alloc(sizeof(int)));
+    hipStream_t st = nullptr;
+    PBX_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PBX_OK, info = 0;
+    float t = 0.f;
+    auto run = [&]() {
+        return pbx_potrf_copy(dD.as<double>(), dB.as<double>(), np, ld, dW.as<double>(), dI.as<int>(), st, mixed);
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemsetAsync(dI.p, 0, sizeof(int), st) != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "time_potrf: event or memset failed");
+        rc = PBX_ERR_HIP;
+    }
+    if (!rc) rc = pbx_launch_spd_fill(dD.as<double>(), n, np, ld, st);
+    if (!rc) rc = run();
+    if (!rc && hipEventRecord(e0, st) != hipSuccess) rc = PBX_ERR_HIP;
+    for (int r = 0; r < reps && !rc; ++r) rc = run();
+    if (!rc && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess ||
+                hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)) {
+        pbx_set_error(PBX_ERR_HIP, "time_potrf: timing failed");
+        rc = PBX_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    if (rc) return rc;
+    if (info) {
+        pbx_set_error(PBX_ERR_STATE, "time_potrf: factorisation status %d", info);
+        return PBX_ERR_STATE;
+    }
+    *ms = (double)t / reps;
+    return PBX_OK;
+}
+
 // C = A B^T, all dimensions multiples of 64 (checks the MFMA tile layout)
 extern "C" int pbx_gemm_nt(const double* A, const double* B, double* C, int M, int N, int K) {
     pbx_clear_error();
@@ -545,7 +608,7 @@ int check_info(PbxSim* s) {
 template <class F>
 int run_graphed(PbxSim* s, int op, const void* key, F&& body) {
     if (!s->graphs) return body();
-    const int variant = pbx_get_potrf_variant();
+    const int variant = pbx_potrf_config_generation();
     for (int q = 0; q < s->n_graphs; ++q) {
         const PbxSim::GraphSlot& g = s->graph_slots[q];
         if (g.op == op && g.key == key && g.variant == variant) {

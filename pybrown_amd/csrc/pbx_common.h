@@ -191,6 +191,11 @@ size_t pbx_dense_work_doubles(int np);
 // then one 64 x 64 scratch tile.
 constexpr int PBX_LA_INTS = 16;
 int pbx_potrf_two_col_ready(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s);
+// Bumped by every change of the factorisation's launch shapes (variant,
+// own-CU threshold, per-order table): captured launch sequences are keyed by it.
+int pbx_potrf_config_generation();
+// a diagonally dominant SPD matrix of order n (pad = identity), for timing
+int pbx_launch_spd_fill(double* A, int n, int np, int ld, hipStream_t s);
 
 // batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()
 // consecutive attempts (xi of attempt k in ring slot (ring_pos + k) mod

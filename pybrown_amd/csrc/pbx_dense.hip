@@ -1123,6 +1123,21 @@ __global__ void copy_lower_kernel(const double* src, double* dst, int np, int ld
     }
 }
 
+// A well-conditioned SPD matrix of order n in an np x np buffer (pad =
+// identity) for timing the factorisation: A_ij = 1 / (1 + |i - j|) off the
+// diagonal, n + 1 on it (strictly diagonally dominant).
+__global__ void spd_fill_kernel(double* A, int n, int np, int ld) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)np * np) return;
+    const int r = (int)(t / np), c = (int)(t % np);
+    double v;
+    if (r < n && c < n)
+        v = (r == c) ? (double)n + 1.0 : 1.0 / (1.0 + (double)abs(r - c));
+    else
+        v = (r == c) ? 1.0 : 0.0;
+    A[(size_t)r * ld + c] = v;
+}
+
 // C = A B^T for an M x K and an N x K matrix, all dims multiples of 64
 // (test entry for the MFMA tile path).
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const double* A, const double* B, double* C, int M, int N,
@@ -1159,14 +1174,75 @@ size_t pbx_dense_work_doubles(int np) { return dense_scratch_offset(np) + (size_
 constexpr int kDefaultPotrf = 6;
 static int g_potrf_variant = kDefaultPotrf;
 static int g_own_cu_tiles = 64;
+static std::atomic<int> g_shape_generation{0};  // launch-shape changes (keys captured graphs)
 static int g_chain_jobs = 0;
 extern "C" int pbx_set_potrf_chain_jobs(int on) {
     g_chain_jobs = on != 0;
     return PBX_OK;
 }
+// Largest tile-column count factored with one workgroup per compute unit
+// (default 64; larger orders keep two per CU).
+extern "C" int pbx_set_potrf_own_cu_tiles(int t) {
+    if (t < 0) return PBX_ERR_ARG;
+    g_own_cu_tiles = t;
+    ++g_shape_generation;
+    return PBX_OK;
+}
+extern "C" int pbx_get_potrf_own_cu_tiles(void) { return g_own_cu_tiles; }
+
+// Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
+// code[T] for a factorisation of T tile columns is 0 (the process-wide
+// variant and own-CU setting), 4 (variant 4), 61 (variant 6, one workgroup
+// per CU) or 62 (variant 6, two per CU).  The MIXED factor exists in the
+// two-column launches only and keeps variant 6.  Every setter bumps the
+// generation that keys captured graphs.
+constexpr int kShapeTableMax = 1024;
+static std::atomic<int> g_shape_table[kShapeTableMax];
+int pbx_potrf_config_generation() { return g_shape_generation.load(); }
+
+struct PotrfShape {
+    int variant;
+    bool own_cu;
+};
+static PotrfShape potrf_shape(int T, int mixed) {
+    PotrfShape sh{g_potrf_variant, T <= g_own_cu_tiles};
+    const int code = T < kShapeTableMax ? g_shape_table[T].load(std::memory_order_relaxed) : 0;
+    if (code == 4) sh.variant = 4;
+    if (code == 61 || code == 62) {
+        sh.variant = 6;
+        sh.own_cu = code == 61;
+    }
+    if (mixed) sh.variant = 6;
+    return sh;
+}
+
+extern "C" int pbx_set_potrf_table(const int* codes, int len) {
+    if (len < 0 || len > kShapeTableMax || (len > 0 && !codes)) {
+        pbx_set_error(PBX_ERR_ARG, "set_potrf_table: length %d outside [0, %d]", len, kShapeTableMax);
+        return PBX_ERR_ARG;
+    }
+    for (int t = 0; t < len; ++t)
+        if (codes[t] != 0 && codes[t] != 4 && codes[t] != 61 && codes[t] != 62) {
+            pbx_set_error(PBX_ERR_ARG, "set_potrf_table: code %d at %d (0, 4, 61 or 62)", codes[t], t);
+            return PBX_ERR_ARG;
+        }
+    for (int t = 0; t < kShapeTableMax; ++t) g_shape_table[t].store(t < len ? codes[t] : 0);
+    ++g_shape_generation;
+    return PBX_OK;
+}
+
+extern "C" int pbx_get_potrf_shape(int T, int* variant, int* own_cu) {
+    if (T <= 0 || !variant || !own_cu) return PBX_ERR_ARG;
+    const PotrfShape sh = potrf_shape(T, 0);
+    *variant = sh.variant;
+    *own_cu = sh.own_cu;
+    return PBX_OK;
+}
+
 extern "C" int pbx_set_potrf_variant(int v) {
     if (v < 0 || v > 6 || v == 3 || v == 5) return PBX_ERR_ARG;
     g_potrf_variant = v ? v : kDefaultPotrf;
+    ++g_shape_generation;
     return PBX_OK;
 }
 extern "C" int pbx_get_potrf_variant(void) { return g_potrf_variant; }
@@ -1214,7 +1290,7 @@ int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s
 // before, on the same stream).
 template <bool MIXED>
 static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s,
-                                 bool la_ready) {
+                                 bool la_ready, bool own_cu) {
     const int T = np / T64;
     int* la = (int*)(work + (size_t)np * T64);
     // Up to 64 tile columns the trailing updates of a launch finish well
@@ -1222,7 +1298,6 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // its own (dynamic LDS beyond half the CU's 160 KB): nothing shares the
     // chain's CU.  Larger orders are throughput-bound and keep two
     // workgroups per CU.
-    const bool own_cu = T <= g_own_cu_tiles;
     const size_t dyn_lds = own_cu ? 24576 : 0;
     if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
@@ -1256,10 +1331,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
 
 static void potrf_two_col(double* A, int np, int ld, double* work, int* info_dev, hipStream_t s, bool mixed,
                           bool la_ready = false) {
+    const bool own_cu = potrf_shape(np / T64, mixed).own_cu;
     if (mixed)
-        potrf_two_col_launch<true>(A, np, ld, work, info_dev, s, la_ready);
+        potrf_two_col_launch<true>(A, np, ld, work, info_dev, s, la_ready, own_cu);
     else
-        potrf_two_col_launch<false>(A, np, ld, work, info_dev, s, la_ready);
+        potrf_two_col_launch<false>(A, np, ld, work, info_dev, s, la_ready, own_cu);
 }
 
 // The two-column factorisation (variant 6, FP64) of A in place, its
@@ -1287,13 +1363,14 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         return PBX_ERR_ARG;
     }
     const int T = np / T64;
-    if (g_potrf_variant == 6 || mixed) {
+    const int variant = potrf_shape(T, mixed).variant;
+    if (variant == 6) {
         potrf_two_col(A, np, ld, work, info_dev, s, mixed != 0);
         pbx_status_sequence_queued(info_dev, s);
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
-    if (g_potrf_variant >= 4) {
+    if (variant >= 4) {
         int* la = (int*)(work + (size_t)np * T64);
         PBX_LAUNCH(potrf_diag_kernel<5>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, la);
         if (T > 1) PBX_LAUNCH(trsm_kernel, dim3(T - 1), dim3(256), 0, s, A, ld, 0, work);
@@ -1305,7 +1382,7 @@ int pbx_potrf(double* A, int np, int ld, double* work, int* info_dev, hipStream_
         PBX_HIP(hipGetLastError());
         return PBX_OK;
     }
-    const bool v2 = g_potrf_variant == 2;
+    const bool v2 = variant == 2;
     if (v2)
         PBX_LAUNCH(potrf_diag_kernel<2>, dim3(1), dim3(256), 0, s, A, ld, 0, work, info_dev, nullptr);
     else
@@ -1333,7 +1410,7 @@ int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int
         return PBX_ERR_ARG;
     }
     const int T = np / T64;
-    const bool two_col = g_potrf_variant == 6 || mixed;
+    const bool two_col = potrf_shape(T, mixed).variant == 6;
     int* la = two_col ? (int*)(work + (size_t)np * T64) : nullptr;
     PBX_LAUNCH(copy_lower_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, s, D, B, np, ld, la);
     if (!two_col) return pbx_potrf(B, np, ld, work, info_dev, s, mixed);
@@ -1403,6 +1480,13 @@ int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld,
     PBX_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
     PBX_HIP(hipStreamSynchronize(s));
     *rel = h[1] > 0 ? sqrt(h[0] / h[1]) : 0.0;
+    return PBX_OK;
+}
+
+int pbx_launch_spd_fill(double* A, int n, int np, int ld, hipStream_t s) {
+    const size_t total = (size_t)np * np;
+    PBX_LAUNCH(spd_fill_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, A, n, np, ld);
+    PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
