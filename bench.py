@@ -105,9 +105,11 @@ def main():
     ap.add_argument("--slots", type=int, default=16, help="c5: concurrent simulations")
     ap.add_argument("--request-steps", type=int, default=200, help="c5: steps per request")
     ap.add_argument("--graphs", action="store_true",
-                    help="c5: replay the factorisation and SPD-inverse launch sequences from HIP graphs")
+                    help="c5: HIP graphs: whole steps of the in-library runs (Ermak without overlap checks, RPY "
+                         "and Ewald), and the factorisation and SPD-inverse sequences elsewhere")
     ap.add_argument("--no-c1-leg", action="store_true", help="c2: skip the small C1 leg")
     ap.add_argument("--no-served-leg", action="store_true", help="c2: skip the served request-stream sample")
+    ap.add_argument("--max-n", type=int, default=0, help="c5: keep only the requests with N <= this")
     ap.add_argument("--tune", action="store_true",
                     help="c5: time the factorisation's launch shapes per order first (pybrown_amd.autotune) and "
                          "serve with the tuned table")
@@ -283,7 +285,7 @@ def mixed_accuracy(box, xyz, a, inp, dt, steps=3, precision="mixed"):
         mix.close()
 
 
-def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None, order="cost"):
+def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None, order="cost", max_n=None):
     """The served request stream on the first `requests` requests (request
     k: seed k; pybrown_amd.serving), `steps` steps each:
       plain   one at a time on the plain FP64 path (Box.propagate per step,
@@ -304,6 +306,8 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
     from pybrown_amd.serving import Server, make_requests_parallel, run_one, run_plain, stream_params, summary
     log = log or (lambda *a: None)
     params = stream_params(requests)
+    if max_n:  # the small-request regime: the head of the stream with N <= max_n
+        params = [p for p in params if p["N"] <= max_n]
     t = time.perf_counter()
     reqs = make_requests_parallel(params, steps=steps, workers=min(16, os.cpu_count() or 1))
     log("c5: %d requests placed in %.1f s (N %d..%d, %s)" % (
@@ -328,8 +332,9 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
         "%d %d %s: %.2f | %.2f-%.2f" % (reqs[i].rid, reqs[i].N, reqs[i].mode, plain[i].end - plain[i].start,
                                         burst_res[i].start, burst_res[i].end) for i in longest))
     rate = 1.5 * burst["sims_per_s"]
-    for r, p in zip(reqs, stream_params(requests, rate=rate)):
-        r.arrival = p["arrival"]
+    timed = {p["rid"]: p["arrival"] for p in stream_params(requests, rate=rate)}
+    for r in reqs:
+        r.arrival = timed[r.rid]
     res = Server(slots=slots, hbm_budget=budget, runner=runner, precision=precision).run(reqs, realtime=True)
     served = summary(res)
     log("c5 served, Poisson arrivals at %.3f/s: %s" % (rate, served))
@@ -365,7 +370,8 @@ def served_legs(requests, steps, slots, precision="fp64", graphs=False, log=None
                      "compared": len(errs)}
         log("c5 served %s against plain FP64: %s" % (precision, mixed_err))
     return {
-        "requests": requests, "steps_per_request": steps, "slots": slots, "precision": precision,
+        "requests": len(reqs), "stream_head": requests, "max_n": max_n, "steps_per_request": steps, "slots": slots,
+        "precision": precision, "graphs": graphs,
         "sims_per_s": served["sims_per_s"], "p50_s": served["p50_s"], "p99_s": served["p99_s"],
         "completed": served["completed"], "failed": served["failed"], "arrival_rate_per_s": rate,
         "plain_one_at_a_time_sims_per_s": base["sims_per_s"],
@@ -396,7 +402,7 @@ def serve_bench(args):
         autotune.apply(res)
         tuned = {"grid": res["grid"], "codes": res["codes"], "seconds": res["seconds"], "device": res["device"]}
     leg = served_legs(args.requests, args.request_steps, args.slots, args.precision, args.graphs,
-                      log=lambda m: print(m, file=sys.stderr))
+                      log=lambda m: print(m, file=sys.stderr), max_n=args.max_n or None)
     leg["potrf_table"] = tuned
     out = {
         "metric": "served sims/s + p99 latency (request stream)",

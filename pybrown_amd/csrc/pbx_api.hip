@@ -25,15 +25,16 @@
 #include <atomic>
 #include <mutex>
 #include <thread>
+#include <shared_mutex>
 #include <vector>
 
 // launchers from pbx_step.hip
 int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, double sq2dt, double mult,
                      double* pos, double* disp, int N, const double* plane_n, const double* plane_p, int flux_on,
-                     const int* label, long long* flux, hipStream_t s);
+                     const int* label, long long* flux, hipStream_t s, const PbxStepCursor* cur = nullptr);
 int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, double k, double mult, double* pos,
                    int N, const double* plane_n, const double* plane_p, int flux_on, const int* label,
-                   long long* flux, hipStream_t s);
+                   long long* flux, hipStream_t s, const PbxStepCursor* cur = nullptr);
 int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
                          const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s);
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s);
@@ -444,6 +445,14 @@ struct PbxSim {
     int graphs = 0;
     int n_graphs = 0;
     GraphSlot graph_slots[8];
+    // whole-step graphs of queued runs (graphs on, Ermak without overlap
+    // checks, RPY or Smith-Ewald): one per (builds D, factors) pair and time
+    // step, replayed with the step's noise slot and flux slot read from the
+    // cursor, which the graph's last node advances
+    PbxStepCursor* cursor_dev = nullptr;
+    PbxStepCursor* cursor_host = nullptr;  // pinned, written once per run
+    int capturing = 0;                     // a whole step is being captured
+    double step_graph_dt = 0.0;
     // midpoint: per-tile-row completion flags of the B0^T solve and its row
     // ticket counter (zeroed at create, flags written with a per-call epoch;
     // not part of a snapshot)
@@ -529,6 +538,21 @@ struct PbxSim {
 
 namespace {
 
+// Graph captures (any simulation's, on its own stream) hold this shared;
+// device frees, which the runtime may order against every stream, hold it
+// exclusively, so no free lands inside another thread's capture.
+std::shared_mutex g_capture_mu;
+
+// Synchronous copies of a simulation go through its own (non-blocking)
+// stream, never the legacy one: another thread may be capturing a graph on
+// its stream, and the runtime refuses legacy-stream work while any capture is
+// in progress.
+hipError_t sim_copy(PbxSim* s, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s->stream);
+}
+
 int sim_alloc(PbxSim* s, void** p, size_t bytes) {
     if (hipMalloc(p, bytes) != hipSuccess) {
         pbx_set_error(PBX_ERR_OOM, "hipMalloc(%zu) failed", bytes);
@@ -607,7 +631,7 @@ int check_info(PbxSim* s) {
 // falls back to eager launches.
 template <class F>
 int run_graphed(PbxSim* s, int op, const void* key, F&& body) {
-    if (!s->graphs) return body();
+    if (!s->graphs || s->capturing) return body();  // (inside a whole-step capture: part of it)
     const int variant = pbx_potrf_config_generation();
     for (int q = 0; q < s->n_graphs; ++q) {
         const PbxSim::GraphSlot& g = s->graph_slots[q];
@@ -617,10 +641,12 @@ int run_graphed(PbxSim* s, int op, const void* key, F&& body) {
         }
     }
     if (s->n_graphs == (int)(sizeof(s->graph_slots) / sizeof(s->graph_slots[0]))) return body();
+    std::shared_lock<std::shared_mutex> capture_lock(g_capture_mu);
     PBX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     const int rc = body();
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    capture_lock.unlock();
     if (rc || e != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         if (rc) return rc;
@@ -644,7 +670,7 @@ void drop_graphs(PbxSim* s) {
     s->n_graphs = 0;
 }
 
-enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3 };
+enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_STEP = 4 };  // GRAPH_STEP + 0..3
 
 // box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
 int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
@@ -804,19 +830,20 @@ struct StepCtx {
     int flux_on;
     long long* flux;  // per-label counters the step's moves add their crossings to
     int queued;       // steps queued back to back: no per-step status copies
+    const PbxStepCursor* cur = nullptr;  // captured step: noise and flux slots from here
 };
 
 int stoch(PbxSim* s, const double* Bm, const double* xi, double mult, StepCtx& c) {
     int diag = s->cfg.hydrodynamics == PBX_NOHI;
     return pbx_launch_stoch(diag ? s->Bd : Bm, s->ld, diag, xi, c.sq2dt, mult, s->pos, s->disp, s->N, c.pn, c.pp,
-                            c.flux_on, s->label, c.flux, s->stream);
+                            c.flux_on, s->label, c.flux, s->stream, c.cur);
 }
 
 int det(PbxSim* s, const double* Dm, double dt, double mult, StepCtx& c) {
     if (!s->F_nonzero) return PBX_OK;  // translating by D.0 leaves positions unchanged
     int diag = s->cfg.hydrodynamics == PBX_NOHI;
     return pbx_launch_det(diag ? s->Dd : Dm, s->ld, diag, s->F, dt / s->kBT, mult, s->pos, s->N, c.pn, c.pp,
-                          c.flux_on, s->label, c.flux, s->stream);
+                          c.flux_on, s->label, c.flux, s->stream, c.cur);
 }
 
 int undo(PbxSim* s, StepCtx& c) {
@@ -860,8 +887,8 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         if (!c.queued) TRY(post_info(s));
     }
     if (!s->cfg.check_overlaps) {
-        const double* xi;
-        TRY(next_xi(s, &xi));
+        const double* xi = s->xi_dev;  // a captured step reads its slot from the cursor; the
+        if (!c.cur) TRY(next_xi(s, &xi));  // host makes it resident and consumes it around the replay
         TRY(stoch(s, s->B, xi, 1.0, c));
         s->last_attempts = 1;
         return PBX_OK;
@@ -1089,8 +1116,8 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             F[3 * i + c] = cfg->external_force[c];
             if (cfg->external_force[c] != 0.0) s->F_nonzero = 1;
         }
-    if (!hip_ok(hipMemcpy(s->label, lab.data(), sizeof(int) * N, hipMemcpyHostToDevice), "labels") ||
-        !hip_ok(hipMemcpy(s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice), "F"))
+    if (!hip_ok(sim_copy(s, s->label, lab.data(), sizeof(int) * N, hipMemcpyHostToDevice), "labels") ||
+        !hip_ok(sim_copy(s, s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice), "F"))
         return fail(PBX_ERR_HIP);
 
     if (cfg->hydrodynamics == PBX_NOHI) {
@@ -1104,8 +1131,8 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             }
         if ((rc = sim_alloc(s, (void**)&s->Dd, sizeof(double) * n3))) return fail(rc);
         if ((rc = sim_alloc(s, (void**)&s->Bd, sizeof(double) * n3))) return fail(rc);
-        if (!hip_ok(hipMemcpy(s->Dd, Dd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Dd") ||
-            !hip_ok(hipMemcpy(s->Bd, Bd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Bd"))
+        if (!hip_ok(sim_copy(s, s->Dd, Dd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Dd") ||
+            !hip_ok(sim_copy(s, s->Bd, Bd.data(), sizeof(double) * n3, hipMemcpyHostToDevice), "Bd"))
             return fail(PBX_ERR_HIP);
         s->have_D = s->have_B = 1;
     } else {
@@ -1146,7 +1173,7 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
             m[(size_t)i * Nt + j] = m[(size_t)j * Nt + i] = 1;
         }
         if ((rc = sim_alloc(s, (void**)&s->ovl_mask, sizeof(int) * m.size()))) return fail(rc);
-        if (!hip_ok(hipMemcpy(s->ovl_mask, m.data(), sizeof(int) * m.size(), hipMemcpyHostToDevice), "mask"))
+        if (!hip_ok(sim_copy(s, s->ovl_mask, m.data(), sizeof(int) * m.size(), hipMemcpyHostToDevice), "mask"))
             return fail(PBX_ERR_HIP);
     }
     if (!hip_ok(hipStreamSynchronize(s->stream), "create sync")) return fail(PBX_ERR_HIP);
@@ -1173,12 +1200,16 @@ extern "C" int pbx_sim_destroy(pbx_sim* s) {
     for (int k = 0; k < 3; ++k)
         if (s->xi_ev[k]) (void)hipEventDestroy(s->xi_ev[k]);
     drop_graphs(s);
-    for (void* p : s->owned) (void)hipFree(p);
+    {
+        std::unique_lock<std::shared_mutex> free_lock(g_capture_mu);
+        for (void* p : s->owned) (void)hipFree(p);
+    }
     if (s->xi_host) (void)hipHostFree(s->xi_host);
     if (s->flag_host) (void)hipHostFree(s->flag_host);
     if (s->pos_host) (void)hipHostFree(s->pos_host);
     if (s->flux_host) (void)hipHostFree(s->flux_host);
     if (s->E_host) (void)hipHostFree(s->E_host);
+    if (s->cursor_host) (void)hipHostFree(s->cursor_host);
     if (s->rd_ev) (void)hipEventDestroy(s->rd_ev);
     if (s->up) (void)hipStreamDestroy(s->up);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -1245,9 +1276,9 @@ int ring_copy(PbxSim* s, long long a0, long long k, double* host, bool to_host) 
         double* dev = s->xi_dev + (size_t)(a % (2 * s->xi_cap)) * s->n;
         double* h = host + (size_t)(a - a0) * s->n;
         if (to_host)
-            PBX_HIP(hipMemcpy(h, dev, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+            PBX_HIP(sim_copy(s, h, dev, sizeof(double) * s->n, hipMemcpyDeviceToHost));
         else
-            PBX_HIP(hipMemcpy(dev, h, sizeof(double) * s->n, hipMemcpyHostToDevice));
+            PBX_HIP(sim_copy(s, dev, h, sizeof(double) * s->n, hipMemcpyHostToDevice));
     }
     return PBX_OK;
 }
@@ -1294,9 +1325,9 @@ extern "C" int pbx_sim_snapshot(pbx_sim* s, void* buf, long long bytes) {
     char* p = (char*)buf;
     memcpy(p, &h, sizeof(h));
     p += sizeof(h);
-    PBX_HIP(hipMemcpy(p, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    PBX_HIP(sim_copy(s, p, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost));
     p += sizeof(double) * s->n;
-    PBX_HIP(hipMemcpy(p, s->flux, sizeof(long long) * h.n_labels, hipMemcpyDeviceToHost));
+    PBX_HIP(sim_copy(s, p, s->flux, sizeof(long long) * h.n_labels, hipMemcpyDeviceToHost));
     p += sizeof(long long) * h.n_labels;
     TRY(ring_copy(s, s->xi_used, h.pending, (double*)p, true));
     p += sizeof(double) * h.pending * s->n;
@@ -1307,10 +1338,10 @@ extern "C" int pbx_sim_snapshot(pbx_sim* s, void* buf, long long bytes) {
     double* mats[3] = {s->D, s->B, s->Rff};
     const size_t mat = sizeof(double) * (size_t)s->np * s->ld;
     for (int m = 0; m < h.nmat; ++m) {
-        PBX_HIP(hipMemcpy(p, mats[m], mat, hipMemcpyDeviceToHost));
+        PBX_HIP(sim_copy(s, p, mats[m], mat, hipMemcpyDeviceToHost));
         p += mat;
     }
-    if (h.wdoubles) PBX_HIP(hipMemcpy(p, s->Wchol, sizeof(double) * h.wdoubles, hipMemcpyDeviceToHost));
+    if (h.wdoubles) PBX_HIP(sim_copy(s, p, s->Wchol, sizeof(double) * h.wdoubles, hipMemcpyDeviceToHost));
     return PBX_OK;
 }
 
@@ -1352,9 +1383,9 @@ extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
     s->xi_user = user;
     s->native = h.native;
     const char* p = (const char*)buf + sizeof(h);
-    PBX_HIP(hipMemcpy(s->pos, p, sizeof(double) * s->n, hipMemcpyHostToDevice));
+    PBX_HIP(sim_copy(s, s->pos, p, sizeof(double) * s->n, hipMemcpyHostToDevice));
     p += sizeof(double) * s->n;
-    PBX_HIP(hipMemcpy(s->flux, p, sizeof(long long) * nl, hipMemcpyHostToDevice));
+    PBX_HIP(sim_copy(s, s->flux, p, sizeof(long long) * nl, hipMemcpyHostToDevice));
     p += sizeof(long long) * nl;
     s->xi_used = h.xi_used;
     s->xi_taken = h.xi_taken;
@@ -1370,11 +1401,11 @@ extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
     double* mats[3] = {s->D, s->B, s->Rff};
     const size_t mat = sizeof(double) * (size_t)s->np * s->ld;
     for (int m = 0; m < h.nmat; ++m) {
-        PBX_HIP(hipMemcpy(mats[m], p, mat, hipMemcpyHostToDevice));
+        PBX_HIP(sim_copy(s, mats[m], p, mat, hipMemcpyHostToDevice));
         p += mat;
     }
     if (h.wdoubles && s->Wchol)
-        PBX_HIP(hipMemcpy(s->Wchol, p, sizeof(double) * std::min<long long>(h.wdoubles, pbx_dense_work_doubles(s->np)),
+        PBX_HIP(sim_copy(s, s->Wchol, p, sizeof(double) * std::min<long long>(h.wdoubles, pbx_dense_work_doubles(s->np)),
                           hipMemcpyHostToDevice));
     s->have_D = h.have_D;
     s->have_B = h.have_B;
@@ -1539,7 +1570,8 @@ namespace {
 // midpoint move with no overlap test, then the wrap.  Crossings add into
 // flux_slot (zero beforehand).  The same kernels as pbx_sim_propagate with
 // check_overlaps off, so the same bits.
-int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* flux_slot) {
+int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* flux_slot,
+                const PbxStepCursor* cur = nullptr) {
     const pbx_config& cf = s->cfg;
     StepCtx c;
     c.sq2dt = sqrt(2 * dt);
@@ -1548,6 +1580,7 @@ int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* f
     c.flux_on = cf.measure_flux;
     c.flux = flux_slot;
     c.queued = 1;
+    c.cur = cur;
     if (cf.hydrodynamics != PBX_NOHI) {
         if (bDff) {
             TRY(build_Dff(s, s->D, s->Rff));
@@ -1582,6 +1615,71 @@ int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* f
     return pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream);
 }
 
+// Whole steps replay from graphs when graphs are on and the step is a fixed
+// launch sequence: Ermak without overlap checks (one attempt per step, no
+// host decision) in the RPY or Smith-Ewald modes without the divergence
+// term.
+bool step_graphs_apply(const PbxSim* s) {
+    const pbx_config& cf = s->cfg;
+    return s->graphs && cf.scheme == PBX_ERMAK && !cf.check_overlaps && !cf.divergence_term &&
+           (cf.hydrodynamics == PBX_RPY || cf.hydrodynamics == PBX_RPY_SMITH);
+}
+
+// Replay (capturing on first use) the step with flags (bDff, chol) at time
+// step dt; the noise slot the cursor points at is resident.
+int step_graph_launch(PbxSim* s, double dt, int bDff, int chol) {
+    if (s->step_graph_dt != dt) {  // dt is baked into the captured launches
+        PBX_HIP(hipStreamSynchronize(s->stream));
+        int k = 0;
+        for (int q = 0; q < s->n_graphs; ++q) {
+            if (s->graph_slots[q].op >= GRAPH_STEP)
+                (void)hipGraphExecDestroy(s->graph_slots[q].exec);
+            else
+                s->graph_slots[k++] = s->graph_slots[q];
+        }
+        s->n_graphs = k;
+        s->step_graph_dt = dt;
+    }
+    const int op = GRAPH_STEP + 2 * (bDff != 0) + (chol != 0);
+    const int gen = pbx_potrf_config_generation();
+    for (int q = 0; q < s->n_graphs; ++q) {
+        const PbxSim::GraphSlot& g = s->graph_slots[q];
+        if (g.op == op && g.key == (const void*)s->cursor_dev && g.variant == gen) {
+            PBX_HIP(hipGraphLaunch(g.exec, s->stream));
+            return PBX_OK;
+        }
+    }
+    auto body = [&]() {
+        TRY(queued_step(s, dt, bDff, bDff, chol, nullptr, s->cursor_dev));
+        return pbx_launch_cursor_advance(s->cursor_dev, s->stream);
+    };
+    if (s->n_graphs == (int)(sizeof(s->graph_slots) / sizeof(s->graph_slots[0]))) return body();
+    std::shared_lock<std::shared_mutex> capture_lock(g_capture_mu);
+    PBX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    s->capturing = 1;
+    const int rc = body();
+    s->capturing = 0;
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s->stream, &graph);
+    capture_lock.unlock();
+    if (rc || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc) return rc;
+        pbx_set_error(PBX_ERR_HIP, "step graph capture: %s", hipGetErrorString(e));
+        return PBX_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "step graph instantiate: %s", hipGetErrorString(ei));
+        return PBX_ERR_HIP;
+    }
+    s->graph_slots[s->n_graphs++] = {op, (const void*)s->cursor_dev, gen, exec};
+    PBX_HIP(hipGraphLaunch(exec, s->stream));
+    return PBX_OK;
+}
+
 // Steps j0 .. j0+steps-1 queued back to back (pbx_sim_run when no step
 // needs a host decision): one flux slot per step, zeroed once; the status
 // word is copied every 32 steps and read one copy later, so the host stays up
@@ -1594,7 +1692,10 @@ int run_queued(PbxSim* s, double dt, long long j0, long long steps, int diff_fre
     if (need > s->flux_log_cap) {
         if (s->flux_log) {
             PBX_HIP(hipStreamSynchronize(s->stream));
-            PBX_HIP(hipFree(s->flux_log));
+            {
+                std::unique_lock<std::shared_mutex> free_lock(g_capture_mu);
+                PBX_HIP(hipFree(s->flux_log));
+            }
             s->owned.erase(std::find(s->owned.begin(), s->owned.end(), (void*)s->flux_log));
             s->device_bytes -= sizeof(long long) * (size_t)s->flux_log_cap;
             s->flux_log = nullptr;
@@ -1604,11 +1705,50 @@ int run_queued(PbxSim* s, double dt, long long j0, long long steps, int diff_fre
     }
     PBX_HIP(hipMemsetAsync(s->flux_log, 0, sizeof(long long) * (size_t)need, s->stream));
     s->spec_ok = 0;  // D is assembled at the start of each step here
+    const bool graphed = step_graphs_apply(s);
+    if (graphed) {
+        if (!s->cursor_dev) {
+            TRY(sim_alloc(s, (void**)&s->cursor_dev, sizeof(PbxStepCursor)));
+            if (hipHostMalloc((void**)&s->cursor_host, sizeof(PbxStepCursor), hipHostMallocDefault) != hipSuccess) {
+                s->cursor_host = nullptr;
+                pbx_set_error(PBX_ERR_OOM, "hipHostMalloc failed");
+                return PBX_ERR_OOM;
+            }
+        }
+        // the previous run ended in a stream synchronisation: the pinned
+        // struct is free to rewrite
+        *s->cursor_host = {0, (long long)ring_pos(s), (long long)ring_attempts(s), (long long)s->n, (long long)nl,
+                           s->xi_dev, s->flux_log};
+        PBX_HIP(hipMemcpyAsync(s->cursor_dev, s->cursor_host, sizeof(PbxStepCursor), hipMemcpyHostToDevice,
+                               s->stream));
+    }
     PBX_HIP(hipEventRecord(s->done_ev, s->stream));
     bool posted = false;
     for (long long t = 0; t < steps; ++t) {
         const long long j = j0 + t;
-        TRY(queued_step(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0, s->flux_log + t * nl));
+        if (graphed) {
+            // the step's draws resident (uploads are queued on the stream
+            // ahead of the replay), the replay, then the draws consumed
+            const int bD = j % diff_freq == 0, ch = j % chol_freq == 0;
+            if (bD) s->have_D = 1;
+            if (ch) {
+                if (!s->have_D) {
+                    pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
+                    return PBX_ERR_STATE;
+                }
+                s->have_B = 1;
+            }
+            if (!s->have_B) {
+                pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
+                return PBX_ERR_STATE;
+            }
+            TRY(noise_ensure(s, 1));
+            TRY(step_graph_launch(s, dt, bD, ch));
+            noise_consume(s, 1);
+            s->last_attempts = 1;
+        } else {
+            TRY(queued_step(s, dt, j % diff_freq == 0, j % lub_freq == 0, j % chol_freq == 0, s->flux_log + t * nl));
+        }
         if ((t + 1) % 32 == 0) {
             if (posted) {  // the copy queued 32 steps ago: read it
                 PBX_HIP(hipEventSynchronize(s->done_ev));
@@ -1736,6 +1876,9 @@ extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
         region |= a.reg_on[c];
     }
     s->forces_set = 1;
+    // captured steps were recorded without the force launches
+    PBX_HIP(hipStreamSynchronize(s->stream));
+    drop_graphs(s);
     s->forces_on = a.lj || f->n_bonds || f->n_angles || region;
     if (!s->forces_on) {
         // constant F = unit * Fex on every bead (box.py:459-460, 414-428)
@@ -1746,13 +1889,13 @@ extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
                 F[3 * i + c] = f->unit == 1.0 ? a.fex[c] : f->unit * a.fex[c];
                 if (F[3 * i + c] != 0.0) s->F_nonzero = 1;
             }
-        PBX_HIP(hipMemcpy(s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice));
+        PBX_HIP(sim_copy(s, s->F, F.data(), sizeof(double) * s->np, hipMemcpyHostToDevice));
         return PBX_OK;
     }
     s->F_nonzero = 1;
     auto upload = [&](void** dst, const void* src, size_t bytes) -> int {
         TRY(sim_alloc(s, dst, bytes ? bytes : 8));
-        if (bytes) PBX_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        if (bytes) PBX_HIP(sim_copy(s, *dst, src, bytes, hipMemcpyHostToDevice));
         return PBX_OK;
     };
     if (a.lj) {
@@ -1899,7 +2042,7 @@ extern "C" int pbx_sim_get_diag(pbx_sim* s, int which, double* out) {
         pbx_set_error(PBX_ERR_ARG, "diagonal D/B exist only for hydrodynamics nohi");
         return PBX_ERR_ARG;
     }
-    PBX_HIP(hipMemcpy(out, src, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+    PBX_HIP(sim_copy(s, out, src, sizeof(double) * s->n, hipMemcpyDeviceToHost));
     return PBX_OK;
 }
 

@@ -197,6 +197,20 @@ int pbx_potrf_config_generation();
 // a diagonally dominant SPD matrix of order n (pad = identity), for timing
 int pbx_launch_spd_fill(double* A, int n, int np, int ld, hipStream_t s);
 
+// Per-step indices of a step replayed from a captured graph: the kernels
+// that take the step's noise vector or flux slot read them here, and the
+// step's last node advances `step`.  Device memory, one per simulation.
+struct PbxStepCursor {
+    long long step;       // steps since the run started
+    long long ring_base;  // noise ring slot of the run's first step (one attempt per step)
+    long long ring_att;   // slots in the noise ring
+    long long n;          // draws per slot (3N)
+    long long nl;         // flux counters per step
+    const double* xi_dev;
+    long long* flux_log;
+};
+int pbx_launch_cursor_advance(PbxStepCursor* cur, hipStream_t s);
+
 // batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()
 // consecutive attempts (xi of attempt k in ring slot (ring_pos + k) mod
 // ring_att), their overlap flags, then the selection of the first accepted

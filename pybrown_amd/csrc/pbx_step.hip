@@ -56,9 +56,19 @@ __device__ __forceinline__ void move_bead(double* pos, const double* v, int b, c
 
 // Stochastic sub-step (box.py:366-375): BX = ((B xi) * sqrt(2 dt)) * mult,
 // B lower-triangular.  One 192-thread workgroup per bead (a wave per row).
-__global__ __launch_bounds__(192) void stoch_kernel(const double* __restrict__ B, int ld, const double* __restrict__ xi,
+// A replayed step's noise vector and flux slot (cursor non-null).
+__device__ __forceinline__ void cursor_pointers(const PbxStepCursor* cur, const double*& xi, long long*& flux) {
+    if (!cur) return;
+    const long long st = cur->step;
+    xi = cur->xi_dev + ((cur->ring_base + st) % cur->ring_att) * cur->n;
+    flux = cur->flux_log + st * cur->nl;
+}
+
+__global__ __launch_bounds__(192) void stoch_kernel(const double* __restrict__ B, int ld, const double* xi,
                                                     double sq2dt, double mult, double* pos, double* disp,
-                                                    FluxPlane f, const int* label, long long* flux) {
+                                                    FluxPlane f, const int* label, long long* flux,
+                                                    const PbxStepCursor* cur) {
+    cursor_pointers(cur, xi, flux);
     __shared__ double v[3];
     const int b = blockIdx.x;
     const int w = threadIdx.x >> 6;
@@ -74,9 +84,10 @@ __global__ __launch_bounds__(192) void stoch_kernel(const double* __restrict__ B
 }
 
 // nohi variant: B is the element-wise sqrt of the diagonal D (box.py:369)
-__global__ void stoch_diag_kernel(const double* __restrict__ Bd, const double* __restrict__ xi, double sq2dt,
+__global__ void stoch_diag_kernel(const double* __restrict__ Bd, const double* xi, double sq2dt,
                                   double mult, double* pos, double* disp, int N, FluxPlane f, const int* label,
-                                  long long* flux) {
+                                  long long* flux, const PbxStepCursor* cur) {
+    cursor_pointers(cur, xi, flux);
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
     double v[3];
@@ -91,7 +102,9 @@ __global__ void stoch_diag_kernel(const double* __restrict__ Bd, const double* _
 // D full symmetric; one wave per row.
 __global__ __launch_bounds__(192) void det_kernel(const double* __restrict__ D, int ld, const double* __restrict__ F,
                                                   double k, double mult, int n, double* pos, FluxPlane f,
-                                                  const int* label, long long* flux) {
+                                                  const int* label, long long* flux, const PbxStepCursor* cur) {
+    const double* unused_xi = nullptr;
+    cursor_pointers(cur, unused_xi, flux);
     __shared__ double v[3];
     const int b = blockIdx.x;
     const int w = threadIdx.x >> 6;
@@ -107,7 +120,10 @@ __global__ __launch_bounds__(192) void det_kernel(const double* __restrict__ D, 
 }
 
 __global__ void det_diag_kernel(const double* __restrict__ Dd, const double* __restrict__ F, double k, double mult,
-                                int N, double* pos, FluxPlane f, const int* label, long long* flux) {
+                                int N, double* pos, FluxPlane f, const int* label, long long* flux,
+                                const PbxStepCursor* cur) {
+    const double* unused_xi = nullptr;
+    cursor_pointers(cur, unused_xi, flux);
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= N) return;
     double v[3];
@@ -515,7 +531,7 @@ struct PbxFlux {
 
 int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, double sq2dt, double mult,
                      double* pos, double* disp, int N, const double* plane_n, const double* plane_p, int flux_on,
-                     const int* label, long long* flux, hipStream_t s) {
+                     const int* label, long long* flux, hipStream_t s, const PbxStepCursor* cur) {
     FluxPlane f;
     for (int c = 0; c < 3; ++c) {
         f.n[c] = plane_n ? plane_n[c] : 0.0;
@@ -524,17 +540,17 @@ int pbx_launch_stoch(const double* B, int ld, int diag_only, const double* xi, d
     f.on = flux_on;
     if (diag_only)
         PBX_LAUNCH(stoch_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, B, xi, sq2dt, mult, pos, disp,
-                           N, f, label, flux);
+                           N, f, label, flux, cur);
     else
         PBX_LAUNCH(stoch_kernel, dim3(N), dim3(192), 0, s, B, ld, xi, sq2dt, mult, pos, disp, f, label,
-                           flux);
+                           flux, cur);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
 int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, double k, double mult, double* pos,
                    int N, const double* plane_n, const double* plane_p, int flux_on, const int* label,
-                   long long* flux, hipStream_t s) {
+                   long long* flux, hipStream_t s, const PbxStepCursor* cur) {
     FluxPlane f;
     for (int c = 0; c < 3; ++c) {
         f.n[c] = plane_n ? plane_n[c] : 0.0;
@@ -543,9 +559,19 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
     f.on = flux_on;
     if (diag_only)
         PBX_LAUNCH(det_diag_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, F, k, mult, N, pos, f, label,
-                           flux);
+                           flux, cur);
     else
-        PBX_LAUNCH(det_kernel, dim3(N), dim3(192), 0, s, D, ld, F, k, mult, 3 * N, pos, f, label, flux);
+        PBX_LAUNCH(det_kernel, dim3(N), dim3(192), 0, s, D, ld, F, k, mult, 3 * N, pos, f, label, flux, cur);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+__global__ void cursor_advance_kernel(PbxStepCursor* cur) {
+    if (threadIdx.x == 0) cur->step += 1;
+}
+
+int pbx_launch_cursor_advance(PbxStepCursor* cur, hipStream_t s) {
+    PBX_LAUNCH(cursor_advance_kernel, dim3(1), dim3(64), 0, s, cur);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -365,6 +365,55 @@ def test_graph_replays_match_eager(case):
         assert np.array_equal(p0, p1) and f0 == f1 and n0 == n1, "step %d" % j
 
 
+@pytest.mark.parametrize("case", ["rpy_smith", "rpy_open_box_force"])
+def test_whole_step_graphs_match_eager_queued_runs(case):
+    """In-library runs (Box.run) whose steps need no host decision replay
+    whole steps from HIP graphs when graphs are on: one graph per (builds D,
+    factors) pair, the step's noise slot and flux slot read from a device
+    cursor the graph advances.  Against the same runs with graphs off,
+    positions, per-step flux and draws agree to the bit, across build and
+    factor cadences, noise chunk uploads every 5 steps, a change of time
+    step, an eager step between runs and (second case) an external force
+    (the deterministic sub-step in the graph).  Replays launch no kernels from
+    the host, so the launch counter shows the graphs were used."""
+    from pybrown_amd._lib import lib
+    from pybrown_amd.serving import OPEN_BOX
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(120, 10.0)
+    Lb = box_for(a, 0.05)
+    xyz = rsa_positions(a, Lb, 6)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 4, "hydrodynamics": "rpy_smith",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 2, "ewald_imag": 2, "measure_flux": FLUX,
+           "check_overlaps": False}
+    if case == "rpy_open_box_force":
+        inp = dict(inp, hydrodynamics="rpy", box_length=OPEN_BOX, external_force=[0.5, -0.25, 0.0])
+
+    def runs(graphs):
+        box = _gpu_box(xyz, a, inp, graphs=graphs, xi_pool_attempts=5)
+        out = []
+        k0 = lib.pbx_kernel_launches()
+        f, _ = box.run(1.0, 23, diff_freq=2, chol_freq=3)
+        out.append((box._pos.copy(), f.copy(), box.draw_count))
+        f, _ = box.run(1.0, 11, j0=23)
+        out.append((box._pos.copy(), f.copy(), box.draw_count))
+        f, _ = box.run(0.5, 7, j0=34, diff_freq=2, chol_freq=2)
+        out.append((box._pos.copy(), f.copy(), box.draw_count))
+        launches = lib.pbx_kernel_launches() - k0
+        box.propagate(1.0)
+        out.append((box._pos.copy(), np.array([box.net_flux["XXX"]]), box.draw_count))
+        f, _ = box.run(1.0, 9, j0=42, diff_freq=3, chol_freq=3)
+        out.append((box._pos.copy(), f.copy(), box.draw_count))
+        box.close()
+        return out, launches
+
+    eager, n_eager = runs(False)
+    graph, n_graph = runs(True)
+    print("%s: launches over 41 steps: eager %d, graphs %d" % (case, n_eager, n_graph))
+    for k, ((p0, f0, d0), (p1, f1, d1)) in enumerate(zip(eager, graph)):
+        assert np.array_equal(p0, p1) and np.array_equal(f0, f1) and d0 == d1, "call %d" % k
+    assert n_graph < n_eager / 3
+
+
 @pytest.mark.parametrize("hydro", ["rpy", "rpy_smith"])
 def test_lookahead_assembly_matches_plain_steps(hydro):
     """Steps with the look-ahead assembly of D (on by default: a step that
