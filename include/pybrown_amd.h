@@ -115,10 +115,13 @@ int pbx_get_potrf_variant(void);
  * bit either way.  For tests; off by default.  No counterpart in the
  * reference (numpy.linalg.cholesky has no such choice). */
 int pbx_set_potrf_chain_jobs(int on);
-/* Variant 6 launch shape (the factor is the same to the bit for every
- * setting): orders of up to `tiles` tile columns (default 64) run one
- * workgroup per compute unit, larger ones two.  Process-wide.  No
- * counterpart in the reference. */
+/* Variant 6 launch shape: orders of up to `tiles` tile columns (default
+ * 64) run one workgroup per compute unit and update every trailing tile in
+ * every launch (rank-128); larger ones run two per CU and alternate
+ * updates of the next two tile columns with rank-256 updates of the whole
+ * trailing matrix, which halves the trailing matrix's HBM traffic.  The
+ * factors agree to rounding.  Process-wide.  No counterpart in the
+ * reference. */
 int pbx_set_potrf_own_cu_tiles(int tiles);
 int pbx_get_potrf_own_cu_tiles(void);
 /* Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):

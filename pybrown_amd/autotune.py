@@ -13,9 +13,10 @@ fastest, each order between grid points taking its nearest grid point's
 (in log order).  A shape replaces the default only when it is faster by more
 than `margin`, so timing noise does not flip choices.
 
-Variant 6 with one or two workgroups per CU gives the same factor to the bit;
-variant 4 differs from variant 6 in the last bits (its trailing updates
-accumulate rank-64 instead of rank-128 sums), within the parity tolerance.
+The shapes differ in the last bits of the factor (variant 4 accumulates
+rank-64 trailing updates; variant 6 with two workgroups per CU alternates
+rank-128 and rank-256 ones, with one per CU it applies rank-128 ones), all
+within the parity tolerance.
 The table is process-wide: simulations run alone and in the mix use the same
 shapes, so served results stay identical to one-at-a-time ones under a fixed
 table.

@@ -402,25 +402,29 @@ __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int
                                              double* sB, v4d S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
-    if (!LEAN && P == 2) {
-        // both panels' operand tiles are requested before the first product,
-        // so their latencies overlap each other and the first product
-        v2d a0[8], a1[8], b0[8], b1[8];
-        stage_fetch(a0, Ai + (size_t)m0 * T64, ld);
-        stage_fetch(a1, Ai + (size_t)(m0 + 1) * T64, ld);
-        if (i != j) {
-            stage_fetch(b0, Aj + (size_t)m0 * T64, ld);
-            stage_fetch(b1, Aj + (size_t)(m0 + 1) * T64, ld);
+    if (!LEAN && P % 2 == 0) {
+        // panels in pairs: both panels' operand tiles are requested before
+        // the pair's first product, so their latencies overlap each other and
+        // that product
+        for (int m = m0; m < m0 + P; m += 2) {
+            v2d a0[8], a1[8], b0[8], b1[8];
+            stage_fetch(a0, Ai + (size_t)m * T64, ld);
+            stage_fetch(a1, Ai + (size_t)(m + 1) * T64, ld);
+            if (i != j) {
+                stage_fetch(b0, Aj + (size_t)m * T64, ld);
+                stage_fetch(b1, Aj + (size_t)(m + 1) * T64, ld);
+            }
+            if (m > m0) __syncthreads();
+            stage_put<true>(sA, a0);
+            if (i != j) stage_put<true>(sB, b0);
+            __syncthreads();
+            mma64(sA, i == j ? sA : sB, S);
+            __syncthreads();
+            stage_put<true>(sA, a1);
+            if (i != j) stage_put<true>(sB, b1);
+            __syncthreads();
+            mma64(sA, i == j ? sA : sB, S);
         }
-        stage_put<true>(sA, a0);
-        if (i != j) stage_put<true>(sB, b0);
-        __syncthreads();
-        mma64(sA, i == j ? sA : sB, S);
-        __syncthreads();
-        stage_put<true>(sA, a1);
-        if (i != j) stage_put<true>(sB, b1);
-        __syncthreads();
-        mma64(sA, i == j ? sA : sB, S);
         return;
     }
     for (int m = m0; m < m0 + P; ++m) {
@@ -439,23 +443,26 @@ __device__ __forceinline__ void panel_update32(const double* A, int ld, int i, i
                                                float* sB, v4f S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
-    if (!LEAN && P == 2) {  // operand tiles of both panels in flight at once, as in panel_update
-        v2d a0[8], a1[8], b0[8], b1[8];
-        stage_fetch(a0, Ai + (size_t)m0 * T64, ld);
-        stage_fetch(a1, Ai + (size_t)(m0 + 1) * T64, ld);
-        if (i != j) {
-            stage_fetch(b0, Aj + (size_t)m0 * T64, ld);
-            stage_fetch(b1, Aj + (size_t)(m0 + 1) * T64, ld);
+    if (!LEAN && P % 2 == 0) {  // operand tiles of a pair of panels in flight at once, as in panel_update
+        for (int m = m0; m < m0 + P; m += 2) {
+            v2d a0[8], a1[8], b0[8], b1[8];
+            stage_fetch(a0, Ai + (size_t)m * T64, ld);
+            stage_fetch(a1, Ai + (size_t)(m + 1) * T64, ld);
+            if (i != j) {
+                stage_fetch(b0, Aj + (size_t)m * T64, ld);
+                stage_fetch(b1, Aj + (size_t)(m + 1) * T64, ld);
+            }
+            if (m > m0) __syncthreads();
+            stage_put_f32t(sA, a0);
+            if (i != j) stage_put_f32t(sB, b0);
+            __syncthreads();
+            mma64_f32(sA, i == j ? sA : sB, S);
+            __syncthreads();
+            stage_put_f32t(sA, a1);
+            if (i != j) stage_put_f32t(sB, b1);
+            __syncthreads();
+            mma64_f32(sA, i == j ? sA : sB, S);
         }
-        stage_put_f32t(sA, a0);
-        if (i != j) stage_put_f32t(sB, b0);
-        __syncthreads();
-        mma64_f32(sA, i == j ? sA : sB, S);
-        __syncthreads();
-        stage_put_f32t(sA, a1);
-        if (i != j) stage_put_f32t(sB, b1);
-        __syncthreads();
-        mma64_f32(sA, i == j ? sA : sB, S);
         return;
     }
     for (int m = m0; m < m0 + P; ++m) {
@@ -630,7 +637,7 @@ __device__ __forceinline__ void job(double* A, int ld, int c0, int P, int j, dou
 template <int V, int OCC, bool MIXED>
 __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
                                                         int upd_target, double* work, double* scr, int* info,
-                                                        int* la, int chain_jobs) {
+                                                        int* la, int chain_jobs, int Pt, int near) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
@@ -741,9 +748,9 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         double* A1 = tile(i, c0 + 1);
         double* Aii = tile(i, i);
         const bool diag = (i == c0 + 2);  // also finishes tile (i, i) for the next launch
-        if (diag && P > 0) {
+        if (diag && Pt > 0) {  // (a trailing tile: it lags Pt panels)
             Updated u;
-            update_tile<MIXED>(A, ld, i, i, c0 - P, P, sA, sB, u);
+            update_tile<MIXED>(A, ld, i, i, c0 - Pt, Pt, sA, sB, u);
             store_updated(Aii, ld, u, true);
             __syncthreads();
         }
@@ -806,13 +813,21 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         }
         return;
     }
-    // trailing tile (i, j), j >= c0 + nc; with two columns, tile
-    // (c0+2, c0+2) (first in this order) belongs to row c0+2
+    // trailing tile (i, j), j >= c0 + nc, less its Pt lagging panels; with
+    // two columns, tile (c0+2, c0+2) (first in this order) belongs to row
+    // c0+2.  `near`: only the next launch's two tile columns (the rest wait
+    // for the launch after, which applies four panels at once)
     int ii, jj;
-    trailing_decode(b - 1 - nh - nw, T - c0 - nc, nc == 2, OCC == 1 ? 7 : 8, ii, jj);
+    const int r = b - 1 - nh - nw, m = T - c0 - nc;
+    if (near) {
+        ii = 1 + (r < m - 1 ? r : r - (m - 1));
+        jj = r < m - 1 ? 0 : 1;
+    } else {
+        trailing_decode(r, m, nc == 2, OCC == 1 ? 7 : 8, ii, jj);
+    }
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
-    update_tile<MIXED>(A, ld, i, j, c0 - P, P, sA, sB, u);
+    update_tile<MIXED>(A, ld, i, j, c0 - Pt, Pt, sA, sB, u);
     double* X = tile(i, j);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -1301,11 +1316,27 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     const size_t dyn_lds = own_cu ? 24576 : 0;
     if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
-    for (int c0 = 0, P = 0; c0 < T;) {
+    // Large orders (two workgroups per CU) alternate their trailing
+    // updates: an even launch updates only the next launch's two tile
+    // columns (by their two lagging panels); an odd launch updates every
+    // trailing tile by its four lagging panels (two in the second launch) as
+    // one rank-256 product, so each far tile is read and written once per
+    // four columns (C4, n = 12000: 41 GB of HBM traffic per factorisation
+    // becomes 28.5 GB).  Orders up to the own-CU limit are bound by the
+    // diagonal chain and update every trailing tile in every launch (the
+    // alternation cost 3 % at n = 3000).  The row and helper tiles of a
+    // launch always lag two panels.
+    const bool alternate = !own_cu;
+    int k = 0;
+    for (int c0 = 0, P = 0; c0 < T; ++k) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
         const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
         const int nw = T - c0 - nc;
-        const int ntr = (P > 0 && nw > 0) ? nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0) : 0;
+        const int near = (alternate && k % 2 == 0) ? 1 : 0;
+        const int Pt = (!alternate || near) ? P : (c0 < 4 ? c0 : 4);
+        int ntr = 0;
+        if (Pt > 0 && nw > 0)
+            ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
         upd_target += nh;
         // own-CU build: roles 1 .. W sit at the indices that are not
         // multiples of 8 (the smallest grid holding W of them; role
@@ -1319,11 +1350,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         if (own_cu)
             PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
                                c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs);
+                               g_chain_jobs, Pt, near);
         else
             PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
                                P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs);
+                               g_chain_jobs, Pt, near);
         P = nc;
         c0 += nc;
     }
