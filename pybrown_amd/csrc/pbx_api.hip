@@ -670,7 +670,7 @@ void drop_graphs(PbxSim* s) {
     s->n_graphs = 0;
 }
 
-enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_STEP = 4 };  // GRAPH_STEP + 0..3
+enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_INV_FACTOR = 4, GRAPH_STEP = 8 };  // GRAPH_STEP + 0..3
 
 // box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
 int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
@@ -689,12 +689,25 @@ int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
                        });
 }
 
-// box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R)
-int build_Dtot(PbxSim* s, const double* RffIn, double* Dout) {
+// A lubrication step that rebuilds D and factors it gets both from one
+// factorisation (pbx_inverse_and_factor) when the SPD inverses and the noise
+// factor are FP64.
+bool fuse_inverse_and_factor(const PbxSim* s) {
+    return is_lub(s) && s->inv_precision == 0 && s->cfg.factor_precision == 0 && s->X && s->B;
+}
+
+// box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R);
+// with_factor: also box.py:546-556, B = chol(D) (fuse_inverse_and_factor)
+int build_Dtot(PbxSim* s, const double* RffIn, double* Dout, bool with_factor = false) {
     const pbx_config& c = s->cfg;
     PBX_HIP(hipMemcpyAsync(Dout, RffIn, sizeof(double) * (size_t)s->np * s->ld, hipMemcpyDeviceToDevice, s->stream));
     TRY(pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff, c.cichocki_correction,
                                  c.viscosity, 1e19, Dout, s->ld, s->stream));
+    if (with_factor)
+        return run_graphed(s, GRAPH_INV_FACTOR, Dout, [&]() {
+            return pbx_inverse_and_factor(Dout, s->X, s->B, s->Wchol, s->Winv, s->n, s->np, s->ld, s->kBT, s->info,
+                                          s->stream);
+        });
     return run_graphed(s, GRAPH_INV_R, Dout, [&]() {
         TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT, s->inv_precision,
                             s->Xref));
@@ -1491,16 +1504,18 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
                 TRY(build_Dff(s, s->D, s->Rff));
             if (is_lub(s)) s->have_Rff = 1; else s->have_D = 1;
         }
+        const bool fused = build_Dnf_flag && cholesky_flag && fuse_inverse_and_factor(s);
         if (is_lub(s) && build_Dnf_flag) {
             if (!s->have_Rff) {
                 pbx_set_error(PBX_ERR_STATE, "lubrication step before the far-field matrix was built");
                 return PBX_ERR_STATE;
             }
-            TRY(build_Dtot(s, s->Rff, s->D));
+            TRY(build_Dtot(s, s->Rff, s->D, fused));
             s->have_D = 1;
+            if (fused) s->have_B = 1;
         }
         if (s->timing) PBX_HIP(hipEventRecord(s->ev[1], s->stream));
-        if (cholesky_flag) {
+        if (cholesky_flag && !fused) {
             if (!s->have_D) {
                 pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
                 return PBX_ERR_STATE;
@@ -1586,15 +1601,17 @@ int queued_step(PbxSim* s, double dt, int bDff, int bDnf, int chol, long long* f
             TRY(build_Dff(s, s->D, s->Rff));
             if (is_lub(s)) s->have_Rff = 1; else s->have_D = 1;
         }
+        const bool fused = bDnf && chol && fuse_inverse_and_factor(s);
         if (is_lub(s) && bDnf) {
             if (!s->have_Rff) {
                 pbx_set_error(PBX_ERR_STATE, "lubrication step before the far-field matrix was built");
                 return PBX_ERR_STATE;
             }
-            TRY(build_Dtot(s, s->Rff, s->D));
+            TRY(build_Dtot(s, s->Rff, s->D, fused));
             s->have_D = 1;
+            if (fused) s->have_B = 1;
         }
-        if (chol) {
+        if (chol && !fused) {
             if (!s->have_D) {
                 pbx_set_error(PBX_ERR_STATE, "Cholesky requested before D was built");
                 return PBX_ERR_STATE;

@@ -182,6 +182,10 @@ int pbx_factor_residual(const double* D, const double* B, int n, int np, int ld,
 int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* info_dev, hipStream_t s,
                     double scale, int precision = 0, double* C = nullptr);
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
+// D = kBT inv(R) (into R) and its Cholesky factor B (with the inverses of
+// its diagonal tiles in Wc) from one factorisation of the reversed R
+int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* Wi, int n, int np, int ld,
+                           double kBT, int* info_dev, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
 int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
                    int mixed);

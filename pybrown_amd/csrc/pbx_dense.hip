@@ -1138,6 +1138,48 @@ __global__ void copy_lower_kernel(const double* src, double* dst, int np, int ld
     }
 }
 
+// Reversal of an np x np matrix, J S J with J the order-np reversal:
+// dst[i][j] = scale * src[np-1-i][np-1-j] inside the image of the source's
+// n x n block (leading when src_low, trailing otherwise), and pad_diag on
+// the diagonal (zero elsewhere) in the image of the pad.  Both triangles are
+// written.
+__global__ void reverse_sym_kernel(const double* src, double* dst, int n, int np, int ld, double scale,
+                                   double pad_diag, int src_low) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)np * np) return;
+    const int i = (int)(t / np), j = (int)(t % np);
+    const int ri = np - 1 - i, rj = np - 1 - j;
+    const int lo = src_low ? 0 : np - n;
+    double v;
+    if (ri >= lo && ri < lo + n && rj >= lo && rj < lo + n)
+        v = scale * src[(size_t)ri * ld + rj];
+    else
+        v = (i == j) ? pad_diag : 0.0;
+    dst[(size_t)i * ld + j] = v;
+}
+
+// B = scale * J X^T J for lower-triangular X (B lower; zero above):
+// B[i][j] = scale * X[np-1-j][np-1-i].
+__global__ void reverse_transpose_lower_kernel(const double* X, double* B, int np, int ld, double scale) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)np * np) return;
+    const int i = (int)(t / np), j = (int)(t % np);
+    B[(size_t)i * ld + j] = (j <= i) ? scale * X[(size_t)(np - 1 - j) * ld + (np - 1 - i)] : 0.0;
+}
+
+// Inverses of B's diagonal tiles for B = c J X^T J, X = L^{-1}: tile k of B
+// is c revT(X_{T-1-k}) (revT(M)[r][q] = M[63-q][63-r]), so its inverse is
+// revT(L_{T-1-k}) / c, read from L's diagonal tiles (lower parts).
+__global__ void reverse_diag_inverse_kernel(const double* L, double* W, int T, int ld, double inv_c) {
+    const int k = blockIdx.x;
+    const double* Lt = L + (size_t)(T - 1 - k) * T64 * ld + (size_t)(T - 1 - k) * T64;
+    double* Wk = W + (size_t)k * T64 * T64;
+    for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x) {
+        const int r = e >> 6, q = e & 63;
+        Wk[e] = (q <= r) ? inv_c * Lt[(size_t)(63 - q) * ld + (63 - r)] : 0.0;
+    }
+}
+
 // A well-conditioned SPD matrix of order n in an np x np buffer (pad =
 // identity) for timing the factorisation: A_ij = 1 / (1 + |i - j|) off the
 // diagonal, n + 1 on it (strictly diagonally dominant).
@@ -1451,16 +1493,22 @@ int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int
     return PBX_OK;
 }
 
+// X = inv(L) (lower tiles; the upper tiles hold the recursion's partial
+// products) from the factor L in A and its diagonal-tile inverses in work.
 template <bool MIXED>
-static void trtri_lauum(double* A, int T, int ld, double* work, double* X, hipStream_t s, double scale) {
-    // every lower tile of X is written below before lauum reads it (the
-    // upper tiles hold the recursion's partial products and are not read)
+static void trtri(const double* A, int T, int ld, const double* work, double* X, hipStream_t s) {
     PBX_LAUNCH(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
     for (int b = 1; b < T; b *= 2) {
         const int npairs = (T - b + 2 * b - 1) / (2 * b);
         PBX_LAUNCH((trtri_level_kernel<0, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
         PBX_LAUNCH((trtri_level_kernel<1, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
     }
+}
+
+template <bool MIXED>
+static void trtri_lauum(double* A, int T, int ld, double* work, double* X, hipStream_t s, double scale) {
+    // every lower tile of X is written below before lauum reads it
+    trtri<MIXED>(A, T, ld, work, X, s);
     PBX_LAUNCH(lauum_kernel<MIXED>, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
 }
 
@@ -1495,6 +1543,39 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
         trtri_lauum<true>(A, T, ld, work, X, s, scale);
     else
         trtri_lauum<false>(A, T, ld, work, X, s, scale);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// D = kBT inv(R) and its lower Cholesky factor B in one pass (box.py:537 and
+// :546-556 for a lubrication step that rebuilds D and factors it).  With J
+// the order-np reversal, J R J = L L^T gives R = (J L J)(J L J)^T with J L J
+// upper triangular, so inv(R) = (J L^{-T} J)(J L^{-T} J)^T and J L^{-T} J is
+// lower triangular with a positive diagonal: the Cholesky factor of inv(R).
+// Hence B = sqrt(kBT) J X^T J and D = kBT J X^T X J with X = inv(L): one
+// factorisation, one triangular inverse and one X^T X, where inverting R and
+// then factoring D take two factorisations.  The reversed pad carries kBT on
+// its diagonal, so D and B come out with identity pads.  Wc receives the
+// inverses of B's diagonal tiles (the factorisation's work area, as a
+// factorisation of D leaves them); R is overwritten with D; X and Wi are
+// scratch (np x np, and a factorisation work area).
+int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* Wi, int n, int np, int ld,
+                           double kBT, int* info_dev, hipStream_t s) {
+    if (np % T64 != 0 || np <= 0 || n <= 0 || n > np || !(kBT > 0.0)) {
+        pbx_set_error(PBX_ERR_ARG, "inverse_and_factor: n %d np %d kBT %g", n, np, kBT);
+        return PBX_ERR_ARG;
+    }
+    const int T = np / T64;
+    const double c = sqrt(kBT);
+    const unsigned g = (unsigned)(((size_t)np * np + 255) / 256);
+    PBX_LAUNCH(reverse_sym_kernel, dim3(g), dim3(256), 0, s, R, X, n, np, ld, 1.0, kBT, 1);
+    const int rc = pbx_potrf(X, np, ld, Wi, info_dev, s, 0);  // X := L
+    if (rc) return rc;
+    trtri<false>(X, T, ld, Wi, R, s);                                 // R := inv(L) (lower tiles)
+    PBX_LAUNCH(reverse_transpose_lower_kernel, dim3(g), dim3(256), 0, s, R, B, np, ld, c);
+    PBX_LAUNCH(reverse_diag_inverse_kernel, dim3(T), dim3(256), 0, s, X, Wc, T, ld, 1.0 / c);
+    PBX_LAUNCH(lauum_kernel<false>, dim3(T * (T + 1) / 2), dim3(256), 0, s, R, ld, X, ld, T, kBT);  // X := kBT X^T X
+    PBX_LAUNCH(reverse_sym_kernel, dim3(g), dim3(256), 0, s, X, R, n, np, ld, 1.0, 1.0, 0);        // R := D
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

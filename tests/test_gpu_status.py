@@ -3,12 +3,15 @@ ran it.  The failure is forced from the host (pbx_debug_inject_wait_failure:
 the k-th factorisation or B0^T solve from now ends with its status word set
 as an exhausted bounded wait sets it); every kernel runs to completion.
 
-Status-reporting sequences of one step, in queue order:
-  lubrication Ermak with the divergence term:  Rff inverse, R inverse, the
-      Cholesky, then two inverses per coordinate in the divergence
+Status-reporting sequences of one step, in queue order (a lubrication step
+that rebuilds D and factors it takes D and B from one factorisation, the
+R inverse-and-factor):
+  lubrication Ermak with the divergence term:  Rff inverse, R
+      inverse-and-factor, then two inverses per coordinate in the divergence
       (box.py:560-610);
-  midpoint:  [Rff inverse, R inverse,] the Cholesky, [the midpoint Rff and R
-      inverses,] the B0^T solve (box.py:254-349)."""
+  midpoint:  the Cholesky, or with lubrication the Rff inverse and the R
+      inverse-and-factor; [the midpoint Rff and R inverses,] the B0^T solve
+      (box.py:254-349)."""
 import numpy as np
 import pytest
 
@@ -46,12 +49,12 @@ def test_divergence_inverse_failure_fails_its_step(overlaps):
     inp = {"box_length": 1.0e6, "T": 298.0, "viscosity": 0.01, "seed": 4, "hydrodynamics": "rpy_lub",
            "lubrication_cutoff": 1.0, "cichocki_correction": True, "divergence_term": True,
            "check_overlaps": overlaps}
-    # sequences 1-3 are the step's own inverses and Cholesky; 4 is the first
-    # inverse of the divergence's rebuilds
+    # sequences 1-2 are the step's own Rff inverse and R inverse-and-factor;
+    # 3 and 4 the first coordinate's inverses in the divergence's rebuilds
     _failing_step_raises(inp, 4)
 
 
-@pytest.mark.parametrize("hydro,k", [("rpy_smith", 2), ("rpy_smith_lub", 6)])
+@pytest.mark.parametrize("hydro,k", [("rpy_smith", 2), ("rpy_smith_lub", 5)])
 def test_midpoint_solve_failure_fails_its_step(hydro, k):
     inp = {"box_length": 12.0, "T": 298.0, "viscosity": 0.01, "seed": 5, "hydrodynamics": hydro,
            "ewald_alpha": np.sqrt(np.pi), "ewald_real": 1, "ewald_imag": 1, "lubrication_cutoff": 1.0,
