@@ -117,13 +117,16 @@ int pbx_get_potrf_variant(void);
 int pbx_set_potrf_chain_jobs(int on);
 /* Variant 6 launch shape: orders of up to `tiles` tile columns (default
  * 64) run one workgroup per compute unit and update every trailing tile in
- * every launch (rank-128); larger ones run two per CU and alternate
- * updates of the next two tile columns with rank-256 updates of the whole
- * trailing matrix, which halves the trailing matrix's HBM traffic.  The
- * factors agree to rounding.  Process-wide.  No counterpart in the
- * reference. */
+ * every launch (rank-128); larger ones run two per CU and update the next
+ * two tile columns in most launches and the whole trailing matrix (rank
+ * 128 k) every k-th, which divides the trailing matrix's HBM traffic by
+ * about k.  The factors agree to rounding.  Process-wide.  No counterpart
+ * in the reference. */
 int pbx_set_potrf_own_cu_tiles(int tiles);
 int pbx_get_potrf_own_cu_tiles(void);
+/* Larger orders update the whole trailing matrix every k launches (1..8,
+ * default 4), the next launch's two tile columns in the others. */
+int pbx_set_potrf_far_period(int k);
 /* Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
  * codes[T] for T tile columns (T < len <= 1024): 0 = the settings above,
  * 4 = variant 4, 61 = variant 6 one workgroup per CU, 62 = variant 6 two

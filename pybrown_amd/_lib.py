@@ -87,6 +87,7 @@ _sig("pbx_get_potrf_variant", _i, [])
 _sig("pbx_set_potrf_chain_jobs", _i, [_i])
 _sig("pbx_set_potrf_own_cu_tiles", _i, [_i])
 _sig("pbx_get_potrf_own_cu_tiles", _i, [])
+_sig("pbx_set_potrf_far_period", _i, [_i])
 _sig("pbx_set_potrf_table", _i, [ctypes.POINTER(ctypes.c_int), _i])
 _sig("pbx_get_potrf_shape", _i, [_i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)])
 _sig("pbx_time_potrf", _i, [_i, _i, _i, ctypes.POINTER(ctypes.c_double)])
@@ -137,7 +138,7 @@ EXPORTED = [
     "pbx_spd_inverse_host", "pbx_spd_inverse_mixed", "pbx_sim_set_inverse_precision", "pbx_gemm_nt", "pbx_cholesky_noalloc",
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
-    "pbx_set_potrf_chain_jobs", "pbx_set_potrf_own_cu_tiles", "pbx_get_potrf_own_cu_tiles",
+    "pbx_set_potrf_chain_jobs", "pbx_set_potrf_own_cu_tiles", "pbx_get_potrf_own_cu_tiles", "pbx_set_potrf_far_period",
     "pbx_set_potrf_table", "pbx_get_potrf_shape", "pbx_time_potrf", "pbx_device_name",
     "pbx_debug_inject_wait_failure",
     "pbx_kernel_launches",

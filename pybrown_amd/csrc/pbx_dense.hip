@@ -491,20 +491,42 @@ __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int 
                                             double* sB, Updated& u) {
     const double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
     if (MIXED && P > 0) {
-        double cv[16];
-        acc32_load(Aij, ld, cv);
-        v4f S[2][2];
-        acc32_zero(S);
-        panel_update32<LEAN>(A, ld, i, j, m0, P, (float*)sA, (float*)sB, S);
+        // the tile in FP64, less FP32 sums over at most two panels (128
+        // deep) each, so the accumulation error does not grow with the
+        // number of lagging panels a launch applies (the tile is the running
+        // FP64 value: no separate copy held through the products)
+        v4d cur[2][2];
+        {
+            double cv[16];
+            acc32_load(Aij, ld, cv);
+            int q = 0;
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v, ++q) cur[x][y][v] = cv[q];
+        }
+        for (int m = m0; m < m0 + P; m += 2) {
+            const int pp = (m0 + P - m) < 2 ? 1 : 2;
+            if (m > m0) __syncthreads();
+            v4f S[2][2];
+            acc32_zero(S);
+            panel_update32<LEAN>(A, ld, i, j, m, pp, (float*)sA, (float*)sB, S);
+#pragma unroll
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) cur[x][y][v] -= (double)S[x][y][v];
+        }
         __syncthreads();
-        int q = 0;
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y)
 #pragma unroll
-                for (int v = 0; v < 4; ++v, ++q)
-                    sA[acc32_row(x, v) * SP + acc_col(y)] = cv[q] - (double)S[x][y][v];
+                for (int v = 0; v < 4; ++v) sA[acc32_row(x, v) * SP + acc_col(y)] = cur[x][y][v];
         __syncthreads();
 #pragma unroll
         for (int q2 = 0; q2 < 16; ++q2) u.v[q2] = sA[u.row(q2) * SP + u.col(q2)];
@@ -1232,6 +1254,7 @@ constexpr int kDefaultPotrf = 6;
 static int g_potrf_variant = kDefaultPotrf;
 static int g_own_cu_tiles = 64;
 static std::atomic<int> g_shape_generation{0};  // launch-shape changes (keys captured graphs)
+static std::atomic<int> g_far_period{4};  // K of the large orders' whole-trailing updates
 static int g_chain_jobs = 0;
 extern "C" int pbx_set_potrf_chain_jobs(int on) {
     g_chain_jobs = on != 0;
@@ -1246,6 +1269,14 @@ extern "C" int pbx_set_potrf_own_cu_tiles(int t) {
     return PBX_OK;
 }
 extern "C" int pbx_get_potrf_own_cu_tiles(void) { return g_own_cu_tiles; }
+// K: the large orders' whole trailing matrix is updated every K launches
+// (1 = every launch; default 4).
+extern "C" int pbx_set_potrf_far_period(int k) {
+    if (k < 1 || k > 8) return PBX_ERR_ARG;
+    g_far_period = k;
+    ++g_shape_generation;
+    return PBX_OK;
+}
 
 // Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
 // code[T] for a factorisation of T tile columns is 0 (the process-wide
@@ -1358,24 +1389,27 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     const size_t dyn_lds = own_cu ? 24576 : 0;
     if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
-    // Large orders (two workgroups per CU) alternate their trailing
-    // updates: an even launch updates only the next launch's two tile
-    // columns (by their two lagging panels); an odd launch updates every
-    // trailing tile by its four lagging panels (two in the second launch) as
-    // one rank-256 product, so each far tile is read and written once per
-    // four columns (C4, n = 12000: 41 GB of HBM traffic per factorisation
-    // becomes 28.5 GB).  Orders up to the own-CU limit are bound by the
-    // diagonal chain and update every trailing tile in every launch (the
-    // alternation cost 3 % at n = 3000).  The row and helper tiles of a
-    // launch always lag two panels.
-    const bool alternate = !own_cu;
-    int k = 0;
-    for (int c0 = 0, P = 0; c0 < T; ++k) {
+    // Large orders (two workgroups per CU) update the whole trailing matrix
+    // every K launches only: every trailing tile not touched since the last
+    // such update (at column c_far) lags the panels c_far .. c0-1, and a
+    // launch updates all of them once that lag reaches 2K panels, as one
+    // rank-128K product, and otherwise only the next launch's two tile
+    // columns.  Each far tile is then read and written once per 2K columns
+    // (C4, n = 12000: 41 GB of HBM traffic per factorisation with K = 1,
+    // 28.5 GB with K = 2; K = 4, the default, was the fastest of 1, 2, 3, 4
+    // and 6 there).  Orders up to the own-CU limit are bound by the diagonal
+    // chain and take K = 1, every trailing tile in every launch (K = 2 cost
+    // 3 % at n = 3000).  The row and helper tiles of a launch always lag
+    // two panels.
+    const int K = own_cu ? 1 : g_far_period.load();
+    int c_far = 0;
+    for (int c0 = 0, P = 0; c0 < T;) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
         const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
         const int nw = T - c0 - nc;
-        const int near = (alternate && k % 2 == 0) ? 1 : 0;
-        const int Pt = (!alternate || near) ? P : (c0 < 4 ? c0 : 4);
+        const int Pt = c0 - c_far;
+        const int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
+        if (Pt > 0 && !near) c_far = c0;
         int ntr = 0;
         if (Pt > 0 && nw > 0)
             ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
