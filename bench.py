@@ -43,8 +43,13 @@ def cpu_baseline(name, steps=3, threads=1):
     omp = threads > 1
     if omp:
         os.environ["OMP_NUM_THREADS"] = str(threads)
-    kind = "reference" if ref_available(omp) else "port"
-    lib = ("ref_omp" if omp else "ref") if kind == "reference" else "orc"
+    ref = ref_available(omp)
+    lib = ("ref_omp" if omp else "ref") if ref else "orc"
+    # the step itself is this repository's restatement of box.py (the
+    # reference's Python is read, not run), so the baseline is a port even
+    # where its kernels are the reference's own C
+    kind = "port"
+
     xyz, a, inp, dt, _ = workload(name, seed=0)
     with threadpool_limits(limits=threads):
         box = OracleBox(xyz, a, ["XXX"] * len(a), inp, lib=lib)
@@ -55,8 +60,9 @@ def cpu_baseline(name, steps=3, threads=1):
             box.propagate(dt)
             times.append(time.perf_counter() - t0)
     per_step = float(np.median(times))
-    how = ("the reference's clib kernels (diff_tensor.c, bead.c) compiled from its sources at -O2 (its own "
-           "default build is unoptimised), driven by a numpy restatement of box.py's step (oracle/box_oracle.py) "
+    kernels = ("the reference's clib kernels (diff_tensor.c, bead.c) compiled from its sources at -O2 (its own "
+               "default build is unoptimised)" if ref else "the oracle's C restatement of those kernels (oracle/orc.c)")
+    how = (kernels + ", driven by a numpy restatement of box.py's step (oracle/box_oracle.py) "
            "with numpy.linalg.cholesky; 1 thread" if not omp else
            "the same, the clib sources compiled -O2 -fopenmp (the reference's own parallel loop over bead rows), "
            "%d OpenMP and BLAS threads" % threads)

@@ -59,3 +59,23 @@ def test_every_shape_factors_correctly(n, clean_table):
         assert rel <= 1e-12
     for a, b in ((61, 62), (4, 61)):
         print("shapes %d and %d equal to the bit: %s" % (a, b, np.array_equal(out[a], out[b])))
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 4])
+def test_whole_trailing_update_period(K):
+    """Orders above 64 tile columns (two workgroups per CU) update the whole
+    trailing matrix every K launches and the next launch's two tile columns
+    in between; every period factors to numpy's factor (n = 4500, 71 tile
+    columns, 36 launches)."""
+    from pybrown_amd._lib import lib
+    from pybrown_amd.diffusion import cholesky
+    A = _spd(4500, 11)
+    Ln = np.linalg.cholesky(A)
+    assert lib.pbx_set_potrf_far_period(K) == 0
+    try:
+        L = cholesky(A)
+    finally:
+        lib.pbx_set_potrf_far_period(4)
+    rel = np.abs(L - Ln).max() / np.abs(Ln).max()
+    print("K=%d: rel %.2e" % (K, rel))
+    assert rel <= 1e-12
