@@ -639,6 +639,30 @@ __device__ __forceinline__ void store_l(const double* sL, double* Akk, int ld) {
     }
 }
 
+#ifdef PBX_CHAIN_STAMPS
+// Diagnostic build only: wall-clock stamps of the two-column launches
+// ([launch][event]), read back with pbx_debug_chain_stamps.
+__device__ long long g_chain_ts[128][12];
+#define CHAIN_TS(c0, i) \
+    do { \
+        if (threadIdx.x == 0) g_chain_ts[(c0) / 2][i] = wall_clock64(); \
+    } while (0)
+#define CHAIN_TV(c0, i, v) \
+    do { \
+        if (threadIdx.x == 0) g_chain_ts[(c0) / 2][i] = (v); \
+    } while (0)
+extern "C" int pbx_debug_chain_stamps(long long* out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_chain_ts), sizeof(g_chain_ts)) == hipSuccess ? 0 : -1;
+}
+#else
+#define CHAIN_TS(c0, i) \
+    do { \
+    } while (0)
+#define CHAIN_TV(c0, i, v) \
+    do { \
+    } while (0)
+#endif
+
 // Job j of launch (c0, P): tile (c0+1, c0+j) through the panels, job 0 to
 // the scratch tile, job 1 in place; then counted in [LA_UPD].  sA and sB are
 // free again on return.
@@ -654,6 +678,131 @@ __device__ __forceinline__ void job(double* A, int ld, int c0, int P, int j, dou
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
     if (threadIdx.x == 0) __hip_atomic_fetch_add(la + LA_UPD, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    CHAIN_TS(c0, 8 + j);
+}
+
+// Products with a lower-triangular factor and lower halves, one 16 x 16
+// block per MFMA chain in mma64's order (k ascending in steps of 4), so each
+// block equals mma64's to the bit (the terms it skips are exact zeros), at
+// 40 (of 64) and 48 (of 64) MFMAs per wave.  Operands are loaded before the
+// chains, so no LDS latency sits inside them.
+//
+// The ten lower 16 x 16 blocks (X, Y) of a 64 x 64 tile, 3/3/2/2 per wave:
+// 0: (0,0) (3,0) (3,3); 1: (1,0) (1,1) (3,1); 2: (2,0) (2,1); 3: (2,2) (3,2).
+struct LowerBlocks {
+    int n, X[3], Y[3];
+};
+__device__ __forceinline__ LowerBlocks lower_blocks(int w) {
+    LowerBlocks lb;
+    lb.n = w < 2 ? 3 : 2;
+    lb.X[0] = w < 2 ? w : 2;
+    lb.Y[0] = w == 3 ? 2 : 0;
+    lb.X[1] = (w == 0 || w == 3) ? 3 : w;
+    lb.Y[1] = w == 0 ? 0 : (w == 3 ? 2 : 1);
+    lb.X[2] = 3;
+    lb.Y[2] = w == 0 ? 3 : 1;
+    return lb;
+}
+
+// acc[b] = block (w, b) of X B^T, B lower triangular (block b sums over
+// k = 0 .. 16b + 15): sXop holds X in operand layout (s[k * SP + r] =
+// X[r][k]), sB holds B row-major.  Register v of acc[b] is element
+// (16w + lk + 4v, 16b + li).
+__device__ __forceinline__ void tri_rowblock(const double* sXop, const double* sB, v4d acc[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    double a[16], bw[40];
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) a[ks] = sXop[(4 * ks + lk) * SP + 16 * w + li];
+#pragma unroll
+    for (int b = 0, i = 0; b < 4; ++b)
+#pragma unroll
+        for (int ks = 0; ks < 4 * b + 4; ++ks, ++i) bw[i] = sB[(16 * b + li) * SP + 4 * ks + lk];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (ks < 4 * b + 4) acc[b] = mfma(a[ks], bw[2 * b * (b + 1) + ks], acc[b]);
+}
+
+// s[k * SP + r] = X[r][k] for the row block tri_rowblock produced
+__device__ __forceinline__ void put_rowblock_t(double* s, const v4d acc[4]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[(16 * b + li) * SP + 16 * w + lk + 4 * q] = acc[b][q];
+}
+
+// S[s] = block (lb.X[s], lb.Y[s]) of X X^T over k = 0 .. 63, sXop = X in
+// operand layout
+__device__ __forceinline__ void lower_syrk(const double* sXop, const LowerBlocks& lb, v4d S[3]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    double pa[3][16], pb[3][16];
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+        const double* pk = sXop + (4 * ks + lk) * SP;
+#pragma unroll
+        for (int s = 0; s < 3; ++s) {
+            pa[s][ks] = (s < lb.n) ? pk[16 * lb.X[s] + li] : 0.0;
+            pb[s][ks] = (s < lb.n) ? pk[16 * lb.Y[s] + li] : 0.0;
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) S[s] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+        for (int s = 0; s < 3; ++s)
+            if (s < lb.n) S[s] = mfma(pa[s][ks], pb[s][ks], S[s]);
+}
+
+// Between the two tile factorisations of a two-column launch, after the
+// fused factorisation of tile (c0, c0) (W = inv(L) in sW, row-major; C =
+// tile (c0+1, c0) staged in sC in operand layout and T = tile (c0+1, c0+1)
+// in sT row-major, all through the lagging panels):
+//   Lo = C W^T  (= L_{c0+1,c0}), tri_rowblock;
+//   W of tile (c0, c0) released to the row workgroups (flag = v): wave 1
+//               wrote it during the factorisation;
+//   Lo also to sLo (row-major, may be sW): the caller stores it to A after
+//               its second factorisation (no workgroup of the launch reads
+//               it);
+//   sC = T - Lo Lo^T over the ten lower blocks, row-major (C is dead by
+//               then): the next tile to factor.  Lo passes through sT.
+// Lo and the new tile equal the sequential path's (mma64, put_diff) bits.
+__device__ __forceinline__ void fused_between(const double* sW, double* sC, double* sT, double* sLo, int* flag,
+                                              int v) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const LowerBlocks lb = lower_blocks(w);
+    double cv[3][4];
+#pragma unroll
+    for (int s = 0; s < 3; ++s)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            cv[s][q] = (s < lb.n) ? sT[(16 * lb.X[s] + lk + 4 * q) * SP + 16 * lb.Y[s] + li] : 0.0;
+    __syncthreads();
+    v4d acc[4];
+    tri_rowblock(sC, sW, acc);
+    put_rowblock_t(sT, acc);
+    __syncthreads();
+    // every wave has read its W operands: sLo (which may be sW) takes Lo
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sLo[(16 * w + lk + 4 * q) * SP + 16 * b + li] = acc[b][q];
+    v4d S[3];
+    lower_syrk(sT, lb, S);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        if (s >= lb.n) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sC[(16 * lb.X[s] + lk + 4 * q) * SP + 16 * lb.Y[s] + li] = cv[s][q] - S[s][q];
+    }
+    // W was written by wave 1 as it formed it: release it
+    publish_agent(flag, v);
 }
 
 template <int V, int OCC, bool MIXED>
@@ -681,9 +830,64 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     double* W1 = W0 + T64 * T64;
 
     if (b == 0) {
+        CHAIN_TS(c0, 0);
         stage<false>(sA, tile(c0, c0), ld);
         __syncthreads();
-        factor_tile<V>(sA, sB, sd, c0 * T64, info);
+        if (OCC == 1 && V == 5 && nc == 2) {
+            // One-workgroup-per-CU build: waves 2 and 3 stage tiles (c0+1, c0)
+            // and (c0+1, c0+1) during the factorisation once the helper jobs
+            // have published them, and the products for the second column
+            // follow at once (fused_between).  If the jobs were late, the
+            // sequential path below runs instead, to the same bits.
+            extern __shared__ double fuse_lds[];
+            double* sC = fuse_lds;
+            double* sT = fuse_lds + T64 * SP;
+            const FuseCtx fx{scr, T64, tile(c0 + 1, c0 + 1), ld, la + LA_UPD, upd_target, sC, sT};
+#ifdef PBX_NO_PROGRESSIVE_W
+            double* const W0p = nullptr;
+            double* const W1p = nullptr;
+#else
+            double* const W0p = W0;
+            double* const W1p = W1;
+#endif
+            bool fused = tile_factor_ws<true>(sA, sB, sd, c0 * T64, info, &fx, W0p);
+            CHAIN_TS(c0, 1);
+            CHAIN_TV(c0, 2, fused ? 1 : 0);
+            if (!fused) {
+                // staged late: now, if the jobs are done (else the
+                // sequential path, which also does unclaimed jobs)
+                __shared__ int jobs_done;
+                if (threadIdx.x == 0)
+                    jobs_done = __hip_atomic_load(la + LA_UPD, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= upd_target;
+                __syncthreads();
+                if (jobs_done) {
+                    acquire_all_waves();
+                    stage<true>(sC, scr, T64);
+                    stage<false>(sT, tile(c0 + 1, c0 + 1), ld);
+                    __syncthreads();
+                    fused = true;
+                }
+            }
+            if (fused) {
+                if (!W0p) store_w_dev(sB, W0);
+                // W0 is in global memory once released: sB then holds Lo
+                fused_between(sB, sC, sT, sB, la + LA_FLAG, c0 + 1);
+                CHAIN_TS(c0, 3);
+                tile_factor_ws(sC, sT, sd, (c0 + 1) * T64, info, nullptr, W1p);
+                CHAIN_TS(c0, 4);
+                if (!W1p) store_w_dev(sT, W1);
+                publish_agent(la + LA_FLAG, c0 + 2);
+                CHAIN_TS(c0, 5);
+                // the factors themselves: read by no workgroup of this launch
+                store_l_dev(sA, tile(c0, c0), ld);
+                for (int e = threadIdx.x; e < T64 * T64; e += blockDim.x)
+                    st_dev(tile(c0 + 1, c0) + (size_t)(e >> 6) * ld + (e & 63), sB[(e >> 6) * SP + (e & 63)]);
+                store_l_dev(sC, tile(c0 + 1, c0 + 1), ld);
+                return;
+            }
+        } else {
+            factor_tile<V>(sA, sB, sd, c0 * T64, info);
+        }
         store_w_dev(sB, W0);
         publish_agent(la + LA_FLAG, c0 + 1);
         store_l_dev(sA, tile(c0, c0), ld);
@@ -740,9 +944,12 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         mma64(sA, sA, S);
         put_diff(sB, cv, S);
         __syncthreads();
+        CHAIN_TS(c0, 3);
         factor_tile<V>(sB, sA, sd, (c0 + 1) * T64, info);
+        CHAIN_TS(c0, 4);
         store_w_dev(sA, W1);
         publish_agent(la + LA_FLAG, c0 + 2);
+        CHAIN_TS(c0, 5);
         store_l_dev(sB, tile(c0 + 1, c0 + 1), ld);
         return;
     }
@@ -774,7 +981,10 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             Updated u;
             update_tile<MIXED>(A, ld, i, i, c0 - Pt, Pt, sA, sB, u);
             store_updated(Aii, ld, u, true);
+            // other threads read these values back (lower-block layout)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
         // tile (i, c0+1) through the panels, back in place (each thread later
         // reads back the values it wrote)
@@ -792,46 +1002,106 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
         }
+        // (products with the triangular W through tri_rowblock: the same
+        // bits as mma64 with fewer MFMAs; rows then in row-block layout)
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+        auto store_rowblock = [&](double* X, const v4d R[4]) {
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) st_dev(X + (size_t)(16 * w + lk + 4 * q) * ld + 16 * bb + li, R[bb][q]);
+        };
         if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
-        stage<true>(sB, W0, T64);  // op(B)(kk, c) = W[c][kk]
+        stage<false>(sB, W0, T64);
         __syncthreads();
-        acc_zero(L0);
-        mma64(sA, sB, L0);
-        acc_for_each(L0, [&](int r, int c, double v) { st_dev(A0 + (size_t)r * ld + c, v); });
+        v4d L0b[4];
+        tri_rowblock(sA, sB, L0b);
+        store_rowblock(A0, L0b);
         // L_{c0+1,c0} = C_{c0+1,c0} W_{c0}^T from workgroup 1's scratch tile
         if (!wait_ge_agent(la + LA_UPD, upd_target, err, info)) return;
         stage<true>(sA, scr, T64);
         __syncthreads();
-        v4d Lo[2][2];
-        acc_zero(Lo);
-        mma64(sA, sB, Lo);
+        v4d Lob[4];
+        tri_rowblock(sA, sB, Lob);
         __syncthreads();
         // C_{i,c0+1} -= L_{i,c0} L_{c0+1,c0}^T
-        put_t(sA, L0);
-        put_t(sB, Lo);
+        put_rowblock_t(sA, L0b);
+        put_rowblock_t(sB, Lob);
         __syncthreads();
+        // diag: the lower blocks of tile (i, i) less L_{i,c0} L_{i,c0}^T,
+        // kept in registers (lower_blocks layout) until L_{i,c0+1} is known
+        const LowerBlocks lb = lower_blocks(w);
+        double dq[3][4], dqq[16];
         {
             double c1[16];
             acc_load(A1, ld, c1);
             v4d S[2][2];
             acc_zero(S);
             mma64(sA, sB, S);
-            if (diag) syrk_store(Aii, ld, sA);
+            if (diag && OCC == 2) {
+                // two workgroups per CU: no room for lower_syrk's operands
+                double cvq[16];
+                acc_load(Aii, ld, cvq);
+                v4d S0[2][2];
+                acc_zero(S0);
+                mma64(sA, sA, S0);
+                int q = 0;
+#pragma unroll
+                for (int x = 0; x < 2; ++x)
+#pragma unroll
+                    for (int y = 0; y < 2; ++y)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v, ++q) dqq[q] = cvq[q] - S0[x][y][v];
+            } else if (diag) {
+                double cvd[3][4];
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        cvd[t][q] = (t < lb.n) ? Aii[(size_t)(16 * lb.X[t] + lk + 4 * q) * ld + 16 * lb.Y[t] + li] : 0.0;
+                v4d S0[3];
+                lower_syrk(sA, lb, S0);
+#pragma unroll
+                for (int t = 0; t < 3; ++t)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) dq[t][q] = cvd[t][q] - S0[t][q];
+            }
             __syncthreads();
             put_diff_t(sA, c1, S);
         }
         if (!wait_ge_agent(la + LA_FLAG, c0 + 2, err, info)) return;
-        stage<true>(sB, W1, T64);
+        if (diag) CHAIN_TS(c0, 6);
+        stage<false>(sB, W1, T64);
         __syncthreads();
-        v4d L1[2][2];
-        acc_zero(L1);
-        mma64(sA, sB, L1);
-        acc_for_each(L1, [&](int r, int c, double v) { st_dev(A1 + (size_t)r * ld + c, v); });
-        if (diag) {
+        if (diag) CHAIN_TS(c0, 7);
+        v4d L1b[4];
+        tri_rowblock(sA, sB, L1b);
+        store_rowblock(A1, L1b);
+        if (diag) CHAIN_TS(c0, 11);
+        if (diag && OCC == 2) {
             __syncthreads();
-            put_t(sA, L1);
+            put_rowblock_t(sA, L1b);
             __syncthreads();
-            syrk_store(Aii, ld, sA);
+            v4d S1[2][2];
+            acc_zero(S1);
+            mma64(sA, sA, S1);
+            store_diff(Aii, ld, dqq, S1, true);
+        } else if (diag) {
+            __syncthreads();
+            put_rowblock_t(sA, L1b);
+            __syncthreads();
+            v4d S1[3];
+            lower_syrk(sA, lb, S1);
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (t >= lb.n) continue;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = 16 * lb.X[t] + lk + 4 * q, c = 16 * lb.Y[t] + li;
+                    if (c <= r) st_dev(Aii + (size_t)r * ld + c, dq[t][q] - S1[t][q]);
+                }
+            }
+            CHAIN_TS(c0, 10);
         }
         return;
     }
@@ -1383,10 +1653,12 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     int* la = (int*)(work + (size_t)np * T64);
     // Up to 64 tile columns the trailing updates of a launch finish well
     // within its diagonal chain, so each workgroup gets a compute unit of
-    // its own (dynamic LDS beyond half the CU's 160 KB): nothing shares the
-    // chain's CU.  Larger orders are throughput-bound and keep two
-    // workgroups per CU.
-    const size_t dyn_lds = own_cu ? 24576 : 0;
+    // its own (static plus dynamic LDS beyond half the CU's 160 KB): nothing
+    // shares the chain's CU.  The dynamic part holds the two tiles the
+    // diagonal chain stages for its fused second column (2 x 64 x SP
+    // doubles).  Larger orders are throughput-bound and keep two workgroups
+    // per CU.
+    const size_t dyn_lds = own_cu ? 2 * T64 * SP * sizeof(double) : 0;
     if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
     // Large orders (two workgroups per CU) update the whole trailing matrix

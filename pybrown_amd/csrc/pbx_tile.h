@@ -375,6 +375,26 @@ __device__ inline void tile_factor(double* sL, double* sW, double* sd, int col0,
 // Hand-offs are LDS counters with workgroup-scope release/acquire; every
 // wait targets a counter released by an earlier step of another wave, so the
 // waits always end (each is also bounded).  Needs exactly 256 threads.
+//
+// FUSE (the first tile of a two-column launch): waves 2 and 3 also stage,
+// once another workgroup has published them and between their own steps
+// (never waiting for them), the tiles the products after the factorisation
+// need: C = tile (c0+1, c0) into fx->sC in operand layout and T = tile
+// (c0+1, c0+1) into fx->sT row-major, 32 rows each.  The result is true when
+// both halves were staged.
+
+// Inputs of the products between the two factorisations of a two-column
+// launch (see fused_between in pbx_dense.hip).
+struct FuseCtx {
+    const double* C;  // tile (c0+1, c0) through the lagging panels (global)
+    int ldc;
+    const double* T;  // tile (c0+1, c0+1) through the lagging panels (global)
+    int ldt;
+    const int* ready; // both are published once *ready >= target
+    int target;
+    double* sC;       // LDS, 64 x SP: C, sC[k * SP + r] = C[r][k]
+    double* sT;       // LDS, 64 x SP: T, row-major
+};
 
 // Bounded wait on an LDS counter; an exhausted budget is reported through
 // the status word (the factorisation is then not valid).
@@ -393,12 +413,18 @@ __device__ __forceinline__ void lds_release_add(int* f, int v) {
     __hip_atomic_fetch_add(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-__device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int col0, int* info) {
+// Wg (optional): wave 1 also writes each panel's rows of inv(L) there
+// (64 x 64, row-major) with device-scope stores as it forms them, so a
+// publication after the factorisation finds them mostly written.
+template <bool FUSE = false>
+__device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* sd, int col0, int* info,
+                                      const FuseCtx* fx = nullptr, double* Wg = nullptr) {
     // [0]: panels published by wave 0; [1], [2]: panels whose next-but-one
     // columns wave 2, wave 3 has staged.  One counter per producer: a shared
     // count of both waves' stagings could reach its target while one wave was
-    // a panel ahead and the other still staging.
-    __shared__ int fl[3];
+    // a panel ahead and the other still staging.  [3] (FUSE): halves of C
+    // and T staged.
+    __shared__ int fl[4];
     __shared__ double sy[64];  // 1 / L_jj
     __shared__ double sB[64];  // the next panel's diagonal block, row-major 8 x 8
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -418,6 +444,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
         fl[0] = 0;
         fl[1] = 0;
         fl[2] = 0;
+        fl[3] = 0;
     }
     if (w >= 2) {
 #pragma unroll
@@ -526,8 +553,14 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
             __builtin_amdgcn_wave_barrier();
         }
     } else if (w >= 2) {
+        bool staged = false;  // FUSE: this wave's halves of C and T
         for (int p = 0; p < 7; ++p) {
             const int c0 = 8 * p, lo = c0 + 8;
+            // FUSE: the readiness word, requested before the wait so its
+            // latency overlaps it
+            const int rd = (FUSE && !staged)
+                               ? __hip_atomic_load(fx->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0;
             lds_wait_ge(&fl[0], p + 1, info);
             const int n2 = c0 + 16;  // first column of panel p+2
             // pass 0: blocks holding panel p+2's columns, which are staged;
@@ -559,6 +592,26 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
                     }
                 }
                 if (pass == 0 && lane == 0) lds_release_add(&fl[w - 1], 1);
+            }
+            if (FUSE && !staged && rd >= fx->target) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                const int r0 = 32 * (w - 2);
+#pragma unroll
+                for (int i0 = 0; i0 < 32; i0 += 8) {
+                    double cv[8], tv[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        cv[i] = fx->C[(size_t)(r0 + i0 + i) * fx->ldc + lane];
+                        tv[i] = fx->T[(size_t)(r0 + i0 + i) * fx->ldt + lane];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        fx->sC[lane * SP + r0 + i0 + i] = cv[i];
+                        fx->sT[(r0 + i0 + i) * SP + lane] = tv[i];
+                    }
+                }
+                staged = true;
+                if (lane == 0) lds_release_add(&fl[3], 1);
             }
         }
     } else {
@@ -593,6 +646,11 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) sW[(c0 + q) * SP + lane] = wv[q];
+            if (Wg) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    __hip_atomic_store(Wg + (c0 + q) * 64 + lane, wv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
             __builtin_amdgcn_wave_barrier();
             if (p == 7) break;
             // R[rows >= lo] -= L[rows, panel] W_p over blocks with columns <= c0+7
@@ -613,6 +671,7 @@ __device__ inline void tile_factor_ws(double* sL, double* sW, double* sd, int co
         }
     }
     __syncthreads();
+    return !FUSE || fl[3] == 2;
 }
 
 }  // namespace pbx_tile
