@@ -413,6 +413,27 @@ __device__ __forceinline__ void lds_release_add(int* f, int v) {
     __hip_atomic_fetch_add(f, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// FUSE: wave w (2 or 3) stages its 32 rows of C (operand layout) and T
+// (row-major) once their producers have published them
+__device__ __forceinline__ void fuse_stage(const FuseCtx* fx, int w, int lane) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int r0 = 32 * (w - 2);
+#pragma unroll
+    for (int i0 = 0; i0 < 32; i0 += 8) {
+        double cv[8], tv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            cv[i] = fx->C[(size_t)(r0 + i0 + i) * fx->ldc + lane];
+            tv[i] = fx->T[(size_t)(r0 + i0 + i) * fx->ldt + lane];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            fx->sC[lane * SP + r0 + i0 + i] = cv[i];
+            fx->sT[(r0 + i0 + i) * SP + lane] = tv[i];
+        }
+    }
+}
+
 // Wg (optional): wave 1 also writes each panel's rows of inv(L) there
 // (64 x 64, row-major) with device-scope stores as it forms them, so a
 // publication after the factorisation finds them mostly written.
@@ -425,7 +446,8 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
     // a panel ahead and the other still staging.  [3] (FUSE): halves of C
     // and T staged.
     __shared__ int fl[4];
-    __shared__ double sy[64];  // 1 / L_jj
+    __shared__ double sy[64];    // 1 / L_jj
+    __shared__ double spiv[64];  // pivots before the root (checked by wave 1)
     __shared__ double sB[64];  // the next panel's diagonal block, row-major 8 x 8
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -467,8 +489,7 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
         for (int q = 0; q < 8; ++q) v[q] = sL[lane * SP + q];
         for (int p = 0; p < 8; ++p) {
             const int c0 = 8 * p;
-            int first_bad = -1;
-            double yq[8];
+            double yq[8], piv[8];
             // Dm: lower 8 x 8 diagonal block, packed (k, m) -> k (k + 1) / 2 + m,
             // from sL for the first panel, else as the previous step left it in sB
             double Dm[36];
@@ -481,7 +502,7 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const double d = Dm[q * (q + 1) / 2 + q];
-                if (!(d > 0.0) && first_bad < 0) first_bad = q;
+                piv[q] = d;
                 const double y = rsqrt_nr(d);
                 yq[q] = y;
 #pragma unroll
@@ -505,20 +526,18 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
 #pragma unroll
                 for (int q = 0; q < 8; ++q) sL[lane * SP + c0 + q] = v[q];
             }
-            {
-                // lane c0+q stores its L_jj = v[q], lane q stores yq[q]: one
-                // store each, operands picked by selects (no per-q branches)
-                double dsel = v[0], ysel = yq[0];
+            // the block's diagonal, reciprocal roots and pivots are the same in
+            // every lane (the diagonal equals lane c0+q's own solve to the
+            // bit): one lane stores them; wave 1 checks the pivots
+            if (lane == 0) {
 #pragma unroll
-                for (int q = 1; q < 8; ++q) {
-                    dsel = (lane - c0 == q) ? v[q] : dsel;
-                    ysel = ((lane & 7) == q) ? yq[q] : ysel;
+                for (int q = 0; q < 8; ++q) {
+                    sd[c0 + q] = Dm[q * (q + 1) / 2 + q];
+                    sy[c0 + q] = yq[q];
+                    spiv[c0 + q] = piv[q];
                 }
-                if (lane >= c0 && lane < c0 + 8) sd[lane] = dsel;
-                if (lane < 8) sy[c0 + lane] = ysel;
+                lds_release_add(&fl[0], 1);
             }
-            if (first_bad >= 0 && lane == 0) atomicCAS(info, 0, col0 + c0 + first_bad + 1);
-            if (lane == 0) lds_release_add(&fl[0], 1);
             if (p == 7) break;
             // panel p+1's columns (updated through p-1 by waves 2 and 3) less
             // panel p's contribution
@@ -536,7 +555,10 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
             }
             double nv[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) nv[q] = (lane >= n0) ? sL[lane * SP + n0 + q] : 0.0;
+            for (int q = 0; q < 8; ++q) {
+                const double t = sL[lane * SP + n0 + q];  // (a plain load: no branch per element)
+                nv[q] = (lane >= n0) ? t : 0.0;
+            }
 #pragma unroll
             for (int q2 = 0; q2 < 8; ++q2) {
                 double t = nv[q2];
@@ -577,7 +599,8 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
                     for (int ks = 0; ks < 8; ks += 4) {
                         const int k = c0 + ks + lk;
                         const int ra = 16 * X + li;
-                        const double a0 = (ra >= lo) ? -sL[ra * SP + k] : 0.0;
+                        const double ta = sL[ra * SP + k];
+                        const double a0 = (ra >= lo) ? -ta : 0.0;
                         const double b0 = sL[(16 * Y + li) * SP + k];
                         acc[s] = mfma(a0, b0, acc[s]);
                     }
@@ -594,30 +617,28 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
                 if (pass == 0 && lane == 0) lds_release_add(&fl[w - 1], 1);
             }
             if (FUSE && !staged && rd >= fx->target) {
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                const int r0 = 32 * (w - 2);
-#pragma unroll
-                for (int i0 = 0; i0 < 32; i0 += 8) {
-                    double cv[8], tv[8];
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        cv[i] = fx->C[(size_t)(r0 + i0 + i) * fx->ldc + lane];
-                        tv[i] = fx->T[(size_t)(r0 + i0 + i) * fx->ldt + lane];
-                    }
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        fx->sC[lane * SP + r0 + i0 + i] = cv[i];
-                        fx->sT[(r0 + i0 + i) * SP + lane] = tv[i];
-                    }
-                }
+                fuse_stage(fx, w, lane);
                 staged = true;
                 if (lane == 0) lds_release_add(&fl[3], 1);
             }
+        }
+        // one more look once this wave's part is done (waves 0 and 1 still
+        // have their last panels ahead).  (Polling here until wave 1 finished
+        // caught every launch's jobs but made the factorisation 3 us slower.)
+        if (FUSE && !staged &&
+            __hip_atomic_load(fx->ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= fx->target) {
+            fuse_stage(fx, w, lane);
+            if (lane == 0) lds_release_add(&fl[3], 1);
         }
     } else {
         for (int p = 0; p < 8; ++p) {
             const int c0 = 8 * p, lo = c0 + 8;
             lds_wait_ge(&fl[0], p + 1, info);
+            // the panel's first non-positive pivot (NaN included), if any
+            if (lane < 8) {
+                const unsigned long long bad = __ballot(!(spiv[c0 + lane] > 0.0));
+                if (lane == 0 && bad) atomicCAS(info, 0, col0 + c0 + __builtin_ctzll(bad) + 1);
+            }
             // R rows c0..c0+7 (columns <= c0+7 lie in block row X = c0 / 16)
             const int X = c0 >> 4;
 #pragma unroll
@@ -635,7 +656,10 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
             // W rows: lane = column, substitution with the panel's diagonal block
             double wv[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) wv[q] = (lane < c0 + 8) ? sW[(c0 + q) * SP + lane] : 0.0;
+            for (int q = 0; q < 8; ++q) {
+                const double t = sW[(c0 + q) * SP + lane];
+                wv[q] = (lane < c0 + 8) ? t : 0.0;
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const double* Lr = sL + (c0 + q) * SP + c0;
@@ -663,7 +687,8 @@ __device__ __forceinline__ bool tile_factor_ws(double* sL, double* sW, double* s
                     for (int ks = 0; ks < 8; ks += 4) {
                         const int k = c0 + ks + lk;
                         const int ra = 16 * XB + li;
-                        const double a0 = (ra >= lo) ? -sL[ra * SP + k] : 0.0;
+                        const double ta = sL[ra * SP + k];
+                        const double a0 = (ra >= lo) ? -ta : 0.0;
                         const double b0 = sW[k * SP + 16 * YB + li];
                         acc[XB * (XB + 1) / 2 + YB] = mfma(a0, b0, acc[XB * (XB + 1) / 2 + YB]);
                     }
