@@ -128,8 +128,10 @@ def main():
         serve_bench(args)
         return
     # the served-sample leg runs 16 simulations at once: one hardware queue
-    # each (read when the HIP runtime starts; one stream alone uses one)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "16")
+    # each (read when the HIP runtime starts; one stream alone uses one).
+    # Set, not defaulted: an inherited smaller value (4 is HIP's default)
+    # would put four simulations on each queue.
+    os.environ["GPU_MAX_HW_QUEUES"] = os.environ.get("PBX_BENCH_HWQ") or "16"
 
     import ctypes
 
