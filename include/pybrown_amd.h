@@ -80,6 +80,14 @@ int pbx_spd_inverse_host(const double* A, int n, double scale, double* X);
  * inversion measured against the FP64 one. */
 int pbx_spd_inverse_mixed(const double* A, int n, double scale, int refine, double* X);
 
+/* x = inv(A) b for the n x n row-major SPD A, through A = L L^T and two
+ * chained triangular solves (y = inv(L) b is returned too).  Replaces the
+ * product of an explicit inverse with a vector where box.py:303-308 reads
+ * D_mid = kBT inv(R_mid) (box.py:537) only as D_mid w.  The solves run reps
+ * times; ms (may be null) receives their mean device time per pair.
+ * Returns 0, k>0 (non-positive pivot at column k-1) or a negative error. */
+int pbx_spd_solve(const double* A, int n, const double* b, int reps, double* y, double* x, double* ms);
+
 /* pbx_cholesky for a matrix already on the device, allocating nothing.  No
  * counterpart in the reference: it is for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, hosts

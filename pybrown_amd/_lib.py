@@ -80,6 +80,7 @@ _sig("pbx_cholesky", _i, [_dp, _i, _dp])
 _sig("pbx_cholesky_mixed", _i, [_dp, _i, _dp])
 _sig("pbx_spd_inverse_host", _i, [_dp, _i, _d, _dp])
 _sig("pbx_spd_inverse_mixed", _i, [_dp, _i, _d, _i, _dp])
+_sig("pbx_spd_solve", _i, [_dp, _i, _dp, _i, _dp, _dp, _dp])
 _sig("pbx_cholesky_noalloc", _i, [ctypes.c_void_p, _i, _i, ctypes.c_void_p, _i, ctypes.c_void_p])
 _sig("pbx_gemm_nt", _i, [_dp, _dp, _dp, _i, _i, _i])
 _sig("pbx_set_potrf_variant", _i, [_i])
@@ -135,7 +136,7 @@ _sig("pbx_version", ctypes.c_char_p, [])
 EXPORTED = [
     "RPY_M_matrix", "RPY_Smith_M_matrix", "JO_2B_R_matrix", "JO_R_lubrication_correction_matrix",
     "pointer_pbc_matrix", "pointer_immobile_pbc_matrix", "check_overlaps", "pbx_cholesky", "pbx_cholesky_mixed",
-    "pbx_spd_inverse_host", "pbx_spd_inverse_mixed", "pbx_sim_set_inverse_precision", "pbx_gemm_nt", "pbx_cholesky_noalloc",
+    "pbx_spd_inverse_host", "pbx_spd_inverse_mixed", "pbx_spd_solve", "pbx_sim_set_inverse_precision", "pbx_gemm_nt", "pbx_cholesky_noalloc",
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs", "pbx_set_potrf_own_cu_tiles", "pbx_get_potrf_own_cu_tiles", "pbx_set_potrf_far_period",

@@ -48,10 +48,13 @@ int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, cons
                          hipStream_t s);
 int pbx_launch_drift_diff(const double* D1, const double* D0, int n, int ld, const double* x, double alpha, double* y,
                           hipStream_t s);
-// flags: np / 64 + 1 ints (row flags, then the row ticket counter), zero
-// before the first call; epoch: 1, 2, ... per call
-int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
-                       int epoch, int* info, hipStream_t s);
+// trans 1: L^T y = x, 0: L y = x.  words: 2 np zeroed 64-bit words before
+// the first call; ticket: one zeroed int; epoch: 1, 2, ... per call
+int pbx_launch_trsv_chain(const double* Lm, int np, int ld, const double* W, const double* x, double* y,
+                          unsigned long long* words, int* ticket, unsigned epoch, int trans, int* info,
+                          hipStream_t s);
+int pbx_launch_drift_solved(const double* Bm, int n, int ld, const double* v, const double* x, double c, double alpha,
+                            double* y, hipStream_t s);
 int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
                              int cich, double mul, double div, double* C, int ld, hipStream_t s);
 
@@ -335,6 +338,67 @@ extern "C" int pbx_spd_inverse_mixed(const double* A, int n, double scale, int r
     return spd_inverse_host(A, n, scale, 1 + refine, X);
 }
 
+// x = inv(A) b for SPD A through its factor A = L L^T and the two chained
+// triangular solves the midpoint uses for D_mid w (box.py:303-308): y =
+// inv(L) b, then x = inv(L^T) y.  The solves run reps times (fresh epochs on
+// the same hand-off words); ms, when non-null, receives their mean device
+// time per pair.
+extern "C" int pbx_spd_solve(const double* A, int n, const double* b, int reps, double* y, double* x, double* ms) {
+    pbx_clear_error();
+    if (n <= 0 || reps <= 0 || !A || !b || !y || !x) return PBX_ERR_ARG;
+    int np = (int)pbx_round_up(n, 64), ld = np;
+    DevBuf dA, dW, dI, dv, dy, dx, dwords, dt;
+    TRY(dA.alloc(sizeof(double) * (size_t)np * ld));
+    TRY(dW.alloc(sizeof(double) * pbx_dense_work_doubles(np)));
+    TRY(dI.alloc(sizeof(int)));
+    TRY(dv.alloc(sizeof(double) * np));
+    TRY(dy.alloc(sizeof(double) * np));
+    TRY(dx.alloc(sizeof(double) * np));
+    TRY(dwords.alloc(sizeof(unsigned long long) * 2 * np));
+    TRY(dt.alloc(sizeof(int)));
+    PBX_HIP(hipMemset(dI.p, 0, sizeof(int)));
+    PBX_HIP(hipMemset(dt.p, 0, sizeof(int)));
+    PBX_HIP(hipMemset(dwords.p, 0, sizeof(unsigned long long) * 2 * np));
+    PBX_HIP(hipMemset(dv.p, 0, sizeof(double) * np));
+    PBX_HIP(hipMemcpy(dv.p, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    PBX_HIP(hipMemcpy2D(dA.p, sizeof(double) * ld, A, sizeof(double) * n, sizeof(double) * n, n,
+                        hipMemcpyHostToDevice));
+    TRY(pbx_fill_pad(dA.as<double>(), n, np, ld, 0));
+    TRY(pbx_potrf(dA.as<double>(), np, ld, dW.as<double>(), dI.as<int>(), 0, 0));
+    hipEvent_t e0, e1;
+    PBX_HIP(hipEventCreate(&e0));
+    PBX_HIP(hipEventCreate(&e1));
+    PBX_HIP(hipEventRecord(e0, 0));
+    unsigned ep = 0;
+    int rc = PBX_OK;
+    for (int r = 0; r < reps && rc == PBX_OK; ++r) {
+        rc = pbx_launch_trsv_chain(dA.as<double>(), np, ld, dW.as<double>(), dv.as<double>(), dy.as<double>(),
+                                   (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 0, dI.as<int>(), 0);
+        if (rc == PBX_OK)
+            rc = pbx_launch_trsv_chain(dA.as<double>(), np, ld, dW.as<double>(), dy.as<double>(), dx.as<double>(),
+                                       (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 1, dI.as<int>(), 0);
+    }
+    (void)hipEventRecord(e1, 0);
+    const hipError_t es = hipEventSynchronize(e1);
+    float t = 0.f;
+    if (es == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    PBX_HIP(es);
+    if (ms) *ms = t / reps;
+    int info = 0;
+    PBX_HIP(hipMemcpy(&info, dI.p, sizeof(int), hipMemcpyDeviceToHost));
+    PBX_HIP(hipMemcpy(y, dy.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    PBX_HIP(hipMemcpy(x, dx.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (info == PBX_INFO_DAG_WAIT) {
+        pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
+        return PBX_ERR_STATE;
+    }
+    if (info) pbx_set_error(info, "Matrix is not positive definite (pivot %d)", info - 1);
+    return info;
+}
+
 // "<name> (<arch>)" of the current device, for records of tuned settings.
 extern "C" int pbx_device_name(char* out, int len) {
     pbx_clear_error();
@@ -422,7 +486,7 @@ struct PbxSim {
     hipStream_t stream = nullptr;
     // device state
     double *pos = nullptr, *a = nullptr, *F = nullptr, *disp = nullptr, *drift = nullptr;
-    double *w = nullptr, *xipad = nullptr;
+    double *w = nullptr, *xipad = nullptr, *u = nullptr;
     int* label = nullptr;
     long long* flux = nullptr;
     int *flag = nullptr, *info = nullptr;
@@ -453,11 +517,15 @@ struct PbxSim {
     PbxStepCursor* cursor_host = nullptr;  // pinned, written once per run
     int capturing = 0;                     // a whole step is being captured
     double step_graph_dt = 0.0;
-    // midpoint: per-tile-row completion flags of the B0^T solve and its row
-    // ticket counter (zeroed at create, flags written with a per-call epoch;
-    // not part of a snapshot)
-    int* trsv_flags = nullptr;
-    int trsv_epoch = 0;
+    // midpoint: the chained triangular solves' hand-off words and row ticket
+    // counter (zeroed at create, words written with a per-call epoch; not
+    // part of a snapshot)
+    unsigned long long* chain_words = nullptr;
+    int* chain_ticket = nullptr;
+    unsigned chain_epoch = 0;
+    // D unformed: D holds inv(L) of the one-factorisation build, and
+    // ensure_D forms D from it when something reads D
+    int D_pending = 0;
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
     // noise: chunks of xi_cap attempts (3N draws each) in order; the device
     // ring holds two chunks (chunk q in slot q % 2), the pinned host side
@@ -670,7 +738,7 @@ void drop_graphs(PbxSim* s) {
     s->n_graphs = 0;
 }
 
-enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_INV_FACTOR = 4, GRAPH_STEP = 8 };  // GRAPH_STEP + 0..3
+enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_INV_FACTOR = 4, GRAPH_FACTOR_RMID = 5, GRAPH_STEP = 8 };  // GRAPH_STEP + 0..3
 
 // box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
 int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
@@ -696,18 +764,38 @@ bool fuse_inverse_and_factor(const PbxSim* s) {
     return is_lub(s) && s->inv_precision == 0 && s->cfg.factor_precision == 0 && s->X && s->B;
 }
 
-// box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R);
-// with_factor: also box.py:546-556, B = chol(D) (fuse_inverse_and_factor)
-int build_Dtot(PbxSim* s, const double* RffIn, double* Dout, bool with_factor = false) {
+// box.py:533-541: R = dR eta/1e19 + Rff (into Rout)
+int build_R(PbxSim* s, const double* RffIn, double* Rout) {
     const pbx_config& c = s->cfg;
-    PBX_HIP(hipMemcpyAsync(Dout, RffIn, sizeof(double) * (size_t)s->np * s->ld, hipMemcpyDeviceToDevice, s->stream));
-    TRY(pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff, c.cichocki_correction,
-                                 c.viscosity, 1e19, Dout, s->ld, s->stream));
-    if (with_factor)
-        return run_graphed(s, GRAPH_INV_FACTOR, Dout, [&]() {
+    PBX_HIP(hipMemcpyAsync(Rout, RffIn, sizeof(double) * (size_t)s->np * s->ld, hipMemcpyDeviceToDevice, s->stream));
+    return pbx_launch_jo_correction(s->pos, nullptr, s->a, s->N, c.box_length, c.lubrication_cutoff,
+                                    c.cichocki_correction, c.viscosity, 1e19, Rout, s->ld, s->stream);
+}
+
+// D itself when the one-factorisation build left it unformed (the same
+// operations as forming it there, so the same values)
+int ensure_D(PbxSim* s) {
+    if (!s->D_pending) return PBX_OK;
+    s->D_pending = 0;
+    return pbx_inverse_and_factor_finish(s->D, s->X, s->n, s->np, s->ld, s->kBT, s->stream);
+}
+
+// box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R);
+// with_factor: also box.py:546-556, B = chol(D) (fuse_inverse_and_factor).
+// With the factor, D itself is formed only when read (ensure_D): a step
+// without forces, divergence term or explicit read propagates with B alone.
+int build_Dtot(PbxSim* s, const double* RffIn, double* Dout, bool with_factor = false) {
+    if (Dout == s->D) s->D_pending = 0;
+    TRY(build_R(s, RffIn, Dout));
+    if (with_factor) {
+        TRY(run_graphed(s, GRAPH_INV_FACTOR, Dout, [&]() {
             return pbx_inverse_and_factor(Dout, s->X, s->B, s->Wchol, s->Winv, s->n, s->np, s->ld, s->kBT, s->info,
-                                          s->stream);
-        });
+                                          s->stream, 0);
+        }));
+        if (Dout == s->D) s->D_pending = 1;
+        else TRY(pbx_inverse_and_factor_finish(Dout, s->X, s->n, s->np, s->ld, s->kBT, s->stream));
+        return PBX_OK;
+    }
     return run_graphed(s, GRAPH_INV_R, Dout, [&]() {
         TRY(pbx_spd_inverse(Dout, s->np, s->ld, s->Winv, s->X, s->info, s->stream, s->kBT, s->inv_precision,
                             s->Xref));
@@ -717,6 +805,7 @@ int build_Dtot(PbxSim* s, const double* RffIn, double* Dout, bool with_factor = 
 
 // box.py:546-556 _decompose_D_matrix
 int factor(PbxSim* s) {
+    TRY(ensure_D(s));
     return run_graphed(s, GRAPH_FACTOR, s->B, [&]() {
         return pbx_potrf_copy(s->D, s->B, s->np, s->ld, s->Wchol, s->info, s->stream, s->cfg.factor_precision);
     });
@@ -893,6 +982,7 @@ int divergence_step(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
 // overlap, and the first clean one is applied; the host reads one index per
 // batch instead of one flag per attempt.  Results do not depend on K.
 int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
+    if (s->F_nonzero || (s->cfg.divergence_term && is_lub(s))) TRY(ensure_D(s));
     TRY(det(s, s->D, dt, 1.0, c));
     if (s->cfg.divergence_term && is_lub(s)) {
         TRY(divergence_step(s, dt, bDff, bDnf, c));
@@ -949,6 +1039,20 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     return PBX_OK;
 }
 
+// The lubrication midpoint without forces reads D_mid only in D_mid w
+// (box.py:303-308), which is kBT R_mid^{-1} w: a factorisation of R_mid and
+// two triangular solves give it without the inverse's triangular inverse
+// and X^T X.  FP64 inverses and factor only.
+bool lean_midpoint(const PbxSim* s, int bDnf) {
+    return is_lub(s) && bDnf && !s->F_nonzero && s->cfg.factor_precision == 0 && s->inv_precision == 0 && s->u;
+}
+
+int chain_solve(PbxSim* s, const double* Lm, const double* W, const double* x, double* y, int trans) {
+    if (++s->chain_epoch == 0) ++s->chain_epoch;
+    return pbx_launch_trsv_chain(Lm, s->np, s->ld, W, x, y, s->chain_words, s->chain_ticket, s->chain_epoch, trans,
+                                 s->info, s->stream);
+}
+
 // box.py:254-349
 int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     const pbx_config& cf = s->cfg;
@@ -957,6 +1061,8 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         return PBX_ERR_ARG;
     }
     const double m = cf.m_midpoint;
+    const bool lean = lean_midpoint(s, bDnf);
+    if (!lean) TRY(ensure_D(s));
     TRY(det(s, s->D, dt, 1.0 / m, c));
     int attempts = 0;
     while (true) {
@@ -977,18 +1083,33 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         const double* Dm = s->D;
         if (bDff || (is_lub(s) && bDnf)) {
             if (bDff) TRY(build_Dff(s, s->Dmid, s->RffMid));
-            if (is_lub(s) && bDnf) TRY(build_Dtot(s, bDff ? s->RffMid : s->Rff, s->Dmid));
+            if (lean) {
+                // Dmid := L_mid with R_mid = L_mid L_mid^T (tile inverses in Winv)
+                TRY(build_R(s, bDff ? s->RffMid : s->Rff, s->Dmid));
+                TRY(run_graphed(s, GRAPH_FACTOR_RMID, s->Dmid, [&]() {
+                    return pbx_potrf(s->Dmid, s->np, s->ld, s->Winv, s->info, s->stream, 0);
+                }));
+            } else if (is_lub(s) && bDnf) {
+                TRY(build_Dtot(s, bDff ? s->RffMid : s->Rff, s->Dmid));
+            }
             if (is_lub(s) || bDff) Dm = s->Dmid;
             if (is_lub(s) && bDff && !bDnf) Dm = s->D;  // self.D untouched by Dff in lub modes
         }
-        // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi
+        // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi, w = B0^{-T} xi
         PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
-        TRY(pbx_launch_trsv_lt(s->B, s->np, s->ld, s->Wchol, s->xipad, s->w, s->trsv_flags, ++s->trsv_epoch,
-                               s->info, s->stream));
-        // status of the midpoint inverses and of the solve's bounded waits,
+        TRY(chain_solve(s, s->B, s->Wchol, s->xipad, s->w, 1));
+        if (lean) {
+            // D_mid w = kBT L_mid^{-T} L_mid^{-1} w
+            TRY(chain_solve(s, s->Dmid, s->Winv, s->w, s->u, 0));
+            TRY(chain_solve(s, s->Dmid, s->Winv, s->u, s->w, 1));
+        }
+        // status of the midpoint inverses and of the solves' bounded waits,
         // read at this attempt's overlap check or at the step's end
         if (!c.queued) TRY(post_info(s));
-        if (cf.factor_precision)
+        if (lean)
+            TRY(pbx_launch_drift_solved(s->B, s->n, s->ld, s->w, s->xipad, s->kBT, m / 2 * sqrt(2 * dt), s->drift,
+                                        s->stream));
+        else if (cf.factor_precision)
             // FP32-bulk factor: the same drift as (D_mid - D0) B0^{-T} xi
             TRY(pbx_launch_drift_diff(Dm, s->D, s->n, s->ld, s->w, m / 2 * sqrt(2 * dt), s->drift, s->stream));
         else
@@ -1155,8 +1276,14 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         if ((rc = sim_alloc(s, (void**)&s->Wchol, sizeof(double) * pbx_dense_work_doubles(s->np)))) return fail(rc);
         if (cfg->scheme == PBX_MIDPOINT) {
             if ((rc = sim_matrix(s, &s->Dmid))) return fail(rc);
-            if ((rc = sim_alloc(s, (void**)&s->trsv_flags, sizeof(int) * (s->np / 64 + 1)))) return fail(rc);
-            if (!hip_ok(hipMemsetAsync(s->trsv_flags, 0, sizeof(int) * (s->np / 64 + 1), s->stream), "trsv flags"))
+            if ((rc = sim_alloc(s, (void**)&s->chain_words, sizeof(unsigned long long) * 2 * s->np)))
+                return fail(rc);
+            if ((rc = sim_alloc(s, (void**)&s->chain_ticket, sizeof(int)))) return fail(rc);
+            if ((rc = sim_alloc(s, (void**)&s->u, sizeof(double) * s->np))) return fail(rc);
+            if (!hip_ok(hipMemsetAsync(s->chain_words, 0, sizeof(unsigned long long) * 2 * s->np, s->stream),
+                        "solve words") ||
+                !hip_ok(hipMemsetAsync(s->chain_ticket, 0, sizeof(int), s->stream), "solve ticket") ||
+                !hip_ok(hipMemsetAsync(s->u, 0, sizeof(double) * s->np, s->stream), "solve vector"))
                 return fail(PBX_ERR_HIP);
         }
         if (is_lub(s)) {
@@ -1304,6 +1431,7 @@ extern "C" long long pbx_sim_snapshot_size(pbx_sim* s) {
 
 extern "C" int pbx_sim_snapshot(pbx_sim* s, void* buf, long long bytes) {
     pbx_clear_error();
+    TRY(ensure_D(s));
     PBX_HIP(hipStreamSynchronize(s->stream));
     SnapHeader h;
     memset(&h, 0, sizeof(h));
@@ -1421,6 +1549,7 @@ extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
         PBX_HIP(sim_copy(s, s->Wchol, p, sizeof(double) * std::min<long long>(h.wdoubles, pbx_dense_work_doubles(s->np)),
                           hipMemcpyHostToDevice));
     s->have_D = h.have_D;
+    s->D_pending = 0;
     s->have_B = h.have_B;
     s->have_Rff = h.have_Rff;
     s->last_attempts = h.last_attempts;
@@ -1985,6 +2114,7 @@ extern "C" int pbx_sim_factor_residual(pbx_sim* s, double* rel) {
         pbx_set_error(PBX_ERR_STATE, "factor_residual: needs a dense D and its factor");
         return PBX_ERR_STATE;
     }
+    TRY(ensure_D(s));
     DevBuf acc;
     TRY(acc.alloc(2 * sizeof(double)));
     return pbx_factor_residual(s->D, s->B, s->n, s->np, s->ld, acc.as<double>(), rel, s->stream);
@@ -2037,7 +2167,10 @@ extern "C" int pbx_sim_get_matrix(pbx_sim* s, int which, double* out) {
         pbx_set_error(PBX_ERR_ARG, "nohi keeps D and B as vectors (pbx_sim_get_diag)");
         return PBX_ERR_ARG;
     }
-    if (which == 0) src = s->D;
+    if (which == 0) {
+        TRY(ensure_D(s));
+        src = s->D;
+    }
     if (which == 1) src = s->B;
     if (which == 2) src = s->Rff;
     if (!src) {

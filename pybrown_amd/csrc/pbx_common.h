@@ -183,9 +183,11 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
                     double scale, int precision = 0, double* C = nullptr);
 int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
 // D = kBT inv(R) (into R) and its Cholesky factor B (with the inverses of
-// its diagonal tiles in Wc) from one factorisation of the reversed R
+// its diagonal tiles in Wc) from one factorisation of the reversed R;
+// form_D 0: B only, R left holding inv(L) for pbx_inverse_and_factor_finish
 int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* Wi, int n, int np, int ld,
-                           double kBT, int* info_dev, hipStream_t s);
+                           double kBT, int* info_dev, hipStream_t s, int form_D = 1);
+int pbx_inverse_and_factor_finish(double* R, double* X, int n, int np, int ld, double kBT, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
 int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
                    int mixed);

@@ -1864,9 +1864,11 @@ int pbx_spd_inverse(double* A, int np, int ld, double* work, double* X, int* inf
 // its diagonal, so D and B come out with identity pads.  Wc receives the
 // inverses of B's diagonal tiles (the factorisation's work area, as a
 // factorisation of D leaves them); R is overwritten with D; X and Wi are
-// scratch (np x np, and a factorisation work area).
+// scratch (np x np, and a factorisation work area).  form_D 0 leaves D
+// unformed: R then holds X = inv(L) (lower tiles), from which
+// pbx_inverse_and_factor_finish forms D later with the same operations.
 int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* Wi, int n, int np, int ld,
-                           double kBT, int* info_dev, hipStream_t s) {
+                           double kBT, int* info_dev, hipStream_t s, int form_D) {
     if (np % T64 != 0 || np <= 0 || n <= 0 || n > np || !(kBT > 0.0)) {
         pbx_set_error(PBX_ERR_ARG, "inverse_and_factor: n %d np %d kBT %g", n, np, kBT);
         return PBX_ERR_ARG;
@@ -1880,6 +1882,16 @@ int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* 
     trtri<false>(X, T, ld, Wi, R, s);                                 // R := inv(L) (lower tiles)
     PBX_LAUNCH(reverse_transpose_lower_kernel, dim3(g), dim3(256), 0, s, R, B, np, ld, c);
     PBX_LAUNCH(reverse_diag_inverse_kernel, dim3(T), dim3(256), 0, s, X, Wc, T, ld, 1.0 / c);
+    if (form_D) return pbx_inverse_and_factor_finish(R, X, n, np, ld, kBT, s);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// R := D = kBT J X^T X J from R = X = inv(L) as pbx_inverse_and_factor left
+// it with form_D 0 (X: np x np scratch).
+int pbx_inverse_and_factor_finish(double* R, double* X, int n, int np, int ld, double kBT, hipStream_t s) {
+    const int T = np / T64;
+    const unsigned g = (unsigned)(((size_t)np * np + 255) / 256);
     PBX_LAUNCH(lauum_kernel<false>, dim3(T * (T + 1) / 2), dim3(256), 0, s, R, ld, X, ld, T, kBT);  // X := kBT X^T X
     PBX_LAUNCH(reverse_sym_kernel, dim3(g), dim3(256), 0, s, X, R, n, np, ld, 1.0, 1.0, 0);        // R := D
     PBX_HIP(hipGetLastError());

@@ -203,84 +203,184 @@ __global__ __launch_bounds__(256) void det_drift_kernel(const double* __restrict
     if (lane == 0) y[r] += s;
 }
 
-// Solve L^T y = x for lower-triangular L (order np = 64 T) given the inverses
-// W_k of its diagonal tiles: y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m).
-// One workgroup per tile row, rows handed out from the last row up in the
-// order the workgroups start (a ticket counter at flags[T], which grows by T
-// per call), so each waits only on rows held by workgroups already running,
-// whatever the dispatch order and whatever else occupies the compute units.
-// Row k accumulates L_mk^T y_m for m = T-1 .. k+1 as each y_m is published
-// (flags[m] = epoch; the tile is requested before the wait), then publishes
-// y_k.  The critical path is one tile product, one W_k product and one
-// publication per tile row.  Waits are bounded: an exhausted wait sets *info
-// to PBX_INFO_DAG_WAIT, and the other workgroups then stop waiting.
-__global__ __launch_bounds__(256) void trsv_lt_rows_kernel(const double* __restrict__ Lm, int np, int ld,
-                                                           const double* __restrict__ W,
-                                                           const double* __restrict__ x, double* y, int* flags,
-                                                           int epoch, int* info) {
-    __shared__ double part[4][64];
-    __shared__ double rv[64];
-    __shared__ int ok;
+// y = alpha (c v - B x) for lower-triangular B: the midpoint's stochastic
+// drift when D_mid w arrives as c v (c = kBT, v = R_mid^{-1} w from two
+// triangular solves) instead of from an explicit D_mid.
+__global__ __launch_bounds__(256) void drift_solved_kernel(const double* __restrict__ Bm, int n, int ld,
+                                                           const double* __restrict__ v,
+                                                           const double* __restrict__ x, double c, double alpha,
+                                                           double* y) {
+    int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    double s = row_dot(Bm + (size_t)r * ld, x, r + 1);
+    if ((threadIdx.x & 63) == 0) y[r] = alpha * (c * v[r] - s);
+}
+
+// ---------------------------------------------------------------------------
+// Triangular solves with lower-triangular L (order np = 64 T) given the
+// inverses W_k of its diagonal tiles:
+//   TR:  L^T y = x,  y_k = W_k^T (x_k - sum_{m>k} L_mk^T y_m)  (rows from the last up)
+//   !TR: L y = x,    y_k = W_k   (x_k - sum_{m<k} L_km y_m)    (rows from the first down)
+// One workgroup per tile row.  Rows are handed out in the order the
+// workgroups start (a ticket counter that grows by T per call), so each
+// waits only on rows held by workgroups already running, whatever the
+// dispatch order and whatever else occupies the compute units.
+//
+// Rows pass their y_k to one another as self-validating words: each double
+// is stored as two 64-bit words, one 32-bit half beside the call's epoch in
+// each (relaxed device-scope atomic stores and loads, each word written and
+// read whole).  A reader that finds the epoch in both words holds the value,
+// so a hand-off is one L2 round trip, with no flag and no fence.  The epoch
+// differs between calls; words from an earlier call never match.
+//
+// In a row's workgroup, lane o of every wave holds output o.  Wave 0 holds
+// the tile of the row published last (the one on the critical path) and W_k
+// in registers from the start; waves 1-3 take the row's other tiles in
+// publication order, round robin, and pass their partial sums to wave 0
+// through LDS.  Products take y_m from the lanes by readlane, one entry at a
+// time, in a fixed order.  The critical path per row is one wait, two
+// 64-long products and the stores.  Waits are bounded: an exhausted wait sets
+// *info to PBX_INFO_DAG_WAIT, and every other wait then gives up.
+
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// acc + sum_i a[i] v_i, v_i held by lane i
+__device__ __forceinline__ double lane_dot(const double (&a)[64], double v, double acc) {
+#pragma unroll
+    for (int i = 0; i < 64; ++i) acc = fma(a[i], lane_value(v, i), acc);
+    return acc;
+}
+
+__device__ __forceinline__ void chain_put(unsigned long long* p, double v, unsigned ep) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned long long e = (unsigned long long)ep << 32;
+    __hip_atomic_store(p, (b & 0xffffffffull) | e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, (b >> 32) | e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every lane of the wave waits for its value (lane o: p[2 o], p[2 o + 1]);
+// false when the wait was given up.
+__device__ __forceinline__ bool chain_get(const unsigned long long* p, unsigned ep, int* info, double& v) {
+    const int o = threadIdx.x & 63;
+    int spins = 0;
+    while (true) {
+        const unsigned long long w0 = __hip_atomic_load(p + 2 * o, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long w1 = __hip_atomic_load(p + 2 * o + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool ok = (unsigned)(w0 >> 32) == ep && (unsigned)(w1 >> 32) == ep;
+        if (__all(ok)) {
+            v = __longlong_as_double((long long)((w0 & 0xffffffffull) | (w1 << 32)));
+            return true;
+        }
+        if ((++spins & 255) == 0) {
+            if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+            if (spins >= (1 << 22)) {
+                if (o == 0) atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
+                return false;
+            }
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// a[i] = A_km[o][i]: L_mk^T (TR: lane o reads column o of L_mk, coalesced)
+// or L_km (lane o reads row o)
+template <bool TR>
+__device__ __forceinline__ void chain_tile(const double* __restrict__ Lm, int ld, int k, int m, int o,
+                                           double (&a)[64]) {
+    if (TR) {
+        const double* p = Lm + (size_t)(64 * m) * ld + 64 * k + o;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) a[i] = p[(size_t)i * ld];
+    } else {
+        const double2* p = (const double2*)(Lm + (size_t)(64 * k + o) * ld + 64 * m);
+#pragma unroll
+        for (int j = 0; j < 32; ++j) {
+            const double2 t = p[j];
+            a[2 * j] = t.x;
+            a[2 * j + 1] = t.y;
+        }
+    }
+}
+
+template <bool TR>
+__global__ __launch_bounds__(256) void trsv_chain_kernel(const double* __restrict__ Lm, int np, int ld,
+                                                         const double* __restrict__ W,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         unsigned long long* words, int* ticket, unsigned ep,
+                                                         int* info) {
+    __shared__ double part[3][64];
+    __shared__ int done;
     __shared__ int row;
     const int T = np / 64;
-    if (threadIdx.x == 0)
-        row = T - 1 - (int)(__hip_atomic_fetch_add((unsigned*)flags + T, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) %
-                            (unsigned)T);
+    if (threadIdx.x == 0) {
+        const unsigned t =
+            __hip_atomic_fetch_add((unsigned*)ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) % (unsigned)T;
+        row = TR ? T - 1 - (int)t : (int)t;
+        done = 0;
+    }
     __syncthreads();
     const int k = row;
-    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    // this thread's entries of W_k^T: W_k[cp][c] for cp = c + g, c + g + 4, ... (W_k lower)
-    const double* Wk = W + (size_t)k * 64 * 64;
-    double wv[16];
+    const int o = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ndep = TR ? T - 1 - k : k;  // tiles this row waits on; dependency q is published q-th
+    if (wv == 0) {
+        double a[64], wk[64];
+        const int mc = TR ? k + 1 : k - 1;  // the row published last
+        if (ndep > 0) chain_tile<TR>(Lm, ld, k, mc, o, a);
+        const double* Wk = W + (size_t)k * 64 * 64;
+        if (TR) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int cp = c + g + 4 * q;
-        wv[q] = cp < 64 ? Wk[cp * 64 + c] : 0.0;
-    }
-    double acc = 0.0;
-    for (int m = T - 1; m > k; --m) {
-        double lv[16];
+            for (int i = 0; i < 64; ++i) {
+                const double t = Wk[i * 64 + o];  // W_k[i][o], nonzero for i >= o
+                wk[i] = i >= o ? t : 0.0;
+            }
+        } else {
+            const double2* p = (const double2*)(Wk + o * 64);  // W_k[o][i], nonzero for i <= o
 #pragma unroll
-        for (int q = 0; q < 16; ++q) lv[q] = Lm[(size_t)(64 * m + g + 4 * q) * ld + 64 * k + c];
-        if (threadIdx.x == 0) {
-            int good = 1;
-            long long spins = 0;
-            while (__hip_atomic_load(flags + m, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
-                if (__hip_atomic_load(info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1 << 22)) {
-                    atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
-                    good = 0;
-                    break;
+            for (int j = 0; j < 32; ++j) {
+                const double2 t = p[j];
+                wk[2 * j] = 2 * j <= o ? t.x : 0.0;
+                wk[2 * j + 1] = 2 * j + 1 <= o ? t.y : 0.0;
+            }
+        }
+        double r = x[64 * k + o];
+        if (ndep > 1) {
+            int it = 0;
+            while (__hip_atomic_load(&done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < 3) {
+                if (++it >= (1 << 24)) {
+                    if (o == 0) atomicCAS(info, 0, PBX_INFO_DAG_WAIT);
+                    return;
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            ok = good;
+            r -= (part[0][o] + part[1][o]) + part[2][o];
         }
-        __syncthreads();
-        if (!ok) return;
-        // y_m was stored by another workgroup of this launch: every wave
-        // acquires for itself
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        const double* ym = y + 64 * m;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) acc += lv[q] * ym[g + 4 * q];
+        if (ndep > 0) {
+            double ym;
+            if (!chain_get(words + (size_t)2 * 64 * mc, ep, info, ym)) return;
+            r -= lane_dot(a, ym, 0.0);
+        }
+        const double yk = lane_dot(wk, r, 0.0);
+        chain_put(words + (size_t)2 * (64 * k + o), yk, ep);
+        y[64 * k + o] = yk;
+    } else {
+        double acc = 0.0;
+        for (int q = wv - 1; q < ndep - 1; q += 3) {
+            const int m = TR ? T - 1 - q : q;
+            double a[64];
+            chain_tile<TR>(Lm, ld, k, m, o, a);
+            double ym;
+            if (!chain_get(words + (size_t)2 * 64 * m, ep, info, ym)) break;
+            acc = lane_dot(a, ym, acc);
+        }
+        part[wv - 1][o] = acc;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (o == 0) __hip_atomic_fetch_add(&done, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    part[g][c] = acc;
-    __syncthreads();
-    if (g == 0) rv[c] = x[64 * k + c] - ((part[0][c] + part[1][c]) + (part[2][c] + part[3][c]));
-    __syncthreads();
-    double u = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int cp = c + g + 4 * q;
-        if (cp < 64) u += wv[q] * rv[cp];
-    }
-    part[g][c] = u;
-    __syncthreads();
-    if (g == 0) y[64 * k + c] = (part[0][c] + part[1][c]) + (part[2][c] + part[3][c]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(flags + k, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ---------------------------------------------------------------------------
@@ -678,14 +778,27 @@ int pbx_launch_det_drift(const double* D1, const double* D0, int n, int ld, cons
 
 void pbx_status_sequence_queued(int* info_dev, hipStream_t s);
 
-int pbx_launch_trsv_lt(const double* Lm, int np, int ld, const double* W, const double* x, double* y, int* flags,
-                       int epoch, int* info, hipStream_t s) {
-    if (np <= 0 || np % 64 || epoch <= 0) {
-        pbx_set_error(PBX_ERR_ARG, "trsv_lt: np=%d epoch=%d", np, epoch);
+int pbx_launch_trsv_chain(const double* Lm, int np, int ld, const double* W, const double* x, double* y,
+                          unsigned long long* words, int* ticket, unsigned epoch, int trans, int* info,
+                          hipStream_t s) {
+    if (np <= 0 || np % 64 || ld < np || ld % 2 || epoch == 0) {
+        pbx_set_error(PBX_ERR_ARG, "trsv_chain: np=%d ld=%d epoch=%u", np, ld, epoch);
         return PBX_ERR_ARG;
     }
-    PBX_LAUNCH(trsv_lt_rows_kernel, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, flags, epoch, info);
+    if (trans)
+        PBX_LAUNCH(trsv_chain_kernel<true>, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, words, ticket, epoch,
+                   info);
+    else
+        PBX_LAUNCH(trsv_chain_kernel<false>, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, words, ticket,
+                   epoch, info);
     pbx_status_sequence_queued(info, s);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_drift_solved(const double* Bm, int n, int ld, const double* v, const double* x, double c, double alpha,
+                            double* y, hipStream_t s) {
+    PBX_LAUNCH(drift_solved_kernel, dim3((n + 3) / 4), dim3(256), 0, s, Bm, n, ld, v, x, c, alpha, y);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

@@ -164,6 +164,34 @@ def test_midpoint_many_tile_rows():
     _run_pair(xyz, a, inp, 6, 1.0)
 
 
+def test_midpoint_lub_many_tile_rows():
+    """N = 400 with lubrication (19 tile rows): D_mid w from R_mid's factor
+    and the two chained solves, the B0^T solve on the same chain."""
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(400, 5.0)
+    Lb = box_for(a, 0.15)
+    xyz = rsa_positions(a, Lb, 10, min_gap=0.2)
+    inp = {"box_length": Lb, "T": 293.15, "viscosity": 0.01005, "seed": 3, "hydrodynamics": "rpy_smith_lub",
+           "ewald_alpha": math.sqrt(math.pi), "ewald_real": 1, "ewald_imag": 1, "lubrication_cutoff": 1.0,
+           "propagation_scheme": "midpoint", "m_midpoint": 100, "measure_flux": FLUX, "check_overlaps": False}
+    _run_pair(xyz, a, inp, 5, 1.0)
+
+
+@pytest.mark.parametrize("hyd", ["rpy_lub", "rpy_smith_lub"])
+def test_midpoint_lub_with_external_force(hyd):
+    """With forces the midpoint reads D and D_mid themselves (explicit
+    inverses; D formed from the one-factorisation build when read), also on
+    steps that reuse the last step's D and B (cadence flags)."""
+    from pybrown_amd.workloads import rsa_positions, box_for
+    a = np.full(45, 1.0)
+    Lb = box_for(a, 0.1)
+    xyz = rsa_positions(a, Lb, 7, min_gap=0.05)
+    inp = {"box_length": Lb, "T": 298.0, "viscosity": 0.01, "seed": 8, "hydrodynamics": hyd,
+           "ewald_real": 1, "ewald_imag": 1, "lubrication_cutoff": 1.0, "propagation_scheme": "midpoint",
+           "m_midpoint": 100, "external_force": [2e-21, 0.0, -3e-21], "measure_flux": FLUX}
+    _run_pair(xyz, a, inp, 16, 0.05, flags=lambda j: (j % 3 == 0, j % 2 == 0, j % 2 == 0))
+
+
 @pytest.mark.parametrize("hyd", ["rpy_lub", "rpy_smith_lub"])
 def test_ermak_lub_with_cadence_flags(hyd):
     from pybrown_amd.workloads import rsa_positions, box_for

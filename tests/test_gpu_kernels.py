@@ -215,6 +215,30 @@ def test_cholesky_reports_non_spd(potrf_variant):
         cholesky(A)
 
 
+@pytest.mark.parametrize("n", [64, 200, 700, 3000, 7232])
+def test_spd_solve_vs_numpy(n):
+    """pbx_spd_solve: y = inv(L) b and x = inv(A) b from the factor and the two
+    chained triangular solves, against numpy at 1e-12; repeated solves on the
+    same hand-off words (new epochs) give the same bits."""
+    from pybrown_amd._lib import lib, ptr
+    rng = np.random.RandomState(n)
+    X = rng.normal(size=(n, n))
+    A = X @ X.T / n + np.eye(n)
+    b = rng.normal(size=n)
+    y1, x1, y4, x4 = (np.zeros(n) for _ in range(4))
+    ms = np.zeros(1)
+    assert lib.pbx_spd_solve(ptr(A), n, ptr(b), 1, ptr(y1), ptr(x1), ptr(ms)) == 0, lib.pbx_last_error()
+    assert lib.pbx_spd_solve(ptr(A), n, ptr(b), 4, ptr(y4), ptr(x4), ptr(ms)) == 0, lib.pbx_last_error()
+    Ln = np.linalg.cholesky(A)
+    y_want = np.linalg.solve(Ln, b)
+    x_want = np.linalg.solve(A, b)
+    ry = np.linalg.norm(y1 - y_want) / np.linalg.norm(y_want)
+    rx = np.linalg.norm(x1 - x_want) / np.linalg.norm(x_want)
+    print("spd solve n=%d rel y %.3e x %.3e, %.1f us per pair of solves" % (n, ry, rx, 1e3 * ms[0]))
+    assert ry <= 1e-12 and rx <= 1e-12
+    assert np.array_equal(y1, y4) and np.array_equal(x1, x4)
+
+
 @pytest.mark.parametrize("n", [64, 200, 700])
 def test_spd_inverse_vs_numpy(n, potrf_variant):
     from pybrown_amd.diffusion import spd_inverse
