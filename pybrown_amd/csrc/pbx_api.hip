@@ -50,9 +50,16 @@ int pbx_launch_drift_diff(const double* D1, const double* D0, int n, int ld, con
                           hipStream_t s);
 // trans 1: L^T y = x, 0: L y = x.  words: 2 np zeroed 64-bit words before
 // the first call; ticket: one zeroed int; epoch: 1, 2, ... per call
+// rev: x and y reversed (x'[i] = x[np-1-i]); y scaled by yscale
 int pbx_launch_trsv_chain(const double* Lm, int np, int ld, const double* W, const double* x, double* y,
                           unsigned long long* words, int* ticket, unsigned epoch, int trans, int* info,
-                          hipStream_t s);
+                          hipStream_t s, int rev, double yscale);
+int pbx_launch_drift_vec(const double* v, const double* bx, int n, double c, double alpha, double* y, hipStream_t s);
+int pbx_launch_rev_lower_gemv(const double* Lm, int np, int ld, const double* x, double scale, double* y,
+                              hipStream_t s);
+int pbx_launch_stoch_vec(const double* bx, double sq2dt, double mult, double* pos, double* disp, int N,
+                         const double* plane_n, const double* plane_p, int flux_on, const int* label,
+                         long long* flux, hipStream_t s);
 int pbx_launch_drift_solved(const double* Bm, int n, int ld, const double* v, const double* x, double c, double alpha,
                             double* y, hipStream_t s);
 int pbx_launch_jo_correction(const double* xyz, const double* P, const double* a, int N, double L, double cutoff,
@@ -373,10 +380,11 @@ extern "C" int pbx_spd_solve(const double* A, int n, const double* b, int reps, 
     int rc = PBX_OK;
     for (int r = 0; r < reps && rc == PBX_OK; ++r) {
         rc = pbx_launch_trsv_chain(dA.as<double>(), np, ld, dW.as<double>(), dv.as<double>(), dy.as<double>(),
-                                   (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 0, dI.as<int>(), 0);
+                                   (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 0, dI.as<int>(), 0, 0, 1.0);
         if (rc == PBX_OK)
             rc = pbx_launch_trsv_chain(dA.as<double>(), np, ld, dW.as<double>(), dy.as<double>(), dx.as<double>(),
-                                       (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 1, dI.as<int>(), 0);
+                                       (unsigned long long*)dwords.p, dt.as<int>(), ++ep, 1, dI.as<int>(), 0, 0,
+                                       1.0);
     }
     (void)hipEventRecord(e1, 0);
     const hipError_t es = hipEventSynchronize(e1);
@@ -523,9 +531,12 @@ struct PbxSim {
     unsigned long long* chain_words = nullptr;
     int* chain_ticket = nullptr;
     unsigned chain_epoch = 0;
-    // D unformed: D holds inv(L) of the one-factorisation build, and
-    // ensure_D forms D from it when something reads D
-    int D_pending = 0;
+    // D and B as the lubrication builds leave them (ensure_B / ensure_D form
+    // what is read): 0 both formed; 1 B formed, D holds inv(L) of the
+    // one-factorisation build; 2 B holds L = chol(J R J) and Wchol the
+    // inverses of L's diagonal tiles (B = sqrt(kBT) J L^{-T} J), D unformed
+    int mat_state = 0;
+    double* bxi = nullptr;  // B xi of the current midpoint attempt (B held as L)
     double *Dd = nullptr, *Bd = nullptr;  // nohi diagonal D and sqrt(D)
     // noise: chunks of xi_cap attempts (3N draws each) in order; the device
     // ring holds two chunks (chunk q in slot q % 2), the pinned host side
@@ -738,7 +749,15 @@ void drop_graphs(PbxSim* s) {
     s->n_graphs = 0;
 }
 
-enum { GRAPH_FACTOR = 1, GRAPH_INV_RFF = 2, GRAPH_INV_R = 3, GRAPH_INV_FACTOR = 4, GRAPH_FACTOR_RMID = 5, GRAPH_STEP = 8 };  // GRAPH_STEP + 0..3
+enum {
+    GRAPH_FACTOR = 1,
+    GRAPH_INV_RFF = 2,
+    GRAPH_INV_R = 3,
+    GRAPH_INV_FACTOR = 4,
+    GRAPH_FACTOR_RMID = 5,
+    GRAPH_FACTOR_REV = 6,
+    GRAPH_STEP = 8  // + 0..3
+};  // GRAPH_STEP + 0..3
 
 // box.py:486-528 _compute_Dff_matrix into (D) or (Rff) depending on mode.
 int build_Dff(PbxSim* s, double* Dout, double* RffOut) {
@@ -772,27 +791,57 @@ int build_R(PbxSim* s, const double* RffIn, double* Rout) {
                                     c.cichocki_correction, c.viscosity, 1e19, Rout, s->ld, s->stream);
 }
 
-// D itself when the one-factorisation build left it unformed (the same
-// operations as forming it there, so the same values)
+// B itself, and then D, when a lubrication build left them unformed (the
+// same operations as forming them there, so the same values)
+int ensure_B(PbxSim* s) {
+    if (s->mat_state != 2) return PBX_OK;
+    s->mat_state = 1;
+    return pbx_factor_reversed_finish(s->B, s->Wchol, s->D, s->np, s->ld, s->kBT, s->stream);
+}
+
 int ensure_D(PbxSim* s) {
-    if (!s->D_pending) return PBX_OK;
-    s->D_pending = 0;
+    TRY(ensure_B(s));
+    if (s->mat_state != 1) return PBX_OK;
+    s->mat_state = 0;
     return pbx_inverse_and_factor_finish(s->D, s->X, s->n, s->np, s->ld, s->kBT, s->stream);
+}
+
+// The lubrication midpoint without forces reads D_mid only in D_mid w
+// (box.py:303-308), which is kBT R_mid^{-1} w: a factorisation of R_mid and
+// two triangular solves give it without the inverse's triangular inverse
+// and X^T X.  Its B0 enters only as B0 xi and B0^{-T} xi, which the factor
+// L of J R J gives with one triangular solve and one triangular product.
+// FP64 inverses and factor only.
+bool lean_midpoint(const PbxSim* s, int bDnf) {
+    return is_lub(s) && bDnf && !s->F_nonzero && s->cfg.factor_precision == 0 && s->inv_precision == 0 && s->u;
 }
 
 // box.py:533-541 _compute_Dtot_matrix: R = dR eta/1e19 + Rff; D = kBT inv(R);
 // with_factor: also box.py:546-556, B = chol(D) (fuse_inverse_and_factor).
 // With the factor, D itself is formed only when read (ensure_D): a step
-// without forces, divergence term or explicit read propagates with B alone.
+// without forces, divergence term or explicit read propagates with B alone;
+// a lean midpoint step (lean_midpoint) keeps B as L too (ensure_B).
 int build_Dtot(PbxSim* s, const double* RffIn, double* Dout, bool with_factor = false) {
-    if (Dout == s->D) s->D_pending = 0;
+    if (Dout == s->D) {
+        // a new D without a new factor: the last factor stays, formed
+        // before D's buffer is reused
+        if (!with_factor) TRY(ensure_B(s));
+        s->mat_state = 0;
+    }
     TRY(build_R(s, RffIn, Dout));
+    if (with_factor && Dout == s->D && s->cfg.scheme == PBX_MIDPOINT && lean_midpoint(s, 1)) {
+        TRY(run_graphed(s, GRAPH_FACTOR_REV, s->B, [&]() {
+            return pbx_factor_reversed(Dout, s->B, s->Wchol, s->n, s->np, s->ld, s->kBT, s->info, s->stream);
+        }));
+        s->mat_state = 2;
+        return PBX_OK;
+    }
     if (with_factor) {
         TRY(run_graphed(s, GRAPH_INV_FACTOR, Dout, [&]() {
             return pbx_inverse_and_factor(Dout, s->X, s->B, s->Wchol, s->Winv, s->n, s->np, s->ld, s->kBT, s->info,
                                           s->stream, 0);
         }));
-        if (Dout == s->D) s->D_pending = 1;
+        if (Dout == s->D) s->mat_state = 1;
         else TRY(pbx_inverse_and_factor_finish(Dout, s->X, s->n, s->np, s->ld, s->kBT, s->stream));
         return PBX_OK;
     }
@@ -982,6 +1031,7 @@ int divergence_step(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
 // overlap, and the first clean one is applied; the host reads one index per
 // batch instead of one flag per attempt.  Results do not depend on K.
 int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
+    TRY(ensure_B(s));
     if (s->F_nonzero || (s->cfg.divergence_term && is_lub(s))) TRY(ensure_D(s));
     TRY(det(s, s->D, dt, 1.0, c));
     if (s->cfg.divergence_term && is_lub(s)) {
@@ -1039,18 +1089,11 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     return PBX_OK;
 }
 
-// The lubrication midpoint without forces reads D_mid only in D_mid w
-// (box.py:303-308), which is kBT R_mid^{-1} w: a factorisation of R_mid and
-// two triangular solves give it without the inverse's triangular inverse
-// and X^T X.  FP64 inverses and factor only.
-bool lean_midpoint(const PbxSim* s, int bDnf) {
-    return is_lub(s) && bDnf && !s->F_nonzero && s->cfg.factor_precision == 0 && s->inv_precision == 0 && s->u;
-}
-
-int chain_solve(PbxSim* s, const double* Lm, const double* W, const double* x, double* y, int trans) {
+int chain_solve(PbxSim* s, const double* Lm, const double* W, const double* x, double* y, int trans, int rev = 0,
+                double yscale = 1.0) {
     if (++s->chain_epoch == 0) ++s->chain_epoch;
     return pbx_launch_trsv_chain(Lm, s->np, s->ld, W, x, y, s->chain_words, s->chain_ticket, s->chain_epoch, trans,
-                                 s->info, s->stream);
+                                 s->info, s->stream, rev, yscale);
 }
 
 // box.py:254-349
@@ -1063,13 +1106,25 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
     const double m = cf.m_midpoint;
     const bool lean = lean_midpoint(s, bDnf);
     if (!lean) TRY(ensure_D(s));
+    // B0 held as L = chol(J R0 J): B0 xi = sqrt(kBT) J L^{-T} J xi (one
+    // solve per attempt) and B0^{-T} xi = J L J xi / sqrt(kBT)
+    const bool implicit = lean && s->mat_state == 2;
+    const double cB = sqrt(s->kBT);
+    const double* xi = nullptr;
+    auto stoch0 = [&](double mult) {
+        if (implicit)
+            return pbx_launch_stoch_vec(s->bxi, c.sq2dt, mult, s->pos, s->disp, s->N, c.pn, c.pp, c.flux_on, s->label,
+                                        c.flux, s->stream);
+        return stoch(s, s->B, xi, mult, c);
+    };
     TRY(det(s, s->D, dt, 1.0 / m, c));
     int attempts = 0;
     while (true) {
         ++attempts;
-        const double* xi;
         TRY(next_xi(s, &xi));
-        TRY(stoch(s, s->B, xi, 1.0 / m, c));
+        PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
+        if (implicit) TRY(chain_solve(s, s->B, s->Wchol, s->xipad, s->bxi, 1, 1, cB));
+        TRY(stoch0(1.0 / m));
         if (cf.check_overlaps) {
             int hit = 0;
             TRY(overlap_now(s, &hit));
@@ -1096,8 +1151,10 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
             if (is_lub(s) && bDff && !bDnf) Dm = s->D;  // self.D untouched by Dff in lub modes
         }
         // stochastic drift: m/2 sqrt(2dt) (D_mid B0^{-T} - B0) xi, w = B0^{-T} xi
-        PBX_HIP(hipMemcpyAsync(s->xipad, xi, sizeof(double) * s->n, hipMemcpyDeviceToDevice, s->stream));
-        TRY(chain_solve(s, s->B, s->Wchol, s->xipad, s->w, 1));
+        if (implicit)
+            TRY(pbx_launch_rev_lower_gemv(s->B, s->np, s->ld, s->xipad, 1.0 / cB, s->w, s->stream));
+        else
+            TRY(chain_solve(s, s->B, s->Wchol, s->xipad, s->w, 1));
         if (lean) {
             // D_mid w = kBT L_mid^{-T} L_mid^{-1} w
             TRY(chain_solve(s, s->Dmid, s->Winv, s->w, s->u, 0));
@@ -1106,7 +1163,9 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         // status of the midpoint inverses and of the solves' bounded waits,
         // read at this attempt's overlap check or at the step's end
         if (!c.queued) TRY(post_info(s));
-        if (lean)
+        if (implicit)
+            TRY(pbx_launch_drift_vec(s->w, s->bxi, s->n, s->kBT, m / 2 * sqrt(2 * dt), s->drift, s->stream));
+        else if (lean)
             TRY(pbx_launch_drift_solved(s->B, s->n, s->ld, s->w, s->xipad, s->kBT, m / 2 * sqrt(2 * dt), s->drift,
                                         s->stream));
         else if (cf.factor_precision)
@@ -1117,14 +1176,14 @@ int midpoint(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         if (s->F_nonzero)
             TRY(pbx_launch_det_drift(Dm, s->D, s->n, s->ld, s->F, m * dt / (2 * s->kBT), s->drift, s->stream));
         TRY(det(s, s->D, dt, 1.0 - 1.0 / m, c));
-        TRY(stoch(s, s->B, xi, 1.0 - 1.0 / m, c));
+        TRY(stoch0(1.0 - 1.0 / m));
         TRY(pbx_launch_translate(s->drift, 1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream));
         if (!cf.check_overlaps) break;
         int hit = 0;
         TRY(overlap_now(s, &hit));
         if (!hit) break;
         TRY(det(s, s->D, dt, -(1.0 - 1.0 / m), c));
-        TRY(stoch(s, s->B, xi, -1.0, c));
+        TRY(stoch0(-1.0));
         TRY(pbx_launch_translate(s->drift, -1.0, s->pos, s->N, c.pn, c.pp, c.flux_on, s->label, c.flux, s->stream));
         if (attempts >= cf.max_move_attempts) {
             s->last_attempts = attempts;
@@ -1280,6 +1339,7 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
                 return fail(rc);
             if ((rc = sim_alloc(s, (void**)&s->chain_ticket, sizeof(int)))) return fail(rc);
             if ((rc = sim_alloc(s, (void**)&s->u, sizeof(double) * s->np))) return fail(rc);
+            if ((rc = sim_alloc(s, (void**)&s->bxi, sizeof(double) * s->np))) return fail(rc);
             if (!hip_ok(hipMemsetAsync(s->chain_words, 0, sizeof(unsigned long long) * 2 * s->np, s->stream),
                         "solve words") ||
                 !hip_ok(hipMemsetAsync(s->chain_ticket, 0, sizeof(int), s->stream), "solve ticket") ||
@@ -1549,7 +1609,7 @@ extern "C" int pbx_sim_restore(pbx_sim* s, const void* buf, long long bytes) {
         PBX_HIP(sim_copy(s, s->Wchol, p, sizeof(double) * std::min<long long>(h.wdoubles, pbx_dense_work_doubles(s->np)),
                           hipMemcpyHostToDevice));
     s->have_D = h.have_D;
-    s->D_pending = 0;
+    s->mat_state = 0;
     s->have_B = h.have_B;
     s->have_Rff = h.have_Rff;
     s->last_attempts = h.last_attempts;
@@ -2171,7 +2231,10 @@ extern "C" int pbx_sim_get_matrix(pbx_sim* s, int which, double* out) {
         TRY(ensure_D(s));
         src = s->D;
     }
-    if (which == 1) src = s->B;
+    if (which == 1) {
+        TRY(ensure_B(s));
+        src = s->B;
+    }
     if (which == 2) src = s->Rff;
     if (!src) {
         pbx_set_error(PBX_ERR_ARG, "matrix %d not held by this simulation", which);

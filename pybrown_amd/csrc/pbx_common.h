@@ -188,6 +188,12 @@ int pbx_fill_pad(double* A, int n, int np, int ld, hipStream_t s);
 int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* Wi, int n, int np, int ld,
                            double kBT, int* info_dev, hipStream_t s, int form_D = 1);
 int pbx_inverse_and_factor_finish(double* R, double* X, int n, int np, int ld, double kBT, hipStream_t s);
+// L := chol(J R J) (pad diagonal kBT), tile inverses in W; then, when B
+// itself is wanted, B := sqrt(kBT) J inv(L)^T J in place of L, W := B's tile
+// inverses, Xout := inv(L)
+int pbx_factor_reversed(const double* R, double* L, double* W, int n, int np, int ld, double kBT, int* info_dev,
+                        hipStream_t s);
+int pbx_factor_reversed_finish(double* L, double* W, double* Xout, int np, int ld, double kBT, hipStream_t s);
 int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s);
 int pbx_potrf_copy(const double* D, double* B, int np, int ld, double* work, int* info_dev, hipStream_t s,
                    int mixed);

@@ -26,6 +26,7 @@
 // inv(L_kk) is produced with each diagonal factorisation and kept in `work`
 // (T tiles of 64 x 64), where the SPD inverse reuses it.
 #include <atomic>
+#include <vector>
 
 #include "pbx_common.h"
 #include "pbx_tile.h"
@@ -175,6 +176,35 @@ __device__ __forceinline__ void trailing_decode(int r, int m, bool skip0, int ns
         ii = c + 16 + (loc - tri) / 16;
         jj = c + (loc - tri) % 16;
     }
+}
+
+// Number of trailing tiles (rows j .. T-1 of each column j) of the
+// tile-column pairs q = q0, q0 + K, ... (columns 2q and 2q + 1 below T).
+__host__ __device__ inline int far_count(int T, int q0, int K) {
+    int n = 0;
+    for (int q = q0; 2 * q < T; q += K)
+        for (int j = 2 * q; j < 2 * q + 2 && j < T; ++j) n += T - j;
+    return n;
+}
+
+// Tile (i, j) (absolute indices) of the r-th workgroup of such a slice.
+// As in trailing_decode, residue class r % ns takes one contiguous chunk of
+// the sequence (columns in order, rows top-down), so an XCD keeps one
+// column's panel rows in its L2 while the row panels stream past.
+__device__ __forceinline__ void far_decode(int r, int T, int q0, int K, int ns, int& i, int& j) {
+    const int Nt = far_count(T, q0, K);
+    const int s = r % ns, k = r / ns;
+    const int base = Nt / ns, extra = Nt % ns;
+    int t = s * base + min(s, extra) + k;
+    for (int q = q0;; q += K)  // ends: t < Nt
+        for (int c = 2 * q; c < 2 * q + 2 && c < T; ++c) {
+            if (t < T - c) {
+                j = c;
+                i = c + t;
+                return;
+            }
+            t -= T - c;
+        }
 }
 
 // A_ij -= L_ik L_jk^T over the trailing lower tiles; the workgroup owning
@@ -808,7 +838,8 @@ __device__ __forceinline__ void fused_between(const double* sW, double* sC, doub
 template <int V, int OCC, bool MIXED>
 __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
                                                         int upd_target, double* work, double* scr, int* info,
-                                                        int* la, int chain_jobs, int Pt, int near) {
+                                                        int* la, int chain_jobs, int Pt, int near, int Pf, int fq0,
+                                                        int fK) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
@@ -1105,21 +1136,30 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         }
         return;
     }
-    // trailing tile (i, j), j >= c0 + nc, less its Pt lagging panels; with
+    // trailing tile (i, j), j >= c0 + nc, less its lagging panels; with
     // two columns, tile (c0+2, c0+2) (first in this order) belongs to row
-    // c0+2.  `near`: only the next launch's two tile columns (the rest wait
-    // for the launch after, which applies four panels at once)
-    int ii, jj;
+    // c0+2.  Without `near`: every trailing tile, Pt panels.  With `near`:
+    // the next launch's two tile columns (Pt panels), then this launch's
+    // slice of the far tile columns (Pf panels; tile-column pairs fq0,
+    // fq0 + fK, ..., every trailing tile of the pair's columns).
+    int ii, jj, Pu = Pt;
     const int r = b - 1 - nh - nw, m = T - c0 - nc;
     if (near) {
-        ii = 1 + (r < m - 1 ? r : r - (m - 1));
-        jj = r < m - 1 ? 0 : 1;
+        if (r < 2 * m - 2) {
+            ii = 1 + (r < m - 1 ? r : r - (m - 1));
+            jj = r < m - 1 ? 0 : 1;
+        } else {
+            far_decode(r - (2 * m - 2), T, fq0, fK, 8, ii, jj);
+            ii -= c0 + nc;
+            jj -= c0 + nc;
+            Pu = Pf;
+        }
     } else {
         trailing_decode(r, m, nc == 2, OCC == 1 ? 7 : 8, ii, jj);
     }
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
-    update_tile<MIXED>(A, ld, i, j, c0 - Pt, Pt, sA, sB, u);
+    update_tile<MIXED>(A, ld, i, j, c0 - Pu, Pu, sA, sB, u);
     double* X = tile(i, j);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -1661,29 +1701,53 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     const size_t dyn_lds = own_cu ? 2 * T64 * SP * sizeof(double) : 0;
     if (!la_ready) PBX_LAUNCH(la_reset_kernel, dim3(1), dim3(64), 0, s, la);
     int upd_target = 0;
-    // Large orders (two workgroups per CU) update the whole trailing matrix
-    // every K launches only: every trailing tile not touched since the last
-    // such update (at column c_far) lags the panels c_far .. c0-1, and a
-    // launch updates all of them once that lag reaches 2K panels, as one
-    // rank-128K product, and otherwise only the next launch's two tile
-    // columns.  Each far tile is then read and written once per 2K columns
-    // (C4, n = 12000: 41 GB of HBM traffic per factorisation with K = 1,
-    // 28.5 GB with K = 2; K = 4, the default, was the fastest of 1, 2, 3, 4
-    // and 6 there).  Orders up to the own-CU limit are bound by the diagonal
-    // chain and take K = 1, every trailing tile in every launch (K = 2 cost
-    // 3 % at n = 3000).  The row and helper tiles of a launch always lag
-    // two panels.
+    // Large orders (two workgroups per CU) update each far trailing tile
+    // once per K launches (2K panels at once, a rank-128K product), so each
+    // far tile is read and written once per 2K columns (C4, n = 12000:
+    // 41 GB of HBM traffic per factorisation with K = 1, 28.5 GB with K = 2;
+    // K = 4, the default, was the fastest of 1, 2, 3, 4 and 6 there).  The
+    // far updates roll: tile-column pair q (columns 2q, 2q+1) takes its
+    // lagging panels in the launches L = c0/2 with L = q mod K, and the rest
+    // in launch q-1, the one before it becomes a row pair (its `near`
+    // columns).  Each launch then carries 1/K of the far work beside its
+    // diagonal chain (n = 7232: 4.71 -> 4.54 ms against all of it in every
+    // K-th launch).  The pairs of a launch's slice were last updated
+    // together (launch L-K, or never), so one lag Pf serves them all, as one
+    // lag Pt serves the near pair.  Orders up to the own-CU limit are bound
+    // by the diagonal chain and take K = 1, every trailing tile in every
+    // launch (K = 2 cost 3 % at n = 3000).  The row and helper tiles of a
+    // launch always lag two panels.
     const int K = own_cu ? 1 : g_far_period.load();
     int c_far = 0;
+    std::vector<int> upd(own_cu ? 0 : (T + 1) / 2 + 2, 0);  // per pair: panels applied, 0 .. upd-1
     for (int c0 = 0, P = 0; c0 < T;) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
         const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
         const int nw = T - c0 - nc;
-        const int Pt = c0 - c_far;
-        const int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
-        if (Pt > 0 && !near) c_far = c0;
+        int Pt = c0 - c_far;
+        int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
+        int Pf = 0, fq0 = 0, nfar = 0;
+        if (own_cu) {
+            if (Pt > 0 && !near) c_far = c0;
+        } else if (nc < 2 || nw == 0) {
+            Pt = near = 0;  // no tile beyond the launch's own columns
+        } else {
+            const int L = c0 / 2, qn = L + 1;
+            Pt = c0 - upd[qn];
+            upd[qn] = c0;
+            near = Pt > 0 ? 1 : 0;
+            fq0 = qn + 1;
+            while (fq0 % K != L % K) ++fq0;
+            if (2 * fq0 < T) {
+                Pf = c0 - upd[fq0];
+                for (int q = fq0; 2 * q < T; q += K) upd[q] = c0;
+                nfar = Pf > 0 ? far_count(T, fq0, K) : 0;
+            }
+        }
         int ntr = 0;
-        if (Pt > 0 && nw > 0)
+        if (!own_cu)
+            ntr = near ? 2 * nw - 2 + nfar : 0;
+        else if (Pt > 0 && nw > 0)
             ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
         upd_target += nh;
         // own-CU build: roles 1 .. W sit at the indices that are not
@@ -1698,11 +1762,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         if (own_cu)
             PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
                                c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near);
+                               g_chain_jobs, Pt, near, 0, 0, 1);
         else
             PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
                                P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near);
+                               g_chain_jobs, Pt, near, Pf, fq0, K);
         P = nc;
         c0 += nc;
     }
@@ -1883,6 +1947,37 @@ int pbx_inverse_and_factor(double* R, double* X, double* B, double* Wc, double* 
     PBX_LAUNCH(reverse_transpose_lower_kernel, dim3(g), dim3(256), 0, s, R, B, np, ld, c);
     PBX_LAUNCH(reverse_diag_inverse_kernel, dim3(T), dim3(256), 0, s, X, Wc, T, ld, 1.0 / c);
     if (form_D) return pbx_inverse_and_factor_finish(R, X, n, np, ld, kBT, s);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+// The factorisation half of pbx_inverse_and_factor: L (np x np) := the
+// Cholesky factor of J R J (pad diagonal kBT), the inverses of its diagonal
+// tiles in W (a factorisation work area).  R is left unchanged.  With
+// B = sqrt(kBT) J L^{-T} J, B xi and B^{-T} xi take one triangular solve and
+// one triangular product with L; pbx_factor_reversed_finish forms B itself.
+int pbx_factor_reversed(const double* R, double* L, double* W, int n, int np, int ld, double kBT, int* info_dev,
+                        hipStream_t s) {
+    if (np % T64 != 0 || np <= 0 || n <= 0 || n > np || !(kBT > 0.0)) {
+        pbx_set_error(PBX_ERR_ARG, "factor_reversed: n %d np %d kBT %g", n, np, kBT);
+        return PBX_ERR_ARG;
+    }
+    const unsigned g = (unsigned)(((size_t)np * np + 255) / 256);
+    PBX_LAUNCH(reverse_sym_kernel, dim3(g), dim3(256), 0, s, R, L, n, np, ld, 1.0, kBT, 1);
+    return pbx_potrf(L, np, ld, W, info_dev, s, 0);
+}
+
+// B (in place of L) and the inverses of its diagonal tiles (in place of W)
+// from pbx_factor_reversed's L and W, with X = inv(L) left in Xout (lower
+// tiles) for pbx_inverse_and_factor_finish: the same operations as
+// pbx_inverse_and_factor, so the same values.
+int pbx_factor_reversed_finish(double* L, double* W, double* Xout, int np, int ld, double kBT, hipStream_t s) {
+    const int T = np / T64;
+    const double c = sqrt(kBT);
+    const unsigned g = (unsigned)(((size_t)np * np + 255) / 256);
+    trtri<false>(L, T, ld, W, Xout, s);
+    PBX_LAUNCH(reverse_diag_inverse_kernel, dim3(T), dim3(256), 0, s, L, W, T, ld, 1.0 / c);
+    PBX_LAUNCH(reverse_transpose_lower_kernel, dim3(g), dim3(256), 0, s, Xout, L, np, ld, c);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

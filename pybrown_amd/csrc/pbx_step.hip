@@ -216,6 +216,48 @@ __global__ __launch_bounds__(256) void drift_solved_kernel(const double* __restr
     if ((threadIdx.x & 63) == 0) y[r] = alpha * (c * v[r] - s);
 }
 
+// y = alpha (c v - bx): the same drift with B0 xi already formed (bx)
+__global__ void drift_vec_kernel(const double* __restrict__ v, const double* __restrict__ bx, int n, double c,
+                                 double alpha, double* y) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) y[r] = alpha * (c * v[r] - bx[r]);
+}
+
+// y = scale J L J x over the whole order np (J the reversal, L lower
+// triangular): y[i] = scale sum_{c <= r} L[r][c] x[np-1-c], r = np-1-i.
+// With L the factor of J R J this is B^{-T} x for the noise factor
+// B = sqrt(kBT) J L^{-T} J of D = kBT inv(R) (scale 1 / sqrt(kBT)).
+__global__ __launch_bounds__(256) void rev_lower_gemv_kernel(const double* __restrict__ Lm, int np, int ld,
+                                                             const double* __restrict__ x, double scale,
+                                                             double* y) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= np) return;
+    const int r = np - 1 - i, lane = threadIdx.x & 63;
+    const double* row = Lm + (size_t)r * ld;
+    double s0 = 0.0, s1 = 0.0;
+    int c = lane;
+    for (; c + 64 <= r; c += 128) {
+        s0 += row[c] * x[np - 1 - c];
+        s1 += row[c + 64] * x[np - 1 - (c + 64)];
+    }
+    if (c <= r) s0 += row[c] * x[np - 1 - c];
+    const double t = wave_sum(s0 + s1);
+    if (lane == 0) y[i] = scale * t;
+}
+
+// Stochastic sub-step from B xi already formed (bx): r += (bx sqrt(2 dt)) mult
+__global__ void stoch_vec_kernel(const double* __restrict__ bx, double sq2dt, double mult, double* pos, double* disp,
+                                 int N, FluxPlane f, const int* label, long long* flux) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= N) return;
+    double v[3];
+    for (int c = 0; c < 3; ++c) {
+        v[c] = (bx[3 * b + c] * sq2dt) * mult;
+        disp[3 * b + c] = v[c];
+    }
+    move_bead(pos, v, b, f, label, flux);
+}
+
 // ---------------------------------------------------------------------------
 // Triangular solves with lower-triangular L (order np = 64 T) given the
 // inverses W_k of its diagonal tiles:
@@ -307,12 +349,14 @@ __device__ __forceinline__ void chain_tile(const double* __restrict__ Lm, int ld
     }
 }
 
+// rev: x and y in reversed order (x'[i] = x[np-1-i]), y scaled by yscale
+// (the hand-off words carry the unscaled solution)
 template <bool TR>
 __global__ __launch_bounds__(256) void trsv_chain_kernel(const double* __restrict__ Lm, int np, int ld,
                                                          const double* __restrict__ W,
                                                          const double* __restrict__ x, double* __restrict__ y,
                                                          unsigned long long* words, int* ticket, unsigned ep,
-                                                         int* info) {
+                                                         int* info, int rev, double yscale) {
     __shared__ double part[3][64];
     __shared__ int done;
     __shared__ int row;
@@ -347,7 +391,8 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const double* __restric
                 wk[2 * j + 1] = 2 * j + 1 <= o ? t.y : 0.0;
             }
         }
-        double r = x[64 * k + o];
+        const int idx = rev ? np - 1 - (64 * k + o) : 64 * k + o;
+        double r = x[idx];
         if (ndep > 1) {
             int it = 0;
             while (__hip_atomic_load(&done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < 3) {
@@ -366,7 +411,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const double* __restric
         }
         const double yk = lane_dot(wk, r, 0.0);
         chain_put(words + (size_t)2 * (64 * k + o), yk, ep);
-        y[64 * k + o] = yk;
+        y[idx] = yscale * yk;
     } else {
         double acc = 0.0;
         for (int q = wv - 1; q < ndep - 1; q += 3) {
@@ -780,18 +825,46 @@ void pbx_status_sequence_queued(int* info_dev, hipStream_t s);
 
 int pbx_launch_trsv_chain(const double* Lm, int np, int ld, const double* W, const double* x, double* y,
                           unsigned long long* words, int* ticket, unsigned epoch, int trans, int* info,
-                          hipStream_t s) {
+                          hipStream_t s, int rev, double yscale) {
     if (np <= 0 || np % 64 || ld < np || ld % 2 || epoch == 0) {
         pbx_set_error(PBX_ERR_ARG, "trsv_chain: np=%d ld=%d epoch=%u", np, ld, epoch);
         return PBX_ERR_ARG;
     }
     if (trans)
         PBX_LAUNCH(trsv_chain_kernel<true>, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, words, ticket, epoch,
-                   info);
+                   info, rev, yscale);
     else
         PBX_LAUNCH(trsv_chain_kernel<false>, dim3(np / 64), dim3(256), 0, s, Lm, np, ld, W, x, y, words, ticket,
-                   epoch, info);
+                   epoch, info, rev, yscale);
     pbx_status_sequence_queued(info, s);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_drift_vec(const double* v, const double* bx, int n, double c, double alpha, double* y, hipStream_t s) {
+    PBX_LAUNCH(drift_vec_kernel, dim3((n + 255) / 256), dim3(256), 0, s, v, bx, n, c, alpha, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_rev_lower_gemv(const double* Lm, int np, int ld, const double* x, double scale, double* y,
+                              hipStream_t s) {
+    PBX_LAUNCH(rev_lower_gemv_kernel, dim3((np + 3) / 4), dim3(256), 0, s, Lm, np, ld, x, scale, y);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_stoch_vec(const double* bx, double sq2dt, double mult, double* pos, double* disp, int N,
+                         const double* plane_n, const double* plane_p, int flux_on, const int* label,
+                         long long* flux, hipStream_t s) {
+    FluxPlane f;
+    for (int c = 0; c < 3; ++c) {
+        f.n[c] = plane_n ? plane_n[c] : 0.0;
+        f.p[c] = plane_p ? plane_p[c] : 0.0;
+    }
+    f.on = flux_on;
+    PBX_LAUNCH(stoch_vec_kernel, dim3((N + 127) / 128), dim3(128), 0, s, bx, sq2dt, mult, pos, disp, N, f, label,
+               flux);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }

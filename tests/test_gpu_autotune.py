@@ -63,10 +63,10 @@ def test_every_shape_factors_correctly(n, clean_table):
 
 @pytest.mark.parametrize("K", [1, 2, 3, 4])
 def test_whole_trailing_update_period(K):
-    """Orders above 64 tile columns (two workgroups per CU) update the whole
-    trailing matrix every K launches and the next launch's two tile columns
-    in between; every period factors to numpy's factor (n = 4500, 71 tile
-    columns, 36 launches)."""
+    """Orders above 64 tile columns (two workgroups per CU) update each far
+    tile column pair every K launches (a rolling 1/K of them per launch) and
+    the next launch's two tile columns in every launch; every period factors
+    to numpy's factor (n = 4500, 71 tile columns, 36 launches)."""
     from pybrown_amd._lib import lib
     from pybrown_amd.diffusion import cholesky
     A = _spd(4500, 11)

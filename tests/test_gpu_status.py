@@ -10,9 +10,10 @@ R inverse-and-factor):
       inverse-and-factor, then two inverses per coordinate in the divergence
       (box.py:560-610);
   midpoint:  the Cholesky, or with lubrication the Rff inverse and the R
-      inverse-and-factor; [the midpoint Rff inverse and R inverse (with no
-      forces: R's factorisation),] the B0^T solve, [with lubrication and no
-      forces: the two solves with R's factor] (box.py:254-349)."""
+      inverse-and-factor; [the midpoint Rff and R inverses,] the B0^T solve
+      (box.py:254-349).  With lubrication and no forces: the Rff inverse,
+      the factorisation of J R J; per attempt the B0 xi solve, the midpoint
+      Rff inverse, the factorisation of R_mid and the two solves with it."""
 import numpy as np
 import pytest
 
@@ -55,12 +56,12 @@ def test_divergence_inverse_failure_fails_its_step(overlaps):
     _failing_step_raises(inp, 4)
 
 
-@pytest.mark.parametrize("hydro,k", [("rpy_smith", 2), ("rpy_smith_lub", 5), ("rpy_smith_lub", 6),
-                                     ("rpy_smith_lub", 7)])
+@pytest.mark.parametrize("hydro,k", [("rpy_smith", 2), ("rpy_smith_lub", 3), ("rpy_smith_lub", 5),
+                                     ("rpy_smith_lub", 6), ("rpy_smith_lub", 7)])
 def test_midpoint_solve_failure_fails_its_step(hydro, k):
     inp = {"box_length": 12.0, "T": 298.0, "viscosity": 0.01, "seed": 5, "hydrodynamics": hydro,
            "ewald_alpha": np.sqrt(np.pi), "ewald_real": 1, "ewald_imag": 1, "lubrication_cutoff": 1.0,
            "propagation_scheme": "midpoint", "m_midpoint": 100.0, "check_overlaps": False}
-    # the k-th sequence is the B0^T solve (with lubrication, 6 and 7 the
-    # solves with R_mid's factor)
+    # the k-th sequence: the B0^T solve; with lubrication the B0 xi solve (3),
+    # R_mid's factorisation (5) and the solves with it (6, 7)
     _failing_step_raises(inp, k)
