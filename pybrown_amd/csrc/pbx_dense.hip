@@ -426,13 +426,16 @@ __device__ __forceinline__ void publish_agent(int* flag, int v) {
 
 // S += sum_{m = m0 .. m0+P-1} L_im L_jm^T (i == j: one operand).  Other
 // waves may still read sA/sB on return: synchronise before reusing them.
-// LEAN: one panel's operands in flight at a time (fewer registers).
-template <bool LEAN = false>
+// LEAN: one panel's operands in flight at a time (fewer registers).  PIPE:
+// the next panel's operands in flight during each product (the trailing
+// tiles of the two-workgroups-per-CU build: 1.5 % at n = 12000, and no
+// register spill there; the pairs below suit the one-per-CU build).
+template <bool LEAN = false, bool PIPE = false>
 __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int j, int m0, int P, double* sA,
                                              double* sB, v4d S[2][2]) {
     const double* Ai = A + (size_t)i * T64 * ld;
     const double* Aj = A + (size_t)j * T64 * ld;
-    if (!LEAN && P % 2 == 0) {
+    if (!LEAN && !PIPE && P % 2 == 0) {
         // panels in pairs: both panels' operand tiles are requested before
         // the pair's first product, so their latencies overlap each other and
         // that product
@@ -453,6 +456,25 @@ __device__ __forceinline__ void panel_update(const double* A, int ld, int i, int
             stage_put<true>(sA, a1);
             if (i != j) stage_put<true>(sB, b1);
             __syncthreads();
+            mma64(sA, i == j ? sA : sB, S);
+        }
+        return;
+    }
+    if (PIPE) {
+        // the next panel's operand tiles are requested before this panel's
+        // product, so their latency overlaps it
+        v2d fa[8], fb[8];
+        stage_fetch(fa, Ai + (size_t)m0 * T64, ld);
+        if (i != j) stage_fetch(fb, Aj + (size_t)m0 * T64, ld);
+        for (int m = m0; m < m0 + P; ++m) {
+            if (m > m0) __syncthreads();
+            stage_put<true>(sA, fa);
+            if (i != j) stage_put<true>(sB, fb);
+            __syncthreads();
+            if (m + 1 < m0 + P) {
+                stage_fetch(fa, Ai + (size_t)(m + 1) * T64, ld);
+                if (i != j) stage_fetch(fb, Aj + (size_t)(m + 1) * T64, ld);
+            }
             mma64(sA, i == j ? sA : sB, S);
         }
         return;
@@ -516,7 +538,7 @@ struct Updated {
     __device__ __forceinline__ int col(int q) const { return acc_col((q >> 2) & 1); }
 };
 
-template <bool MIXED, bool LEAN = false>
+template <bool MIXED, bool LEAN = false, bool PIPE = false>
 __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int j, int m0, int P, double* sA,
                                             double* sB, Updated& u) {
     const double* Aij = A + (size_t)i * T64 * ld + (size_t)j * T64;
@@ -566,7 +588,7 @@ __device__ __forceinline__ void update_tile(const double* A, int ld, int i, int 
     acc_load(Aij, ld, u.v);
     v4d S[2][2];
     acc_zero(S);
-    if (P > 0) panel_update<LEAN>(A, ld, i, j, m0, P, sA, sB, S);
+    if (P > 0) panel_update<LEAN, PIPE>(A, ld, i, j, m0, P, sA, sB, S);
     int q = 0;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -993,7 +1015,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         double* A0 = tile(i, c0);
         if (nc == 1) {
             Updated u;
-            update_tile<MIXED>(A, ld, i, c0, c0 - P, P, sA, sB, u);
+            update_tile<MIXED, false, OCC == 2>(A, ld, i, c0, c0 - P, P, sA, sB, u);
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
             if (!wait_ge_agent(la + LA_FLAG, c0 + 1, err, info)) return;
@@ -1010,7 +1032,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         const bool diag = (i == c0 + 2);  // also finishes tile (i, i) for the next launch
         if (diag && Pt > 0) {  // (a trailing tile: it lags Pt panels)
             Updated u;
-            update_tile<MIXED>(A, ld, i, i, c0 - Pt, Pt, sA, sB, u);
+            update_tile<MIXED, false, OCC == 2>(A, ld, i, i, c0 - Pt, Pt, sA, sB, u);
             store_updated(Aii, ld, u, true);
             // other threads read these values back (lower-block layout)
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -1021,7 +1043,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         // reads back the values it wrote)
         if (P > 0) {
             Updated u;
-            update_tile<MIXED>(A, ld, i, c0 + 1, c0 - P, P, sA, sB, u);
+            update_tile<MIXED, false, OCC == 2>(A, ld, i, c0 + 1, c0 - P, P, sA, sB, u);
             store_updated(A1, ld, u, false);
             __syncthreads();
         }
@@ -1029,7 +1051,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         v4d L0[2][2];
         {
             Updated u;
-            update_tile<MIXED>(A, ld, i, c0, c0 - P, P, sA, sB, u);
+            update_tile<MIXED, false, OCC == 2>(A, ld, i, c0, c0 - P, P, sA, sB, u);
             if (P > 0) __syncthreads();
             put_updated_t(sA, u);
         }
@@ -1159,7 +1181,7 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     }
     const int i = c0 + nc + ii, j = c0 + nc + jj;
     Updated u;
-    update_tile<MIXED>(A, ld, i, j, c0 - Pu, Pu, sA, sB, u);
+    update_tile<MIXED, false, OCC == 2>(A, ld, i, j, c0 - Pu, Pu, sA, sB, u);
     double* X = tile(i, j);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -1262,8 +1284,14 @@ __device__ __forceinline__ void level_product(int k0, int k1, FA tileA, FB tileB
 
 template <int PHASE, bool MIXED>
 __global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int ld, double* X, int ldx, int T, int b) {
-    const int pair = blockIdx.x / (b * b), rem = blockIdx.x - pair * b * b;
-    const int p = rem / b, q = rem - p * b;
+    // Workgroups in order of decreasing chain length, so the short chains
+    // fill the launch's tail: phase 0's tile (p, q) sums b - q products,
+    // phase 1's p + 1.  Index = (length rank, pair, other coordinate).
+    const int npairs = (T + b - 1) / (2 * b);
+    const int rank = blockIdx.x / (npairs * b), rem2 = blockIdx.x - rank * npairs * b;
+    const int pair = rem2 / b, o = rem2 - pair * b;
+    const int p = PHASE == 0 ? o : b - 1 - rank;
+    const int q = PHASE == 0 ? rank : o;
     const int s = pair * 2 * b;
     const int h2 = (s + 2 * b < T ? s + 2 * b : T) - (s + b);
     if (p >= h2) return;  // whole workgroup: B2 is shorter than b
