@@ -1374,6 +1374,95 @@ __global__ __launch_bounds__(256) void residual_kernel(const double* D, const do
 // Y_ij = sum_{m >= i} X_mi^T X_mj for i >= j.  MIXED: products on the FP32
 // matrix cores (operands rounded to FP32, FP32 accumulation within each
 // 64-deep slab, FP64 across slabs).
+// The same in FP64 over 128 x 128 blocks (tile rows 2P, 2P+1 by tile
+// columns 2Q, 2Q+1, P >= Q; half blocks at an odd T's edge): Y_ij =
+// scale sum_{m >= i} X_mi^T X_mj for the block's lower tiles.  Both
+// operands are slabs of X's rows (X[64m + k][128P + r] and [128Q + c]),
+// staged 32 rows at a time as they lie, with the next slab's loads in
+// flight during each product; wave w holds the 64 x 64 quadrant (w/2, w%2)
+// as 4 x 4 MFMA blocks (half the LDS reads per product of the 64 x 64
+// tiles' 2 x 2).  Row m = 2P adds to tile row 2P only (X_{2P,2P+1} is
+// above the diagonal: zero here), so every tile sums its products over m
+// and k in the order lauum_kernel does: the same values.
+__global__ __launch_bounds__(256, 2) void lauum128_kernel(const double* X, int ldx, double* Y, int ld, int T,
+                                                          double scale) {
+    constexpr int KC = 32, SPX = 129, NV = 128 * KC / 2 / 256;
+    __shared__ double sA[KC * SPX];
+    __shared__ double sB[KC * SPX];
+    int P, Q;
+    tri_decode(blockIdx.x, P, Q);  // P ascending: the longest sums first
+    const int wA = 64 * T - 128 * P < 128 ? 64 : 128, wB = 64 * T - 128 * Q < 128 ? 64 : 128;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 64, c0 = (w & 1) * 64;
+    const double* XA = X + 128 * P;
+    const double* XB = X + 128 * Q;
+    v2d fa[NV], fb[NV];
+    // slab rows k0 .. k0+31 of the matrix (row 64m + kk): columns beyond the
+    // block's width, and on row tile 2P the columns of tile 2P+1, are zero
+    auto fetch = [&](int k0) {
+        const int lim = k0 < 128 * P + 64 ? 64 : 128;  // row tile 2P: tile 2P+1 of A (and B when P == Q) is zero
+#pragma unroll
+        for (int it = 0; it < NV; ++it) {
+            const int e = threadIdx.x + it * 256, kk = e / 64, col = (e % 64) * 2;
+            const size_t off = (size_t)(k0 + kk) * ldx + col;
+            fa[it] = (col < wA && col < lim) ? *(const v2d*)(XA + off) : (v2d){0.0, 0.0};
+            fb[it] = (col < wB && (P != Q || col < lim)) ? *(const v2d*)(XB + off) : (v2d){0.0, 0.0};
+        }
+    };
+    v4d acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int kbeg = 128 * P, kend = 64 * T;
+    fetch(kbeg);
+    for (int k0 = kbeg; k0 < kend; k0 += KC) {
+        if (k0 > kbeg) __syncthreads();
+#pragma unroll
+        for (int it = 0; it < NV; ++it) {
+            const int e = threadIdx.x + it * 256, kk = e / 64, col = (e % 64) * 2;
+            sA[kk * SPX + col] = fa[it].x;
+            sA[kk * SPX + col + 1] = fa[it].y;
+            sB[kk * SPX + col] = fb[it].x;
+            sB[kk * SPX + col + 1] = fb[it].y;
+        }
+        __syncthreads();
+        if (k0 + KC < kend) fetch(k0 + KC);
+#pragma unroll 2
+        for (int k = 0; k < KC; k += 4) {
+            const double* pa = sA + (k + lk) * SPX;
+            const double* pb = sB + (k + lk) * SPX;
+            double a[4], b[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) a[x] = pa[r0 + 16 * x + li];
+#pragma unroll
+            for (int y = 0; y < 4; ++y) b[y] = pb[c0 + 16 * y + li];
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] = mfma(a[x], b[y], acc[x][y]);
+        }
+    }
+    // tile (i, j) of this quadrant; above the diagonal or beyond the edge: none
+    const int i = 2 * P + (r0 >> 6), j = 2 * Q + (c0 >> 6);
+    if (i >= T || j >= T || j > i) return;
+    double* Yij = Y + (size_t)i * T64 * ld + (size_t)j * T64;
+    double* Yji = Y + (size_t)j * T64 * ld + (size_t)i * T64;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * x + lk + 4 * v, c = 16 * y + li;
+                const double sv = acc[x][y][v] * scale;
+                if (i != j || c <= r) {
+                    Yij[(size_t)r * ld + c] = sv;
+                    Yji[(size_t)c * ld + r] = sv;
+                }
+            }
+}
+
 template <bool MIXED>
 __global__ __launch_bounds__(256) void lauum_kernel(const double* X, int ldx, double* Y, int ld, int T,
                                                     double scale) {
@@ -1907,7 +1996,12 @@ template <bool MIXED>
 static void trtri_lauum(double* A, int T, int ld, double* work, double* X, hipStream_t s, double scale) {
     // every lower tile of X is written below before lauum reads it
     trtri<MIXED>(A, T, ld, work, X, s);
-    PBX_LAUNCH(lauum_kernel<MIXED>, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+    if (MIXED) {
+        PBX_LAUNCH(lauum_kernel<MIXED>, dim3(T * (T + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+    } else {
+        const int nb = (T + 1) / 2;
+        PBX_LAUNCH(lauum128_kernel, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, X, ld, A, ld, T, scale);
+    }
 }
 
 // A := scale * inv(A) for SPD A (both triangles written).  X is an np x np
@@ -2015,7 +2109,8 @@ int pbx_factor_reversed_finish(double* L, double* W, double* Xout, int np, int l
 int pbx_inverse_and_factor_finish(double* R, double* X, int n, int np, int ld, double kBT, hipStream_t s) {
     const int T = np / T64;
     const unsigned g = (unsigned)(((size_t)np * np + 255) / 256);
-    PBX_LAUNCH(lauum_kernel<false>, dim3(T * (T + 1) / 2), dim3(256), 0, s, R, ld, X, ld, T, kBT);  // X := kBT X^T X
+    const int nb = (T + 1) / 2;
+    PBX_LAUNCH(lauum128_kernel, dim3(nb * (nb + 1) / 2), dim3(256), 0, s, R, ld, X, ld, T, kBT);  // X := kBT X^T X
     PBX_LAUNCH(reverse_sym_kernel, dim3(g), dim3(256), 0, s, X, R, n, np, ld, 1.0, 1.0, 0);        // R := D
     PBX_HIP(hipGetLastError());
     return PBX_OK;
