@@ -1312,6 +1312,111 @@ __global__ __launch_bounds__(256) void trtri_level_kernel(const double* A, int l
     }
 }
 
+// The same level in FP64 over 128 x 128 blocks of tile pairs (b even):
+// block (P, Q) holds the tiles p = 2P, 2P+1 and q = 2Q, 2Q+1 of one pair's
+// product (rows p >= h2 absent).  32-deep slices of both operands pass
+// through LDS with the next slice's loads in flight, and wave w holds the
+// 64 x 64 quadrant (w/2, w%2) as 4 x 4 MFMA blocks (half the LDS reads per
+// product of the 64 x 64 tiles).  A block sums over the k range of its
+// longest tile; the terms outside a tile's own range have a zero operand
+// there (phase 0: X(s+2Q, s+2Q+1), above the diagonal, for q = 2Q+1;
+// phase 1: k = 2P+1 for p = 2P), so each tile sums its products in
+// trtri_level_kernel's order: the same values.
+template <int PHASE>
+__global__ __launch_bounds__(256, 2) void trtri128_kernel(const double* A, int ld, double* X, int ldx, int T, int b) {
+    constexpr int KC = 32, SPX = 129, NV = 128 * KC / 2 / 256;
+    __shared__ double sA[KC * SPX];
+    __shared__ double sB[KC * SPX];
+    const int hb = b / 2, npairs = (T + b - 1) / (2 * b);
+    // decreasing chain length: phase 0's block (P, Q) sums b - 2Q tiles,
+    // phase 1's up to 2P + 2
+    const int rank = blockIdx.x / (npairs * hb), rem2 = blockIdx.x - rank * npairs * hb;
+    const int pair = rem2 / hb, o = rem2 - pair * hb;
+    const int P = PHASE == 0 ? o : hb - 1 - rank;
+    const int Q = PHASE == 0 ? rank : o;
+    const int s = pair * 2 * b;
+    const int h2 = (s + 2 * b < T ? s + 2 * b : T) - (s + b);
+    if (2 * P >= h2) return;
+    const int rows = 2 * P + 1 < h2 ? 128 : 64;  // rows of tile p = 2P+1 exist
+    const int kt0 = PHASE == 0 ? 2 * Q : 0;
+    const int kt1 = PHASE == 0 ? b - 1 : (2 * P + 1 < h2 ? 2 * P + 1 : 2 * P);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+    const int r0 = (w >> 1) * 64, c0 = (w & 1) * 64;
+    v2d fa[NV], fb[NV];
+    // this thread's staging coordinates: A rows rA + 16 it (the block's two
+    // tile rows lie consecutively), columns kc, kc + 1 of the slice; B slice
+    // rows kB + 4 it, block columns cB, cB + 1
+    const int rA = threadIdx.x / (KC / 2), kc = (threadIdx.x % (KC / 2)) * 2;
+    const int kB = threadIdx.x / 64, cB = (threadIdx.x % 64) * 2;
+    const double* Arow = PHASE == 0 ? A + (size_t)(T64 * (s + b + 2 * P) + rA) * ld
+                                    : X + (size_t)(T64 * (s + b + 2 * P) + rA) * ldx;
+    const int lda = PHASE == 0 ? ld : ldx;
+    const double* Bcol = PHASE == 0 ? X + (size_t)kB * ldx + T64 * (s + 2 * Q) + cB
+                                    : X + (size_t)(T64 * (s + 2 * Q + (cB >> 6)) + kB) * ldx + (cB & 63);
+    const bool bHigh = cB >= 64;
+    // slice kk0 .. kk0+31 of k tile kt
+    auto fetch = [&](int kt, int kk0) {
+        const double* pa = Arow + (PHASE == 0 ? T64 * (s + kt) : T64 * (s + b + kt)) + kk0 + kc;
+        const double* pb = PHASE == 0 ? Bcol + (size_t)(T64 * (s + kt) + kk0) * ldx
+                                      : Bcol + (size_t)kk0 * ldx + T64 * (s + b + kt);
+        const bool bLive = PHASE == 0 ? !(kt == 2 * Q && bHigh) : true;
+        const int aLo = (PHASE == 1 && kt == 2 * P + 1) ? 64 : 0;  // phase 1: tile row 2P takes no k = 2P+1
+#pragma unroll
+        for (int it = 0; it < NV; ++it) {
+            const int r = rA + 16 * it;
+            fa[it] = (r < rows && r >= aLo) ? *(const v2d*)(pa + (size_t)(16 * it) * lda) : (v2d){0.0, 0.0};
+            fb[it] = bLive ? *(const v2d*)(pb + (size_t)(4 * it) * ldx) : (v2d){0.0, 0.0};
+        }
+    };
+    v4d acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = (v4d){0.0, 0.0, 0.0, 0.0};
+    const int nsl = (kt1 - kt0 + 1) * (T64 / KC);  // slices, two per k tile
+    fetch(kt0, 0);
+    for (int t = 0; t < nsl; ++t) {
+            if (t > 0) __syncthreads();
+#pragma unroll
+            for (int it = 0; it < NV; ++it) {
+                const int r = rA + 16 * it, kk = kB + 4 * it;
+                sA[kc * SPX + r] = fa[it].x;
+                sA[(kc + 1) * SPX + r] = fa[it].y;
+                sB[kk * SPX + cB] = fb[it].x;
+                sB[kk * SPX + cB + 1] = fb[it].y;
+            }
+            __syncthreads();
+            if (t + 1 < nsl) fetch(kt0 + (t + 1) / 2, ((t + 1) & 1) * KC);
+#pragma unroll 2
+            for (int k = 0; k < KC; k += 4) {
+                const double* pa = sA + (k + lk) * SPX;
+                const double* pb = sB + (k + lk) * SPX;
+                double a[4], bb[4];
+#pragma unroll
+                for (int x = 0; x < 4; ++x) a[x] = pa[r0 + 16 * x + li];
+#pragma unroll
+                for (int y = 0; y < 4; ++y) bb[y] = pb[c0 + 16 * y + li];
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) acc[x][y] = mfma(a[x], bb[y], acc[x][y]);
+            }
+        }
+    const int p = 2 * P + (r0 >> 6), q = 2 * Q + (c0 >> 6);
+    if (p >= h2) return;
+    double* out = PHASE == 0 ? X + (size_t)T64 * (s + q) * ldx + (size_t)T64 * (s + b + p)
+                             : X + (size_t)T64 * (s + b + p) * ldx + (size_t)T64 * (s + q);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * x + lk + 4 * v, c = 16 * y + li;
+                out[(size_t)r * ldx + c] = PHASE == 0 ? acc[x][y][v] : -acc[x][y][v];
+            }
+}
+
 // Factorisation residual: acc[0] += ||D - B B^T||^2, acc[1] += ||D||^2 over
 // the leading n x n block (B lower; each strictly lower tile counts twice).
 // One workgroup per lower tile (i, j): E_ij = D_ij - sum_{k <= j} B_ik B_jk^T.
@@ -1681,7 +1786,12 @@ constexpr int kDefaultPotrf = 6;
 static int g_potrf_variant = kDefaultPotrf;
 static int g_own_cu_tiles = 64;
 static std::atomic<int> g_shape_generation{0};  // launch-shape changes (keys captured graphs)
-static std::atomic<int> g_far_period{4};  // K of the large orders' whole-trailing updates
+static std::atomic<int> g_far_period{4};
+// smallest level (block size in tiles) of the triangular inverse that runs
+// over 128 x 128 blocks: below it the levels have too few blocks to fill
+// the GPU with the larger ones (n = 7232: level 32 0.44 against 0.38 ms,
+// level 64 1.18 + 0.79 against 1.43 + 0.89 ms)
+constexpr int g_trtri128_min_b = 64;  // K of the large orders' whole-trailing updates
 static int g_chain_jobs = 0;
 extern "C" int pbx_set_potrf_chain_jobs(int on) {
     g_chain_jobs = on != 0;
@@ -1987,6 +2097,12 @@ static void trtri(const double* A, int T, int ld, const double* work, double* X,
     PBX_LAUNCH(copy_diag_inv_kernel, dim3(T), dim3(256), 0, s, X, ld, work);
     for (int b = 1; b < T; b *= 2) {
         const int npairs = (T - b + 2 * b - 1) / (2 * b);
+        if (!MIXED && b >= g_trtri128_min_b) {
+            const int hb = b / 2;
+            PBX_LAUNCH(trtri128_kernel<0>, dim3(npairs * hb * hb), dim3(256), 0, s, A, ld, X, ld, T, b);
+            PBX_LAUNCH(trtri128_kernel<1>, dim3(npairs * hb * hb), dim3(256), 0, s, A, ld, X, ld, T, b);
+            continue;
+        }
         PBX_LAUNCH((trtri_level_kernel<0, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
         PBX_LAUNCH((trtri_level_kernel<1, MIXED>), dim3(npairs * b * b), dim3(256), 0, s, A, ld, X, ld, T, b);
     }
