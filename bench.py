@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--no-c1-leg", action="store_true", help="c2: skip the small C1 leg")
     ap.add_argument("--no-served-leg", action="store_true", help="c2: skip the served request-stream sample")
     ap.add_argument("--max-n", type=int, default=0, help="c5: keep only the requests with N <= this")
+    ap.add_argument("--request", type=int, default=None, metavar="RID",
+                    help="c5: time stream request RID alone (--request-steps steps after 2 untimed ones)")
     ap.add_argument("--tune", action="store_true",
                     help="c5: time the factorisation's launch shapes per order first (pybrown_amd.autotune) and "
                          "serve with the tuned table")
@@ -409,6 +411,8 @@ def serve_bench(args):
                             log=lambda m: print("tune " + m, file=sys.stderr))
         autotune.apply(res)
         tuned = {"grid": res["grid"], "codes": res["codes"], "seconds": res["seconds"], "device": res["device"]}
+    if args.request is not None:
+        return one_request_bench(args)
     leg = served_legs(args.requests, args.request_steps, args.slots, args.precision, args.graphs,
                       log=lambda m: print(m, file=sys.stderr), max_n=args.max_n or None)
     leg["potrf_table"] = tuned
@@ -434,6 +438,29 @@ def serve_bench(args):
         "roofline": None,
     }
     print(json.dumps(out))
+
+
+def one_request_bench(args):
+    """--workload c5 --request RID: one request of the stream alone, in the
+    library's queued runs (Box.run), its step time after two untimed steps."""
+    from pybrown_amd import serving
+    from pybrown_amd.bead import Bead
+    from pybrown_amd.box import Box
+    p = serving.stream_params(max(args.requests, args.request + 1))[args.request]
+    req = serving.make_request(p, steps=args.request_steps + 2)
+    beads = [Bead(req.xyz[i].copy(), float(req.radii[i]), label="XXX") for i in range(req.N)]
+    box = Box(beads, req.inp, graphs=args.graphs, precision=serving.request_precision(req, args.precision))
+    try:
+        box.run(req.dt, 2)
+        t0 = time.perf_counter()
+        box.run(req.dt, args.request_steps)
+        dt = time.perf_counter() - t0
+    finally:
+        box.close()
+    ms = dt * 1e3 / args.request_steps
+    print(json.dumps({"metric": "ms/step of one stream request", "value": ms, "unit": "ms/step",
+                      "higher_is_better": False, "request": p, "n": 3 * req.N, "steps": args.request_steps,
+                      "dtype": DTYPE_TEXT[args.precision], "data": "synthetic"}))
 
 
 METRIC = {"c2": "steps/s at N=1000 RPY-Ewald", "c1": "steps/s at N=100 RPY", "c3": "steps/s at N=500 RPY-Ewald+JO midpoint",
