@@ -861,7 +861,7 @@ template <int V, int OCC, bool MIXED>
 __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int T, int c0, int P, int nc,
                                                         int upd_target, double* work, double* scr, int* info,
                                                         int* la, int chain_jobs, int Pt, int near, int Pf, int fq0,
-                                                        int fK) {
+                                                        int fK, int Pc) {
     __shared__ double sA[T64 * SP];
     __shared__ double sB[T64 * SP];
     __shared__ double sd[T64];
@@ -1161,17 +1161,25 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
     // trailing tile (i, j), j >= c0 + nc, less its lagging panels; with
     // two columns, tile (c0+2, c0+2) (first in this order) belongs to row
     // c0+2.  Without `near`: every trailing tile, Pt panels.  With `near`:
-    // the next launch's two tile columns (Pt panels), then this launch's
-    // slice of the far tile columns (Pf panels; tile-column pairs fq0,
-    // fq0 + fK, ..., every trailing tile of the pair's columns).
+    // the next launch's two tile columns (Pt panels); the pair after them,
+    // brought up to date (Pc panels), so the next launch's near columns lag
+    // two panels only; then this launch's slice of the far tile columns (Pf
+    // panels; tile-column pairs fq0, fq0 + fK, ..., every trailing tile of
+    // the pair's columns).
     int ii, jj, Pu = Pt;
     const int r = b - 1 - nh - nw, m = T - c0 - nc;
     if (near) {
+        const int qc = c0 / 2 + 2, ncu = Pc > 0 ? far_count(T, qc, T) : 0;
         if (r < 2 * m - 2) {
             ii = 1 + (r < m - 1 ? r : r - (m - 1));
             jj = r < m - 1 ? 0 : 1;
+        } else if (r < 2 * m - 2 + ncu) {
+            far_decode(r - (2 * m - 2), T, qc, T, 8, ii, jj);
+            ii -= c0 + nc;
+            jj -= c0 + nc;
+            Pu = Pc;
         } else {
-            far_decode(r - (2 * m - 2), T, fq0, fK, 8, ii, jj);
+            far_decode(r - (2 * m - 2) - ncu, T, fq0, fK, 8, ii, jj);
             ii -= c0 + nc;
             jj -= c0 + nc;
             Pu = Pf;
@@ -1938,9 +1946,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // in launch q-1, the one before it becomes a row pair (its `near`
     // columns).  Each launch then carries 1/K of the far work beside its
     // diagonal chain (n = 7232: 4.71 -> 4.54 ms against all of it in every
-    // K-th launch).  The pairs of a launch's slice were last updated
-    // together (launch L-K, or never), so one lag Pf serves them all, as one
-    // lag Pt serves the near pair.  Orders up to the own-CU limit are bound
+    // K-th launch).  Launch L also brings pair L+2 up to date (lag Pc), so
+    // the next launch's near columns and its diagonal row lag two panels
+    // only (n = 7232: 4.54 -> 4.40 ms).  The pairs of a launch's slice
+    // (q >= L+3) were last updated together (launch L-K, or never), so one
+    // lag Pf serves them all.  Orders up to the own-CU limit are bound
     // by the diagonal chain and take K = 1, every trailing tile in every
     // launch (K = 2 cost 3 % at n = 3000).  The row and helper tiles of a
     // launch always lag two panels.
@@ -1953,22 +1963,27 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         const int nw = T - c0 - nc;
         int Pt = c0 - c_far;
         int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
-        int Pf = 0, fq0 = 0, nfar = 0;
+        int Pf = 0, fq0 = 0, nfar = 0, Pc = 0;
         if (own_cu) {
             if (Pt > 0 && !near) c_far = c0;
         } else if (nc < 2 || nw == 0) {
             Pt = near = 0;  // no tile beyond the launch's own columns
         } else {
-            const int L = c0 / 2, qn = L + 1;
+            const int L = c0 / 2, qn = L + 1, qc = L + 2;
             Pt = c0 - upd[qn];
             upd[qn] = c0;
             near = Pt > 0 ? 1 : 0;
-            fq0 = qn + 1;
+            if (2 * qc < T) {  // catch-up of the next launch's near pair
+                Pc = c0 - upd[qc];
+                upd[qc] = c0;
+                nfar += Pc > 0 ? far_count(T, qc, T) : 0;
+            }
+            fq0 = qc + 1;
             while (fq0 % K != L % K) ++fq0;
             if (2 * fq0 < T) {
                 Pf = c0 - upd[fq0];
                 for (int q = fq0; 2 * q < T; q += K) upd[q] = c0;
-                nfar = Pf > 0 ? far_count(T, fq0, K) : 0;
+                nfar += Pf > 0 ? far_count(T, fq0, K) : 0;
             }
         }
         int ntr = 0;
@@ -1989,11 +2004,11 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         if (own_cu)
             PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
                                c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near, 0, 0, 1);
+                               g_chain_jobs, Pt, near, 0, 0, 1, 0);
         else
             PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
                                P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near, Pf, fq0, K);
+                               g_chain_jobs, Pt, near, Pf, fq0, K, Pc);
         P = nc;
         c0 += nc;
     }
