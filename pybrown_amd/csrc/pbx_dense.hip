@@ -1955,8 +1955,13 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // launch (K = 2 cost 3 % at n = 3000).  The row and helper tiles of a
     // launch always lag two panels.
     const int K = own_cu ? 1 : g_far_period.load();
+    // rolling far updates up to 128 tile columns; above, every K-th launch
+    // updates the whole trailing matrix (n = 12000: 15.0 ms either way, but
+    // 32 GB of HBM traffic against 49 GB rolling, whose slices re-read each
+    // row panel from HBM once per slice)
+    const bool roll = !own_cu && T <= 128;
     int c_far = 0;
-    std::vector<int> upd(own_cu ? 0 : (T + 1) / 2 + 2, 0);  // per pair: panels applied, 0 .. upd-1
+    std::vector<int> upd(roll ? (T + 1) / 2 + 2 : 0, 0);  // per pair: panels applied, 0 .. upd-1
     for (int c0 = 0, P = 0; c0 < T;) {
         const int nc = T - c0 < 2 ? T - c0 : 2;
         const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
@@ -1964,7 +1969,7 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
         int Pt = c0 - c_far;
         int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
         int Pf = 0, fq0 = 0, nfar = 0, Pc = 0;
-        if (own_cu) {
+        if (!roll) {
             if (Pt > 0 && !near) c_far = c0;
         } else if (nc < 2 || nw == 0) {
             Pt = near = 0;  // no tile beyond the launch's own columns
@@ -1987,7 +1992,7 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
             }
         }
         int ntr = 0;
-        if (!own_cu)
+        if (roll)
             ntr = near ? 2 * nw - 2 + nfar : 0;
         else if (Pt > 0 && nw > 0)
             ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
