@@ -132,8 +132,10 @@ int pbx_set_potrf_chain_jobs(int on);
  * in the reference. */
 int pbx_set_potrf_own_cu_tiles(int tiles);
 int pbx_get_potrf_own_cu_tiles(void);
-/* Larger orders update the whole trailing matrix every k launches (1..8,
- * default 4), the next launch's two tile columns in the others. */
+/* Larger orders update each far trailing tile once every k launches (1..8,
+ * default 4): up to 128 tile columns a rolling 1/k of the far tile-column
+ * pairs in each launch, beyond that the whole trailing matrix in every k-th
+ * launch; the next launch's two tile columns in every launch. */
 int pbx_set_potrf_far_period(int k);
 /* Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
  * codes[T] for T tile columns (T < len <= 1024): 0 = the settings above,
@@ -158,6 +160,13 @@ int pbx_device_name(char* out, int len);
  * default); every kernel still runs to completion.  No reference
  * counterpart. */
 int pbx_debug_inject_wait_failure(int k);
+/* Every tile update of a T-tile two-column factorisation (the launch plan
+ * and the kernel's tile decoding, run on the host): records (launch, i, j,
+ * m0, P), five ints each, of tile (i, j) less panels m0 .. m0+P-1, into rec
+ * (up to cap records); returns the count.  own_cu: the one-workgroup-per-CU
+ * build; K: the far period.  No reference counterpart: it lets a CPU test
+ * check that every tile receives each panel once, in order, in time. */
+int pbx_debug_two_col_updates(int T, int own_cu, int K, int* rec, int cap);
 /* Kernels this library has launched in this process so far (every device,
  * every simulation).  Launches per step bound the step rate of small
  * simulations; no reference counterpart. */

@@ -89,6 +89,7 @@ _sig("pbx_set_potrf_chain_jobs", _i, [_i])
 _sig("pbx_set_potrf_own_cu_tiles", _i, [_i])
 _sig("pbx_get_potrf_own_cu_tiles", _i, [])
 _sig("pbx_set_potrf_far_period", _i, [_i])
+_sig("pbx_debug_two_col_updates", _i, [_i, _i, _i, _ip, _i])
 _sig("pbx_set_potrf_table", _i, [ctypes.POINTER(ctypes.c_int), _i])
 _sig("pbx_get_potrf_shape", _i, [_i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)])
 _sig("pbx_time_potrf", _i, [_i, _i, _i, ctypes.POINTER(ctypes.c_double)])
@@ -141,7 +142,7 @@ EXPORTED = [
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs", "pbx_set_potrf_own_cu_tiles", "pbx_get_potrf_own_cu_tiles", "pbx_set_potrf_far_period",
     "pbx_set_potrf_table", "pbx_get_potrf_shape", "pbx_time_potrf", "pbx_device_name",
-    "pbx_debug_inject_wait_failure",
+    "pbx_debug_inject_wait_failure", "pbx_debug_two_col_updates",
     "pbx_kernel_launches",
     "pbx_sim_create", "pbx_sim_destroy", "pbx_sim_set_noise_source", "pbx_sim_set_seed",
     "pbx_sim_set_attempt_batch", "pbx_sim_set_graphs", "pbx_sim_set_lookahead", "pbx_rng_normals", "pbx_sim_snapshot_size", "pbx_sim_snapshot",

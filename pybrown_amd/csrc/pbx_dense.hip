@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(double* A, int ld, int k, con
     acc_for_each(acc, [&](int r, int c, double v) { Aik[(size_t)r * ld + c] = v; });
 }
 
-__device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
+__host__ __device__ inline void tri_decode(int t, int& ii, int& jj) {
     int r = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while ((r + 1) * (r + 2) / 2 <= t) ++r;
     while (r * (r + 1) / 2 > t) --r;
@@ -153,21 +153,21 @@ __device__ __forceinline__ void tri_decode(int t, int& ii, int& jj) {
 // tile sequence, and the sequence runs through 16-tile column bands (within
 // a band, rows top-down): an XCD then keeps one band's panel rows in its L2
 // while the row panels stream past once.
-__device__ __forceinline__ void trailing_decode(int r, int m, bool skip0, int ns, int& ii, int& jj) {
+__host__ __device__ inline void trailing_decode(int r, int m, bool skip0, int ns, int& ii, int& jj) {
     const int off = skip0 ? 1 : 0;
     const int Nt = m * (m + 1) / 2 - off;
     const int s = r % ns, k = r / ns;
     const int base = Nt / ns, extra = Nt % ns;
-    const int t = s * base + min(s, extra) + k + off;
+    const int t = s * base + (s < extra ? s : extra) + k + off;
     int c = 0, start = 0;
     for (;;) {  // ends: t < m (m + 1) / 2, the sum of the band counts
-        const int w = min(16, m - c);
-        const int cnt = w * (w + 1) / 2 + max(0, m - c - 16) * 16;
+        const int w = m - c < 16 ? m - c : 16;
+        const int cnt = w * (w + 1) / 2 + (m - c - 16 > 0 ? m - c - 16 : 0) * 16;
         if (t < start + cnt) break;
         start += cnt;
         c += 16;
     }
-    const int w = min(16, m - c), loc = t - start, tri = w * (w + 1) / 2;
+    const int w = m - c < 16 ? m - c : 16, loc = t - start, tri = w * (w + 1) / 2;
     if (loc < tri) {
         tri_decode(loc, ii, jj);
         ii += c;
@@ -191,11 +191,11 @@ __host__ __device__ inline int far_count(int T, int q0, int K) {
 // As in trailing_decode, residue class r % ns takes one contiguous chunk of
 // the sequence (columns in order, rows top-down), so an XCD keeps one
 // column's panel rows in its L2 while the row panels stream past.
-__device__ __forceinline__ void far_decode(int r, int T, int q0, int K, int ns, int& i, int& j) {
+__host__ __device__ inline void far_decode(int r, int T, int q0, int K, int ns, int& i, int& j) {
     const int Nt = far_count(T, q0, K);
     const int s = r % ns, k = r / ns;
     const int base = Nt / ns, extra = Nt % ns;
-    int t = s * base + min(s, extra) + k;
+    int t = s * base + (s < extra ? s : extra) + k;
     for (int q = q0;; q += K)  // ends: t < Nt
         for (int c = 2 * q; c < 2 * q + 2 && c < T; ++c) {
             if (t < T - c) {
@@ -205,6 +205,41 @@ __device__ __forceinline__ void far_decode(int r, int T, int q0, int K, int ns, 
             }
             t -= T - c;
         }
+}
+
+// Tile (i, j) and lagging panels m0 .. m0+P-1 of trailing workgroup r of
+// a two-column launch (potrf2_kernel; the launch's parameters as
+// two_col_plan gives them), t0 = c0 + nc.  Without `near`: every trailing
+// tile but (t0, t0) when nc = 2, Pt panels.  With `near`: the next launch's
+// two tile columns (Pt panels; (t0, t0) belongs to row t0), the pair after
+// them (Pc panels), then the far slice (Pf panels; pairs fq0, fq0 + fK, ...).
+__host__ __device__ inline void two_col_trailing(int r, int T, int c0, int nc, int near, int Pt, int Pf, int fq0,
+                                                 int fK, int Pc, int ns, int& i, int& j, int& m0, int& P) {
+    const int m = T - c0 - nc;
+    int ii, jj;
+    P = Pt;
+    if (near) {
+        const int qc = c0 / 2 + 2, ncu = Pc > 0 ? far_count(T, qc, T) : 0;
+        if (r < 2 * m - 2) {
+            ii = 1 + (r < m - 1 ? r : r - (m - 1));
+            jj = r < m - 1 ? 0 : 1;
+        } else if (r < 2 * m - 2 + ncu) {
+            far_decode(r - (2 * m - 2), T, qc, T, ns, ii, jj);
+            ii -= c0 + nc;
+            jj -= c0 + nc;
+            P = Pc;
+        } else {
+            far_decode(r - (2 * m - 2) - ncu, T, fq0, fK, ns, ii, jj);
+            ii -= c0 + nc;
+            jj -= c0 + nc;
+            P = Pf;
+        }
+    } else {
+        trailing_decode(r, m, nc == 2, ns, ii, jj);
+    }
+    i = c0 + nc + ii;
+    j = c0 + nc + jj;
+    m0 = c0 - P;
 }
 
 // A_ij -= L_ik L_jk^T over the trailing lower tiles; the workgroup owning
@@ -1158,38 +1193,11 @@ __global__ __launch_bounds__(256, OCC) void potrf2_kernel(double* A, int ld, int
         }
         return;
     }
-    // trailing tile (i, j), j >= c0 + nc, less its lagging panels; with
-    // two columns, tile (c0+2, c0+2) (first in this order) belongs to row
-    // c0+2.  Without `near`: every trailing tile, Pt panels.  With `near`:
-    // the next launch's two tile columns (Pt panels); the pair after them,
-    // brought up to date (Pc panels), so the next launch's near columns lag
-    // two panels only; then this launch's slice of the far tile columns (Pf
-    // panels; tile-column pairs fq0, fq0 + fK, ..., every trailing tile of
-    // the pair's columns).
-    int ii, jj, Pu = Pt;
-    const int r = b - 1 - nh - nw, m = T - c0 - nc;
-    if (near) {
-        const int qc = c0 / 2 + 2, ncu = Pc > 0 ? far_count(T, qc, T) : 0;
-        if (r < 2 * m - 2) {
-            ii = 1 + (r < m - 1 ? r : r - (m - 1));
-            jj = r < m - 1 ? 0 : 1;
-        } else if (r < 2 * m - 2 + ncu) {
-            far_decode(r - (2 * m - 2), T, qc, T, 8, ii, jj);
-            ii -= c0 + nc;
-            jj -= c0 + nc;
-            Pu = Pc;
-        } else {
-            far_decode(r - (2 * m - 2) - ncu, T, fq0, fK, 8, ii, jj);
-            ii -= c0 + nc;
-            jj -= c0 + nc;
-            Pu = Pf;
-        }
-    } else {
-        trailing_decode(r, m, nc == 2, OCC == 1 ? 7 : 8, ii, jj);
-    }
-    const int i = c0 + nc + ii, j = c0 + nc + jj;
+    // trailing tile (i, j), j >= c0 + nc, less its lagging panels
+    int i, j, m0u, Pu;
+    two_col_trailing(b - 1 - nh - nw, T, c0, nc, near, Pt, Pf, fq0, fK, Pc, OCC == 1 ? 7 : 8, i, j, m0u, Pu);
     Updated u;
-    update_tile<MIXED, false, OCC == 2>(A, ld, i, j, c0 - Pu, Pu, sA, sB, u);
+    update_tile<MIXED, false, OCC == 2>(A, ld, i, j, m0u, Pu, sA, sB, u);
     double* X = tile(i, j);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -1918,6 +1926,106 @@ int pbx_copy_lower(const double* src, double* dst, int np, int ld, hipStream_t s
     return PBX_OK;
 }
 
+// One two-column launch of a factorisation: its columns (c0, nc), the
+// previous launch's column count P, its workgroup counts (nh helper jobs,
+// nw rows, ntr trailing tiles) and the lags of its trailing tiles
+// (two_col_trailing).
+struct TwoColLaunch {
+    int c0, P, nc, nh, nw, Pt, near, Pf, fq0, K, Pc, ntr;
+};
+
+// The launch sequence of a T-tile factorisation (K: the far period; one
+// workgroup per CU: K = 1 and no rolling).
+static void two_col_plan(int T, bool own_cu, int K, std::vector<TwoColLaunch>& plan) {
+    // rolling far updates up to 128 tile columns; above, every K-th launch
+    // updates the whole trailing matrix (n = 12000: 15.0 ms either way, but
+    // 32 GB of HBM traffic against 49 GB rolling, whose slices re-read each
+    // row panel from HBM once per slice)
+    const bool roll = !own_cu && T <= 128;
+    int c_far = 0;
+    std::vector<int> upd(roll ? (T + 1) / 2 + 2 : 0, 0);  // per pair: panels applied, 0 .. upd-1
+    plan.clear();
+    for (int c0 = 0, P = 0; c0 < T;) {
+        const int nc = T - c0 < 2 ? T - c0 : 2;
+        const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
+        const int nw = T - c0 - nc;
+        int Pt = c0 - c_far;
+        int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
+        int Pf = 0, fq0 = 0, nfar = 0, Pc = 0;
+        if (!roll) {
+            if (Pt > 0 && !near) c_far = c0;
+        } else if (nc < 2 || nw == 0) {
+            Pt = near = 0;  // no tile beyond the launch's own columns
+        } else {
+            const int L = c0 / 2, qn = L + 1, qc = L + 2;
+            Pt = c0 - upd[qn];
+            upd[qn] = c0;
+            near = Pt > 0 ? 1 : 0;
+            if (2 * qc < T) {  // catch-up of the next launch's near pair
+                Pc = c0 - upd[qc];
+                upd[qc] = c0;
+                nfar += Pc > 0 ? far_count(T, qc, T) : 0;
+            }
+            fq0 = qc + 1;
+            while (fq0 % K != L % K) ++fq0;
+            if (2 * fq0 < T) {
+                Pf = c0 - upd[fq0];
+                for (int q = fq0; 2 * q < T; q += K) upd[q] = c0;
+                nfar += Pf > 0 ? far_count(T, fq0, K) : 0;
+            }
+        }
+        int ntr = 0;
+        if (roll)
+            ntr = near ? 2 * nw - 2 + nfar : 0;
+        else if (Pt > 0 && nw > 0)
+            ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
+        plan.push_back({c0, P, nc, nh, nw, Pt, near, Pf, fq0, roll ? K : 1, Pc, ntr});
+        P = nc;
+        c0 += nc;
+    }
+}
+
+// Every tile update a T-tile factorisation's launches make, as the kernel's
+// roles decode them: records (launch, i, j, m0, P) of tile (i, j) less
+// panels m0 .. m0+P-1, five ints each, into rec (up to cap records).
+// Returns the number of records (possibly more than cap).  The rows also
+// subtract their launch's own two columns from tile (c0+2, c0+2) (recorded
+// as panels c0, c0+1); the diagonal chain's own tiles are not recorded.
+extern "C" int pbx_debug_two_col_updates(int T, int own_cu, int K, int* rec, int cap) {
+    if (T <= 0 || K <= 0) return PBX_ERR_ARG;
+    std::vector<TwoColLaunch> plan;
+    two_col_plan(T, own_cu != 0, own_cu ? 1 : K, plan);
+    int n = 0;
+    auto put = [&](int L, int i, int j, int m0, int P) {
+        if (P > 0 && n < cap && rec) {
+            int* r = rec + 5 * n;
+            r[0] = L, r[1] = i, r[2] = j, r[3] = m0, r[4] = P;
+        }
+        if (P > 0) ++n;
+    };
+    for (int L = 0; L < (int)plan.size(); ++L) {
+        const TwoColLaunch& l = plan[L];
+        if (l.nc == 2) {  // helper jobs
+            put(L, l.c0 + 1, l.c0, l.c0 - l.P, l.P);
+            put(L, l.c0 + 1, l.c0 + 1, l.c0 - l.P, l.P);
+        }
+        for (int i = l.c0 + l.nc; i < T; ++i) {  // rows
+            put(L, i, l.c0, l.c0 - l.P, l.P);
+            if (l.nc == 2) put(L, i, l.c0 + 1, l.c0 - l.P, l.P);
+            if (l.nc == 2 && i == l.c0 + 2) {
+                put(L, i, i, l.c0 - l.Pt, l.Pt);
+                put(L, i, i, l.c0, 2);
+            }
+        }
+        for (int r = 0; r < l.ntr; ++r) {
+            int i, j, m0, P;
+            two_col_trailing(r, T, l.c0, l.nc, l.near, l.Pt, l.Pf, l.fq0, l.K, l.Pc, own_cu ? 7 : 8, i, j, m0, P);
+            put(L, i, j, m0, P);
+        }
+    }
+    return n;
+}
+
 // Variant 6 launch sequence: launch (c0, P, nc) for c0 = 0, 2, 4, ...
 // la_ready: the look-ahead words are already zero (reset by the kernel
 // before, on the same stream).
@@ -1954,68 +2062,25 @@ static void potrf_two_col_launch(double* A, int np, int ld, double* work, int* i
     // by the diagonal chain and take K = 1, every trailing tile in every
     // launch (K = 2 cost 3 % at n = 3000).  The row and helper tiles of a
     // launch always lag two panels.
-    const int K = own_cu ? 1 : g_far_period.load();
-    // rolling far updates up to 128 tile columns; above, every K-th launch
-    // updates the whole trailing matrix (n = 12000: 15.0 ms either way, but
-    // 32 GB of HBM traffic against 49 GB rolling, whose slices re-read each
-    // row panel from HBM once per slice)
-    const bool roll = !own_cu && T <= 128;
-    int c_far = 0;
-    std::vector<int> upd(roll ? (T + 1) / 2 + 2 : 0, 0);  // per pair: panels applied, 0 .. upd-1
-    for (int c0 = 0, P = 0; c0 < T;) {
-        const int nc = T - c0 < 2 ? T - c0 : 2;
-        const int nh = nc == 2 ? (P > 0 ? 2 : 1) : 0;
-        const int nw = T - c0 - nc;
-        int Pt = c0 - c_far;
-        int near = (Pt > 0 && Pt < 2 * K) ? 1 : 0;
-        int Pf = 0, fq0 = 0, nfar = 0, Pc = 0;
-        if (!roll) {
-            if (Pt > 0 && !near) c_far = c0;
-        } else if (nc < 2 || nw == 0) {
-            Pt = near = 0;  // no tile beyond the launch's own columns
-        } else {
-            const int L = c0 / 2, qn = L + 1, qc = L + 2;
-            Pt = c0 - upd[qn];
-            upd[qn] = c0;
-            near = Pt > 0 ? 1 : 0;
-            if (2 * qc < T) {  // catch-up of the next launch's near pair
-                Pc = c0 - upd[qc];
-                upd[qc] = c0;
-                nfar += Pc > 0 ? far_count(T, qc, T) : 0;
-            }
-            fq0 = qc + 1;
-            while (fq0 % K != L % K) ++fq0;
-            if (2 * fq0 < T) {
-                Pf = c0 - upd[fq0];
-                for (int q = fq0; 2 * q < T; q += K) upd[q] = c0;
-                nfar += Pf > 0 ? far_count(T, fq0, K) : 0;
-            }
-        }
-        int ntr = 0;
-        if (roll)
-            ntr = near ? 2 * nw - 2 + nfar : 0;
-        else if (Pt > 0 && nw > 0)
-            ntr = near ? 2 * nw - 2 : nw * (nw + 1) / 2 - (nc == 2 ? 1 : 0);
-        upd_target += nh;
+    std::vector<TwoColLaunch> plan;
+    two_col_plan(T, own_cu, own_cu ? 1 : g_far_period.load(), plan);
+    for (const TwoColLaunch& l : plan) {
+        upd_target += l.nh;
         // own-CU build: roles 1 .. W sit at the indices that are not
         // multiples of 8 (the smallest grid holding W of them; role
         // b - floor(b / 8) of index b runs over 1 .. W exactly once)
-        int grid = 1 + nh + nw + ntr;
         if (own_cu) {
-            const int W = nh + nw + ntr;
-            grid = 1;
+            const int W = l.nh + l.nw + l.ntr;
+            int grid = 1;
             while ((grid - 1) - (grid - 1) / 8 < W) ++grid;
+            PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T, l.c0, l.P, l.nc,
+                       upd_target, work, work + dense_scratch_offset(np), info_dev, la, g_chain_jobs, l.Pt, l.near,
+                       l.Pf, l.fq0, l.K, l.Pc);
+        } else {
+            PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + l.nh + l.nw + l.ntr), dim3(256), 0, s, A, ld, T, l.c0,
+                       l.P, l.nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la, g_chain_jobs,
+                       l.Pt, l.near, l.Pf, l.fq0, l.K, l.Pc);
         }
-        if (own_cu)
-            PBX_LAUNCH((potrf2_kernel<5, 1, MIXED>), dim3(grid), dim3(256), dyn_lds, s, A, ld, T,
-                               c0, P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near, 0, 0, 1, 0);
-        else
-            PBX_LAUNCH((potrf2_kernel<5, 2, MIXED>), dim3(1 + nh + nw + ntr), dim3(256), 0, s, A, ld, T, c0,
-                               P, nc, upd_target, work, work + dense_scratch_offset(np), info_dev, la,
-                               g_chain_jobs, Pt, near, Pf, fq0, K, Pc);
-        P = nc;
-        c0 += nc;
     }
 }
 
