@@ -137,6 +137,11 @@ int pbx_get_potrf_own_cu_tiles(void);
  * pairs in each launch, beyond that the whole trailing matrix in every k-th
  * launch; the next launch's two tile columns in every launch. */
 int pbx_set_potrf_far_period(int k);
+/* Orders of one workgroup per compute unit: 0 updates every trailing tile in
+ * every launch; k = 1..8 rolls 1/k of the far tile-column pairs through each
+ * launch (default 4; C2, n = 3000: 1.8 % faster).  The factors agree to
+ * rounding.  Process-wide.  No counterpart in the reference. */
+int pbx_set_potrf_own_cu_roll(int k);
 /* Per-order launch shapes, as chosen by a tuner (pybrown_amd.autotune):
  * codes[T] for T tile columns (T < len <= 1024): 0 = the settings above,
  * 4 = variant 4, 61 = variant 6 one workgroup per CU, 62 = variant 6 two

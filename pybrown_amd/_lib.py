@@ -90,6 +90,7 @@ _sig("pbx_set_potrf_own_cu_tiles", _i, [_i])
 _sig("pbx_get_potrf_own_cu_tiles", _i, [])
 _sig("pbx_set_potrf_far_period", _i, [_i])
 _sig("pbx_debug_two_col_updates", _i, [_i, _i, _i, _ip, _i])
+_sig("pbx_set_potrf_own_cu_roll", _i, [_i])
 _sig("pbx_set_potrf_table", _i, [ctypes.POINTER(ctypes.c_int), _i])
 _sig("pbx_get_potrf_shape", _i, [_i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)])
 _sig("pbx_time_potrf", _i, [_i, _i, _i, ctypes.POINTER(ctypes.c_double)])
@@ -141,6 +142,7 @@ EXPORTED = [
     "pbx_set_potrf_variant",
     "pbx_get_potrf_variant",
     "pbx_set_potrf_chain_jobs", "pbx_set_potrf_own_cu_tiles", "pbx_get_potrf_own_cu_tiles", "pbx_set_potrf_far_period",
+    "pbx_set_potrf_own_cu_roll",
     "pbx_set_potrf_table", "pbx_get_potrf_shape", "pbx_time_potrf", "pbx_device_name",
     "pbx_debug_inject_wait_failure", "pbx_debug_two_col_updates",
     "pbx_kernel_launches",

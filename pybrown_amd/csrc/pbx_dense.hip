@@ -1936,7 +1936,24 @@ struct TwoColLaunch {
 
 // The launch sequence of a T-tile factorisation (K: the far period; one
 // workgroup per CU: K = 1 and no rolling).
+// Far period of the one-workgroup-per-CU orders' rolling schedule (0: every
+// trailing tile in every launch).  C2 (n = 3000): 0.974 ms with every tile
+// in every launch; rolling 0.967 (period 2), 0.971 (3), 0.957 (4),
+// 0.976 (5), 0.981 (6), 1.056 ms (8).
+static std::atomic<int> g_own_cu_roll{4};
+extern "C" int pbx_set_potrf_own_cu_roll(int k) {
+    if (k < 0 || k > 8) return PBX_ERR_ARG;
+    g_own_cu_roll = k;
+    ++g_shape_generation;
+    return PBX_OK;
+}
+
 static void two_col_plan(int T, bool own_cu, int K, std::vector<TwoColLaunch>& plan) {
+    if (own_cu && g_own_cu_roll.load() > 0) {
+        // the rolling schedule (below) in the one-per-CU build
+        K = g_own_cu_roll.load();
+        own_cu = false;
+    }
     // rolling far updates up to 128 tile columns; above, every K-th launch
     // updates the whole trailing matrix (n = 12000: 15.0 ms either way, but
     // 32 GB of HBM traffic against 49 GB rolling, whose slices re-read each

@@ -57,9 +57,22 @@ def _check(T, own_cu, K):
             assert pos == c0s[col_launch[j]], ("incomplete", T, own_cu, K, i, j, segs)
 
 
-@pytest.mark.parametrize("own_cu", [1, 0])
-def test_every_tile_receives_each_panel_once_in_time(own_cu):
-    Ks = [1] if own_cu else [1, 2, 3, 4, 6]
-    for K in Ks:
-        for T in list(range(1, 71)) + [113, 127, 128, 129, 188, 191]:
-            _check(T, own_cu, K)
+@pytest.mark.parametrize("own_cu_roll", [4, 0, 2])
+def test_one_per_cu_orders(own_cu_roll):
+    """The one-workgroup-per-CU schedule: rolling far updates (the default,
+    period 4; period 2) and every trailing tile in every launch (0)."""
+    from pybrown_amd._lib import lib
+    assert lib.pbx_set_potrf_own_cu_roll(own_cu_roll) == 0
+    try:
+        for T in list(range(1, 71)) + [113, 128, 129]:
+            _check(T, 1, 1)
+    finally:
+        lib.pbx_set_potrf_own_cu_roll(4)
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 6])
+def test_two_per_cu_orders(K):
+    """Two workgroups per CU: far periods 1-6, rolling up to 128 tile
+    columns and whole-trailing launches beyond."""
+    for T in list(range(1, 71)) + [113, 127, 128, 129, 188, 191]:
+        _check(T, 0, K)
