@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <condition_variable>
+#include <functional>
 #include <atomic>
 #include <mutex>
 #include <thread>
@@ -38,6 +39,7 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
 int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
                          const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s);
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s);
+int pbx_launch_wrap_if(double* pos, int N, double L, const int* sel, int K, hipStream_t s);
 int pbx_launch_div_shift(double* pos, int idx, double delta, hipStream_t s);
 int pbx_launch_div_init(const double* D, int n, int ld, double* div, hipStream_t s);
 int pbx_launch_div_add_row(const double* D, int row, int n, int ld, double* div, hipStream_t s);
@@ -982,6 +984,13 @@ struct StepCtx {
     long long* flux;  // per-label counters the step's moves add their crossings to
     int queued;       // steps queued back to back: no per-step status copies
     const PbxStepCursor* cur = nullptr;  // captured step: noise and flux slots from here
+    // The step's tail (wrap, read-backs, look-ahead assembly) and the wait
+    // for its read-backs.  A batch of move attempts queues the tail behind
+    // its selection, the wrap guarded by the batch's outcome, before the host
+    // waits for that outcome; tail_done says the accepted batch's tail ran.
+    const std::function<int(const int*, int)>* tail = nullptr;
+    const std::function<int()>* wait = nullptr;
+    int tail_done = 0;
 };
 
 int stoch(PbxSim* s, const double* Bm, const double* xi, double mult, StepCtx& c) {
@@ -1068,12 +1077,18 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
         TRY(pbx_launch_attempt_select(s->cand, s->N, (int)K, s->cflag, s->pos, c.pn, c.pp, c.flux_on, s->label,
                                       c.flux, s->flag, s->stream));
         PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        PBX_HIP(hipStreamSynchronize(s->stream));
+        if (c.tail) {
+            TRY((*c.tail)(s->flag, (int)K));
+            TRY((*c.wait)());
+        } else {
+            PBX_HIP(hipStreamSynchronize(s->stream));
+        }
         TRY(take_info(s));
         const int a = s->flag_host[0];
         if (a < K) {
             attempts += a + 1;
             noise_consume(s, a + 1);
+            c.tail_done = c.tail != nullptr;
             break;
         }
         attempts += K;
@@ -1721,30 +1736,51 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     }
     // box.py:190: forces from the positions at the start of the step
     if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
-    if (cf.scheme == PBX_ERMAK)
-        TRY(ermak(s, dt, build_Dff_flag, build_Dnf_flag, c));
-    else
-        TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
-    TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
     const bool timed = s->timing && cf.hydrodynamics != PBX_NOHI;
-    if (timed) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
-    PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
-    if (s->forces_on) PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    if (cf.measure_flux) {
-        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
-        PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
-    }
     const bool ahead = s->Dspec && s->lookahead && !s->graphs && build_Dff_flag && s->next_builds != 0 &&
                        cf.hydrodynamics != PBX_NOHI;
     const int prev = s->ahead_pending, q = prev == 0 ? 1 : 0;
-    if (ahead) {
-        PBX_HIP(hipEventRecord(s->done_ev, s->stream));
-        if (s->timing) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
-        TRY(assemble_M(s, s->Dspec, s->kBT * 1e19 / cf.viscosity));  // as build_Dff
-        if (s->timing) PBX_HIP(hipEventRecord(s->ev[5 + 2 * q], s->stream));
-        PBX_HIP(hipEventSynchronize(s->done_ev));
+    // wrap (bead.py:170-183), the positions, energy and flux read-backs, and
+    // the next step's far-field assembly from the wrapped positions
+    const std::function<int(const int*, int)> tail = [&](const int* sel, int K) -> int {
+        if (sel)
+            TRY(pbx_launch_wrap_if(s->pos, s->N, cf.box_length, sel, K, s->stream));
+        else
+            TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
+        if (timed) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
+        PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+        if (s->forces_on)
+            PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+        if (cf.measure_flux) {
+            int nl = cf.n_labels > 0 ? cf.n_labels : 1;
+            PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
+        }
+        if (ahead) {
+            PBX_HIP(hipEventRecord(s->done_ev, s->stream));
+            if (s->timing) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
+            TRY(assemble_M(s, s->Dspec, s->kBT * 1e19 / cf.viscosity));  // as build_Dff
+            if (s->timing) PBX_HIP(hipEventRecord(s->ev[5 + 2 * q], s->stream));
+        }
+        return PBX_OK;
+    };
+    // the read-backs are complete; a look-ahead assembly may still run
+    const std::function<int()> wait = [&]() -> int {
+        if (ahead)
+            PBX_HIP(hipEventSynchronize(s->done_ev));
+        else
+            PBX_HIP(hipStreamSynchronize(s->stream));
+        return PBX_OK;
+    };
+    if (cf.scheme == PBX_ERMAK) {
+        c.tail = &tail;
+        c.wait = &wait;
+        TRY(ermak(s, dt, build_Dff_flag, build_Dnf_flag, c));
     } else {
-        PBX_HIP(hipStreamSynchronize(s->stream));
+        TRY(midpoint(s, dt, build_Dff_flag, build_Dnf_flag, c));
+    }
+    if (!c.tail_done) {
+        TRY(tail(nullptr, 0));
+        TRY(wait());
     }
     if (timed) {  // the events are complete: read them after the one synchronisation
         float t01 = 0, t12 = 0, t23 = 0;

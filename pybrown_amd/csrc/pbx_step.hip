@@ -141,9 +141,12 @@ __global__ void translate_kernel(const double* __restrict__ vin, double sign, in
 }
 
 // bead.py:170-183 keep_in_box: wrap into [-L/2, L/2)
-__global__ void wrap_kernel(double* pos, int n3, double L) {
+// With sel non-null the wrap is part of a batch of K move attempts queued
+// before the host knows its outcome: it applies only when the batch accepted
+// a move (*sel < K); a rejected batch leaves the translated-back positions.
+__global__ void wrap_kernel(double* pos, int n3, double L, const int* __restrict__ sel, int K) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n3) return;
+    if (t >= n3 || (sel && *sel >= K)) return;
     double x = pos[t];
     if (!isfinite(x)) return;
     if (fabs(x) > 64.0 * L) x = fmod(x, L);
@@ -442,6 +445,7 @@ __global__ __launch_bounds__(256) void trsv_chain_kernel(const double* __restric
 constexpr int BT = 64;        // B tile edge
 constexpr int BSP = 65;       // LDS stride
 constexpr int KC_MAX = 32;    // candidates per batch
+constexpr int SLAB_MAX = 12;  // column slabs of the partial products (pbx_attempt_slab_width)
 
 // Partial products P[s][r][k] = sum_{c in slab s, c <= r} B[r][c] xi_k[c] for
 // row block i = blockIdx.x and column slab s = blockIdx.y (S tiles wide).
@@ -547,18 +551,30 @@ __global__ __launch_bounds__(128) void cand_kernel(const double* __restrict__ po
     if (b < N && k < K) {
         int slot = ring_pos + k;
         if (slot >= ring_att) slot -= ring_att;
-        for (int c = 0; c < 3; ++c) {
-            const int r = 3 * b + c;
-            double d;
-            if (Bd) {
-                d = ((Bd[r] * ring[(size_t)slot * n + r]) * sq2dt) * 1.0;
-            } else {
-                const int ns = (r / BT) / S + 1;
-                double acc = 0.0;
-                for (int s = 0; s < ns; ++s) acc += P[((size_t)s * np + r) * KC + k];
-                d = (acc * sq2dt) * 1.0;
+        if (Bd) {
+            for (int c = 0; c < 3; ++c) {
+                const int r = 3 * b + c;
+                sd[bl][c][k] = ((Bd[r] * ring[(size_t)slot * n + r]) * sq2dt) * 1.0;
             }
-            sd[bl][c][k] = d;
+        } else {
+            // every slab partial requested before the first is added (the
+            // sums keep slab order)
+            double v[3][SLAB_MAX];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int r = 3 * b + c, ns = (r / BT) / S + 1;
+#pragma unroll
+                for (int s = 0; s < SLAB_MAX; ++s) v[c][s] = s < ns ? P[((size_t)s * np + r) * KC + k] : 0.0;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int r = 3 * b + c, ns = (r / BT) / S + 1;
+                double acc = 0.0;
+#pragma unroll
+                for (int s = 0; s < SLAB_MAX; ++s)
+                    if (s < ns) acc += v[c][s];
+                sd[bl][c][k] = (acc * sq2dt) * 1.0;
+            }
         }
     }
     __syncthreads();
@@ -789,7 +805,15 @@ int pbx_launch_translate(const double* v, double sign, double* pos, int N, const
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s) {
     PBX_CHECK_BOX(L);
     int n3 = 3 * N;
-    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L);
+    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L, nullptr, 0);
+    PBX_HIP(hipGetLastError());
+    return PBX_OK;
+}
+
+int pbx_launch_wrap_if(double* pos, int N, double L, const int* sel, int K, hipStream_t s) {
+    PBX_CHECK_BOX(L);
+    int n3 = 3 * N;
+    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L, sel, K);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
@@ -913,6 +937,10 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
             return PBX_ERR_ARG;
         }
         const int T = np / BT;
+        if ((T + S - 1) / S > SLAB_MAX) {
+            pbx_set_error(PBX_ERR_ARG, "attempt batch: %d column slabs exceed %d", (T + S - 1) / S, SLAB_MAX);
+            return PBX_ERR_ARG;
+        }
         dim3 grid(T, (T + S - 1) / S);
         if (KC == 16)
             PBX_LAUNCH(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
