@@ -39,7 +39,8 @@ int pbx_launch_det(const double* D, int ld, int diag_only, const double* F, doub
 int pbx_launch_translate(const double* v, double sign, double* pos, int N, const double* plane_n,
                          const double* plane_p, int flux_on, const int* label, long long* flux, hipStream_t s);
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s);
-int pbx_launch_wrap_if(double* pos, int N, double L, const int* sel, int K, hipStream_t s);
+int pbx_launch_wrap_publish(double* pos, int N, double L, const int* sel, int K, const PbxWrapOut& o,
+                            hipStream_t s);
 int pbx_launch_div_shift(double* pos, int idx, double delta, hipStream_t s);
 int pbx_launch_div_init(const double* D, int n, int ld, double* div, hipStream_t s);
 int pbx_launch_div_add_row(const double* D, int row, int n, int ld, double* div, hipStream_t s);
@@ -578,6 +579,12 @@ struct PbxSim {
     double* pos_host = nullptr;      // pinned copy of positions (end of step)
     long long* flux_host = nullptr;  // pinned copy of flux counters
     int info_pending = 0;            // a status copy is queued into flag_host[1]
+    // device-side aliases of flag_host, pos_host, flux_host and E_host (the
+    // end-of-step values are written there by the wrap kernel)
+    int* flag_hd = nullptr;
+    double* pos_hd = nullptr;
+    long long* flux_hd = nullptr;
+    double* E_hd = nullptr;
     int last_attempts = 0;
     int have_D = 0, have_B = 0, have_Rff = 0;
     std::vector<void*> owned;
@@ -680,6 +687,7 @@ int take_info(PbxSim* s) {
     s->info_pending = 0;
     int info = s->flag_host[1];
     if (!info) return PBX_OK;
+    s->flag_host[1] = 0;  // reported once
     PBX_HIP(hipMemsetAsync(s->info, 0, sizeof(int), s->stream));
     if (info == PBX_INFO_DAG_WAIT) {
         pbx_set_error(PBX_ERR_STATE, "a bounded dependency wait ran out (factorisation or solve)");
@@ -1076,11 +1084,11 @@ int ermak(PbxSim* s, double dt, int bDff, int bDnf, StepCtx& c) {
                                      s->stream));
         TRY(pbx_launch_attempt_select(s->cand, s->N, (int)K, s->cflag, s->pos, c.pn, c.pp, c.flux_on, s->label,
                                       c.flux, s->flag, s->stream));
-        PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-        if (c.tail) {
+        if (c.tail) {  // the tail's wrap kernel publishes the outcome
             TRY((*c.tail)(s->flag, (int)K));
             TRY((*c.wait)());
         } else {
+            PBX_HIP(hipMemcpyAsync(s->flag_host, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
             PBX_HIP(hipStreamSynchronize(s->stream));
         }
         TRY(take_info(s));
@@ -1294,6 +1302,12 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
         hipHostMalloc((void**)&s->flux_host, sizeof(long long) * nl, hipHostMallocDefault) != hipSuccess) {
         pbx_set_error(PBX_ERR_OOM, "hipHostMalloc failed");
         return fail(PBX_ERR_OOM);
+    }
+    if (hipHostGetDevicePointer((void**)&s->flag_hd, s->flag_host, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->pos_hd, s->pos_host, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->flux_hd, s->flux_host, 0) != hipSuccess) {
+        pbx_set_error(PBX_ERR_HIP, "hipHostGetDevicePointer failed");
+        return fail(PBX_ERR_HIP);
     }
     auto hip_ok = [&](hipError_t e, const char* what) {
         if (e != hipSuccess) {
@@ -1732,7 +1746,10 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
             pbx_set_error(PBX_ERR_STATE, "propagation without a noise factor");
             return PBX_ERR_STATE;
         }
-        TRY(post_info(s));
+        if (cf.scheme == PBX_ERMAK)
+            s->info_pending = 1;  // the step's wrap kernel publishes the status word
+        else
+            TRY(post_info(s));
     }
     // box.py:190: forces from the positions at the start of the step
     if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
@@ -1742,19 +1759,22 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     const int prev = s->ahead_pending, q = prev == 0 ? 1 : 0;
     // wrap (bead.py:170-183), the positions, energy and flux read-backs, and
     // the next step's far-field assembly from the wrapped positions
+    PbxWrapOut wo{};
+    wo.pos = s->pos_hd;
+    wo.status = s->flag_hd;
+    wo.info = s->info;
+    if (cf.measure_flux) {
+        wo.flux = s->flux_hd;
+        wo.flux_dev = s->flux;
+        wo.nl = cf.n_labels > 0 ? cf.n_labels : 1;
+    }
+    if (s->forces_on && s->E_hd) {
+        wo.E = s->E_hd;
+        wo.E_dev = s->E_dev;
+    }
     const std::function<int(const int*, int)> tail = [&](const int* sel, int K) -> int {
-        if (sel)
-            TRY(pbx_launch_wrap_if(s->pos, s->N, cf.box_length, sel, K, s->stream));
-        else
-            TRY(pbx_launch_wrap(s->pos, s->N, cf.box_length, s->stream));
+        TRY(pbx_launch_wrap_publish(s->pos, s->N, cf.box_length, sel, K, wo, s->stream));
         if (timed) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
-        PBX_HIP(hipMemcpyAsync(s->pos_host, s->pos, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
-        if (s->forces_on)
-            PBX_HIP(hipMemcpyAsync(s->E_host, s->E_dev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-        if (cf.measure_flux) {
-            int nl = cf.n_labels > 0 ? cf.n_labels : 1;
-            PBX_HIP(hipMemcpyAsync(s->flux_host, s->flux, sizeof(long long) * nl, hipMemcpyDeviceToHost, s->stream));
-        }
         if (ahead) {
             PBX_HIP(hipEventRecord(s->done_ev, s->stream));
             if (s->timing) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
@@ -2183,6 +2203,7 @@ extern "C" int pbx_sim_set_forces(pbx_sim* s, const pbx_forces* f) {
     if (!s->E_host) {
         PBX_HIP(hipHostMalloc((void**)&s->E_host, sizeof(double)));
         *s->E_host = 0.0;
+        PBX_HIP(hipHostGetDevicePointer((void**)&s->E_hd, s->E_host, 0));
     }
     return PBX_OK;
 }

@@ -223,6 +223,20 @@ struct PbxStepCursor {
 };
 int pbx_launch_cursor_advance(PbxStepCursor* cur, hipStream_t s);
 
+// End-of-step values the wrap kernel writes straight into mapped pinned host
+// memory (device-side aliases of hipHostMalloc buffers), in place of one
+// copy per value.  Any pointer may be null.
+struct PbxWrapOut {
+    double* pos;                 // the wrapped positions
+    int* status;                 // [0] the batch's accepted index, [1] the factorisation status word
+    const int* info;             // device status word
+    long long* flux;             // per-label crossing counters
+    const long long* flux_dev;
+    int nl;
+    double* E;                   // force-field energy
+    const double* E_dev;
+};
+
 // batched move attempts (pbx_step.hip): candidates for K <= pbx_attempt_kmax()
 // consecutive attempts (xi of attempt k in ring slot (ring_pos + k) mod
 // ring_att), their overlap flags, then the selection of the first accepted

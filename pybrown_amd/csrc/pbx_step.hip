@@ -144,15 +144,25 @@ __global__ void translate_kernel(const double* __restrict__ vin, double sign, in
 // With sel non-null the wrap is part of a batch of K move attempts queued
 // before the host knows its outcome: it applies only when the batch accepted
 // a move (*sel < K); a rejected batch leaves the translated-back positions.
-__global__ void wrap_kernel(double* pos, int n3, double L, const int* __restrict__ sel, int K) {
+// The outcome, the status word, the flux counters, the energy and the
+// wrapped positions are written to the host-mapped buffers of `o`.
+__global__ void wrap_kernel(double* pos, int n3, double L, const int* __restrict__ sel, int K, PbxWrapOut o) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) {
+        if (o.status && sel) o.status[0] = *sel;
+        if (o.status && o.info) o.status[1] = *o.info;
+        if (o.E) *o.E = *o.E_dev;
+    }
+    if (o.flux && t < o.nl) o.flux[t] = o.flux_dev[t];
     if (t >= n3 || (sel && *sel >= K)) return;
     double x = pos[t];
-    if (!isfinite(x)) return;
-    if (fabs(x) > 64.0 * L) x = fmod(x, L);
-    while (x < -L / 2) x += L;
-    while (x >= L / 2) x -= L;
-    pos[t] = x;
+    if (isfinite(x)) {
+        if (fabs(x) > 64.0 * L) x = fmod(x, L);
+        while (x < -L / 2) x += L;
+        while (x >= L / 2) x -= L;
+        pos[t] = x;
+    }
+    if (o.pos) o.pos[t] = x;
 }
 
 // y = (L x) * scale for lower-triangular L, one wave per row
@@ -469,10 +479,11 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
 #pragma unroll
     for (int cb = 0; cb < KC / 16; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
     const double* Bblk = B + (size_t)i * BT * ld;
-    // global loads of tile j into registers (B tile and its xi block)
-    v2d vb[8];
-    double vx[KC / 4];
-    auto fetch = [&](int j) {
+    // global loads of tile j into registers (B tile and its xi block); two
+    // register sets, so the next two tiles are in flight during a product
+    v2d vb0[8], vb1[8];
+    double vx0[KC / 4], vx1[KC / 4];
+    auto fetch = [&](int j, v2d(&vb)[8], double(&vx)[KC / 4]) {
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int e = tid + it * 256;
@@ -487,8 +498,7 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             vx[it] = (k < K && c < n) ? ring[(size_t)slot * n + c] : 0.0;
         }
     };
-    fetch(j0);
-    for (int j = j0; j < j1; ++j) {
+    auto stage = [&](int j, const v2d(&vb)[8], const double(&vx)[KC / 4]) {
         __syncthreads();  // previous tile fully consumed
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
@@ -504,13 +514,26 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             sX[(e >> 6) * BSP + (e & 63)] = vx[it];
         }
         __syncthreads();
-        if (j + 1 < j1) fetch(j + 1);  // in flight during this tile's products
+    };
+    auto product = [&]() {
 #pragma unroll 4
         for (int c = 0; c < BT; c += 4) {
             const double a = sB[(16 * w + li) * BSP + c + lk];
 #pragma unroll
             for (int cb = 0; cb < KC / 16; ++cb)
                 acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sX[(16 * cb + li) * BSP + c + lk], acc[cb], 0, 0, 0);
+        }
+    };
+    fetch(j0, vb0, vx0);
+    if (j0 + 1 < j1) fetch(j0 + 1, vb1, vx1);
+    for (int j = j0; j < j1; j += 2) {
+        stage(j, vb0, vx0);
+        if (j + 2 < j1) fetch(j + 2, vb0, vx0);
+        product();
+        if (j + 1 < j1) {
+            stage(j + 1, vb1, vx1);
+            if (j + 3 < j1) fetch(j + 3, vb1, vx1);
+            product();
         }
     }
     // accumulator register v of lane group lk holds row lk + 4v, column li
@@ -616,47 +639,58 @@ __global__ __launch_bounds__(128) void cand_kernel(const double* __restrict__ po
 // candidate only if its current minimum-image distance is at most
 // S + thr + delta_i + delta_j (the torus distance moves by at most the two
 // displacements, and the reference test flags only pairs within S + thr);
-// other pairs skip the loop.
+// the workgroup lists those pairs in LDS and its threads then take the
+// (pair, candidate) tests together, every candidate position requested at
+// once.
 __global__ __launch_bounds__(256) void overlaps_batch_kernel(const double* __restrict__ cand,
                                                              const double* __restrict__ pos,
                                                              const double* __restrict__ delta,
                                                              const double* __restrict__ a, int N, int Nt, int K,
                                                              double L, double thr, const int* __restrict__ mask,
                                                              int* cflag) {
+    __shared__ int s_np;
+    __shared__ unsigned s_hit;
+    __shared__ int s_i[256], s_j[256];
     int bi, bj;
     pbx_pair_tile(blockIdx.x, (N + 15) / 16, bi, bj);
     const int i = bi * 16 + (threadIdx.x >> 4), j = bj * 16 + (threadIdx.x & 15);
-    bool near = false;
-    double ai = 0.0, aj = 0.0;
+    if (threadIdx.x == 0) {
+        s_np = 0;
+        s_hit = 0;
+    }
+    __syncthreads();
     // j < i and j < N: j moves; i moves when i < N.  mask: the reference's
     // connection matrix (bonded pairs are not tested)
     if (i < Nt && j < i && j < N && !(mask && mask[(size_t)j + (size_t)Nt * i])) {
-        ai = a[i];
-        aj = a[j];
         double s2 = 0.0;
         for (int c = 0; c < 3; ++c) {
             const double d = pbx_min_image(pos[3 * i + c] - pos[3 * j + c], L);
             s2 += d * d;
         }
-        const double reach = ai + aj + thr + (i < N ? delta[i] : 0.0) + delta[j];
+        const double reach = a[i] + a[j] + thr + (i < N ? delta[i] : 0.0) + delta[j];
         // a relative margin far above the rounding of these few operations
-        near = !(sqrt(s2) > reach * (1.0 + 1e-9) + 1e-9 * L);
-    }
-    if (!__any(near)) return;
-    const int n = 3 * N;
-    const bool fixed_i = i >= N;
-    unsigned m = 0;  // bit k: some pair of this wave overlaps in candidate k
-#pragma unroll 4
-    for (int k = 0; k < K; ++k) {
-        bool hit = false;
-        if (near) {
-            const double* C = cand + (size_t)k * n;
-            const double* Ri = fixed_i ? pos + 3 * i : C + 3 * i;
-            hit = pbx_pair_overlaps(Ri[0] - C[3 * j], Ri[1] - C[3 * j + 1], Ri[2] - C[3 * j + 2], ai, aj, L, thr);
+        if (!(sqrt(s2) > reach * (1.0 + 1e-9) + 1e-9 * L)) {
+            const int q = atomicAdd(&s_np, 1);
+            s_i[q] = i;
+            s_j[q] = j;
         }
-        if (__any(hit)) m |= 1u << k;
     }
-    if (m && (threadIdx.x & 63) == 0) atomicOr((unsigned*)cflag, m);
+    __syncthreads();
+    const int np_ = s_np;
+    if (np_ == 0) return;
+    const int n = 3 * N;
+    unsigned m = 0;  // bit k: some listed pair overlaps in candidate k
+    for (int it = threadIdx.x; it < np_ * K; it += 256) {
+        const int q = it / K, k = it - q * K;
+        const int pi = s_i[q], pj = s_j[q];
+        const double* C = cand + (size_t)k * n;
+        const double* Ri = pi >= N ? pos + 3 * pi : C + 3 * pi;
+        if (pbx_pair_overlaps(Ri[0] - C[3 * pj], Ri[1] - C[3 * pj + 1], Ri[2] - C[3 * pj + 2], a[pi], a[pj], L, thr))
+            m |= 1u << k;
+    }
+    if (m) atomicOr(&s_hit, m);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_hit) atomicOr((unsigned*)cflag, s_hit);
 }
 
 // Apply the first candidate without an overlap (or the chain end when all K
@@ -805,15 +839,16 @@ int pbx_launch_translate(const double* v, double sign, double* pos, int N, const
 int pbx_launch_wrap(double* pos, int N, double L, hipStream_t s) {
     PBX_CHECK_BOX(L);
     int n3 = 3 * N;
-    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L, nullptr, 0);
+    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L, nullptr, 0, PbxWrapOut{});
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
 
-int pbx_launch_wrap_if(double* pos, int N, double L, const int* sel, int K, hipStream_t s) {
+int pbx_launch_wrap_publish(double* pos, int N, double L, const int* sel, int K, const PbxWrapOut& o,
+                            hipStream_t s) {
     PBX_CHECK_BOX(L);
-    int n3 = 3 * N;
-    PBX_LAUNCH(wrap_kernel, dim3((n3 + 255) / 256), dim3(256), 0, s, pos, n3, L, sel, K);
+    const int n3 = 3 * N, nt = o.flux && o.nl > n3 ? o.nl : n3;
+    PBX_LAUNCH(wrap_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, pos, n3, L, sel, K, o);
     PBX_HIP(hipGetLastError());
     return PBX_OK;
 }
