@@ -355,7 +355,9 @@ int pbx_sim_last_attempts(pbx_sim* sim);
 int pbx_sim_sync(pbx_sim* sim);
 /* Stage timing with HIP events on the simulation's stream (no extra sync
  * when off).  get_timing fills stage_ms[3] = cumulative milliseconds in
- * assembly (+lubrication), Cholesky, propagation; returns timed steps. */
+ * assembly (+lubrication), Cholesky, propagation; returns timed steps.
+ * on = 1 records every stage bound (six events per step); on = 2 only the
+ * two around the factorisation (stage_ms[0] and [2] stay zero). */
 int pbx_sim_set_timing(pbx_sim* sim, int on);
 long long pbx_sim_get_timing(pbx_sim* sim, double* stage_ms);
 

@@ -664,12 +664,12 @@ bool is_smith(const PbxSim* s) {
 }
 
 // Mobility (optionally Ewald) times `scale`, written into M.
-int assemble_M(PbxSim* s, double* M, double scale) {
+int assemble_M(PbxSim* s, double* M, double scale, const int* sel = nullptr, int selK = 0) {
     const pbx_config& c = s->cfg;
     if (is_smith(s))
         return pbx_launch_smith(s->pos, s->a, s->N, c.box_length, c.ewald_alpha, c.ewald_real, c.ewald_imag, scale,
-                                M, s->ld, 0, s->stream);
-    return pbx_launch_rpy(s->pos, s->a, s->N, c.box_length, scale, M, s->ld, 0, s->stream);
+                                M, s->ld, 0, s->stream, sel, selK);
+    return pbx_launch_rpy(s->pos, s->a, s->N, c.box_length, scale, M, s->ld, 0, s->stream, sel, selK);
 }
 
 // Queue the factorisation status word for the next synchronisation (no
@@ -1711,7 +1711,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
         int nl = cf.n_labels > 0 ? cf.n_labels : 1;
         PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
     }
-    if (s->timing) PBX_HIP(hipEventRecord(s->ev[0], s->stream));
+    if (s->timing == 1) PBX_HIP(hipEventRecord(s->ev[0], s->stream));
     const int spec = s->spec_ok;
     s->spec_ok = 0;
     if (cf.hydrodynamics != PBX_NOHI) {
@@ -1754,6 +1754,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     // box.py:190: forces from the positions at the start of the step
     if (s->forces_on) TRY(pbx_launch_forces(s->fa, s->pos, s->F, s->e_part, s->E_dev, s->stream));
     const bool timed = s->timing && cf.hydrodynamics != PBX_NOHI;
+    const bool staged = timed && s->timing == 1;  // all stage bounds, not just the factorisation's
     const bool ahead = s->Dspec && s->lookahead && !s->graphs && build_Dff_flag && s->next_builds != 0 &&
                        cf.hydrodynamics != PBX_NOHI;
     const int prev = s->ahead_pending, q = prev == 0 ? 1 : 0;
@@ -1774,12 +1775,12 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     }
     const std::function<int(const int*, int)> tail = [&](const int* sel, int K) -> int {
         TRY(pbx_launch_wrap_publish(s->pos, s->N, cf.box_length, sel, K, wo, s->stream));
-        if (timed) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
+        if (staged) PBX_HIP(hipEventRecord(s->ev[3], s->stream));
         if (ahead) {
             PBX_HIP(hipEventRecord(s->done_ev, s->stream));
-            if (s->timing) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
-            TRY(assemble_M(s, s->Dspec, s->kBT * 1e19 / cf.viscosity));  // as build_Dff
-            if (s->timing) PBX_HIP(hipEventRecord(s->ev[5 + 2 * q], s->stream));
+            if (s->timing == 1) PBX_HIP(hipEventRecord(s->ev[4 + 2 * q], s->stream));
+            TRY(assemble_M(s, s->Dspec, s->kBT * 1e19 / cf.viscosity, sel, K));  // as build_Dff
+            if (s->timing == 1) PBX_HIP(hipEventRecord(s->ev[5 + 2 * q], s->stream));
         }
         return PBX_OK;
     };
@@ -1804,20 +1805,22 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     }
     if (timed) {  // the events are complete: read them after the one synchronisation
         float t01 = 0, t12 = 0, t23 = 0;
-        PBX_HIP(hipEventElapsedTime(&t01, s->ev[0], s->ev[1]));
         PBX_HIP(hipEventElapsedTime(&t12, s->ev[1], s->ev[2]));
-        PBX_HIP(hipEventElapsedTime(&t23, s->ev[2], s->ev[3]));
+        if (staged) {
+            PBX_HIP(hipEventElapsedTime(&t01, s->ev[0], s->ev[1]));
+            PBX_HIP(hipEventElapsedTime(&t23, s->ev[2], s->ev[3]));
+        }
         s->stage_ms[0] += t01;
         s->stage_ms[1] += t12;
         s->stage_ms[2] += t23;
         s->timed_steps += 1;
     }
-    if (s->timing && prev >= 0) {  // the last step's look-ahead assembly preceded this step's kernels
+    if (s->timing == 1 && prev >= 0) {  // the last step's look-ahead assembly preceded this step's kernels
         float ta = 0;
         PBX_HIP(hipEventElapsedTime(&ta, s->ev[4 + 2 * prev], s->ev[5 + 2 * prev]));
         s->stage_ms[0] += ta;
     }
-    s->ahead_pending = (ahead && s->timing) ? q : -1;
+    s->ahead_pending = (ahead && s->timing == 1) ? q : -1;
     const int rc = take_info(s);
     if (rc == PBX_OK && ahead) s->spec_ok = 1;
     return rc;
@@ -2245,6 +2248,7 @@ extern "C" long long pbx_sim_device_bytes(pbx_sim* s) { return s ? (long long)s-
 extern "C" int pbx_sim_set_timing(pbx_sim* s, int on) {
     if (on && !s->ev[0])
         for (int k = 0; k < 8; ++k) PBX_HIP(hipEventCreate(&s->ev[k]));
+    if (on < 0 || on > 2) return PBX_ERR_ARG;
     s->timing = on;
     s->ahead_pending = -1;
     s->stage_ms[0] = s->stage_ms[1] = s->stage_ms[2] = 0.0;

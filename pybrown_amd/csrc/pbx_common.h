@@ -124,10 +124,12 @@ static inline bool pbx_box_ok(double L) { return L > 0.0 && L < 1e300; }
 // launchers shared across translation units
 
 // pair assembly (pbx_pairs.hip)
+// sel/selK: skip the launch on the device when *sel >= selK (null: never)
 int pbx_launch_rpy(const double* xyz, const double* a, int N, double L, double scale, double* M, int ld,
-                   int write_pad, hipStream_t s);
+                   int write_pad, hipStream_t s, const int* sel = nullptr, int selK = 0);
 int pbx_launch_smith(const double* xyz, const double* a, int N, double L, double alpha, int m_max, int n_max,
-                     double scale, double* M, int ld, int write_pad, hipStream_t s);
+                     double scale, double* M, int ld, int write_pad, hipStream_t s, const int* sel = nullptr,
+                     int selK = 0);
 int pbx_launch_rpy_packed(const double* a, const double* P, int N, double* M, int ld, hipStream_t s);
 int pbx_launch_smith_packed(const double* a, const double* P, int N, double L, double alpha, int m_max, int n_max,
                             double* M, int ld, hipStream_t s);

@@ -296,6 +296,11 @@ struct GeomXYZ {
     const double* xyz;
     double L;
     int N;
+    // optional guard: the whole launch is skipped when *sel >= selK (a batch
+    // of move attempts that accepted no move leaves the positions unchanged)
+    const int* sel = nullptr;
+    int selK = 0;
+    __device__ bool skipped() const { return sel && *sel >= selK; }
     // separation vector the reference hands to its pair routines for i > j:
     // -wrap(r_i - r_j) (bead.c:42-89 then diffusion.py:40)
     __device__ void sep(int i, int j, double& x, double& y, double& z) const {
@@ -308,6 +313,7 @@ struct GeomXYZ {
 struct GeomPacked {
     const double* P;
     int N;
+    __device__ bool skipped() const { return false; }
     __device__ void sep(int i, int j, double& x, double& y, double& z) const {
         // pair (j < i) sits at results_position(i-1, j, N-1) (diff_tensor.c:107)
         size_t s = (size_t)j * (size_t)(N - 1) - (size_t)j * (size_t)(j + 1) / 2 + (size_t)(i - 1);
@@ -326,6 +332,7 @@ __global__ __launch_bounds__(256) void assemble_tile_kernel(Geom g, const double
     __shared__ double tile[TE * TLD];
     __shared__ RecipCoef tab[RECIP_TAB];
     const int N = g.N;
+    if (g.skipped()) return;
     int bi, bj;
     tile_decode(blockIdx.x, bi, bj);
     if (KIND == K_SMITH) recip_table_fill(tab, n_max, alpha);
@@ -451,10 +458,10 @@ inline int tiles_for(int N) {
 // launchers
 
 int pbx_launch_rpy(const double* xyz, const double* a, int N, double L, double scale, double* M, int ld, int,
-                   hipStream_t s) {
+                   hipStream_t s, const int* sel, int selK) {
     if (N <= 0) return PBX_OK;
     PBX_CHECK_BOX(L);
-    GeomXYZ g{xyz, L, N};
+    GeomXYZ g{xyz, L, N, sel, selK};
     PBX_LAUNCH((assemble_tile_kernel<K_RPY, GeomXYZ>), dim3(tiles_for(N)), dim3(256), 0, s, g, a, L, 0.0, 0,
                        0, scale, M, ld);
     PBX_HIP(hipGetLastError());
@@ -462,14 +469,14 @@ int pbx_launch_rpy(const double* xyz, const double* a, int N, double L, double s
 }
 
 int pbx_launch_smith(const double* xyz, const double* a, int N, double L, double alpha, int m_max, int n_max,
-                     double scale, double* M, int ld, int, hipStream_t s) {
+                     double scale, double* M, int ld, int, hipStream_t s, const int* sel, int selK) {
     if (N <= 0) return PBX_OK;
     PBX_CHECK_BOX(L);
     if (n_max < 0 || n_max > 8 || m_max < 0 || m_max > 32) {
         pbx_set_error(PBX_ERR_ARG, "Ewald truncation out of range (m_max=%d <= 32, n_max=%d <= 8)", m_max, n_max);
         return PBX_ERR_ARG;
     }
-    GeomXYZ g{xyz, L, N};
+    GeomXYZ g{xyz, L, N, sel, selK};
     launch_smith(g, a, L, alpha, m_max, n_max, scale, M, ld, tiles_for(N), s);
     PBX_HIP(hipGetLastError());
     return PBX_OK;

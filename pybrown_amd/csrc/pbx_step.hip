@@ -479,11 +479,10 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
 #pragma unroll
     for (int cb = 0; cb < KC / 16; ++cb) acc[cb] = (v4d){0.0, 0.0, 0.0, 0.0};
     const double* Bblk = B + (size_t)i * BT * ld;
-    // global loads of tile j into registers (B tile and its xi block); two
-    // register sets, so the next two tiles are in flight during a product
-    v2d vb0[8], vb1[8];
-    double vx0[KC / 4], vx1[KC / 4];
-    auto fetch = [&](int j, v2d(&vb)[8], double(&vx)[KC / 4]) {
+    // global loads of tile j into registers (B tile and its xi block)
+    v2d vb[8];
+    double vx[KC / 4];
+    auto fetch = [&](int j) {
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
             const int e = tid + it * 256;
@@ -498,7 +497,8 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             vx[it] = (k < K && c < n) ? ring[(size_t)slot * n + c] : 0.0;
         }
     };
-    auto stage = [&](int j, const v2d(&vb)[8], const double(&vx)[KC / 4]) {
+    fetch(j0);
+    for (int j = j0; j < j1; ++j) {
         __syncthreads();  // previous tile fully consumed
 #pragma unroll
         for (int it = 0; it < 8; ++it) {
@@ -514,26 +514,13 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
             sX[(e >> 6) * BSP + (e & 63)] = vx[it];
         }
         __syncthreads();
-    };
-    auto product = [&]() {
+        if (j + 1 < j1) fetch(j + 1);  // in flight during this tile's products
 #pragma unroll 4
         for (int c = 0; c < BT; c += 4) {
             const double a = sB[(16 * w + li) * BSP + c + lk];
 #pragma unroll
             for (int cb = 0; cb < KC / 16; ++cb)
                 acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sX[(16 * cb + li) * BSP + c + lk], acc[cb], 0, 0, 0);
-        }
-    };
-    fetch(j0, vb0, vx0);
-    if (j0 + 1 < j1) fetch(j0 + 1, vb1, vx1);
-    for (int j = j0; j < j1; j += 2) {
-        stage(j, vb0, vx0);
-        if (j + 2 < j1) fetch(j + 2, vb0, vx0);
-        product();
-        if (j + 1 < j1) {
-            stage(j + 1, vb1, vx1);
-            if (j + 3 < j1) fetch(j + 3, vb1, vx1);
-            product();
         }
     }
     // accumulator register v of lane group lk holds row lk + 4v, column li
