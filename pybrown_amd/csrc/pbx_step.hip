@@ -458,18 +458,20 @@ constexpr int KC_MAX = 32;    // candidates per batch
 constexpr int SLAB_MAX = 12;  // column slabs of the partial products (pbx_attempt_slab_width)
 
 // Partial products P[s][r][k] = sum_{c in slab s, c <= r} B[r][c] xi_k[c] for
-// row block i = blockIdx.x and column slab s = blockIdx.y (S tiles wide).
+// row block i = blockIdx.x, column slab s = blockIdx.y (S tiles wide) and
+// candidates KC * blockIdx.z .. + KC - 1 (P holds KCP candidates per row).
 // Wave w computes rows 16w..16w+15 of the block for all KC candidates with
 // v_mfma_f64_16x16x4_f64 (A = B rows from LDS, B operand = xi tile from LDS).
 template <int KC>
 __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restrict__ B, int ld, int np, int n,
                                                           const double* __restrict__ ring, int ring_pos, int ring_att,
-                                                          int K, int S, double* __restrict__ P) {
+                                                          int K, int S, int KCP, double* __restrict__ P) {
     typedef double v4d __attribute__((ext_vector_type(4)));
     typedef double v2d __attribute__((ext_vector_type(2)));
     __shared__ double sB[BT * BSP];
     __shared__ double sX[KC * BSP];
     const int i = blockIdx.x, sl = blockIdx.y;
+    const int kz = blockIdx.z * KC;  // first candidate of this workgroup
     const int j0 = sl * S;
     if (j0 > i) return;
     const int j1 = min(j0 + S, i + 1);
@@ -491,7 +493,7 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
 #pragma unroll
         for (int it = 0; it < KC / 4; ++it) {
             const int e = tid + it * 256;
-            const int k = e >> 6, c = j * BT + (e & 63);
+            const int k = kz + (e >> 6), c = j * BT + (e & 63);
             int slot = ring_pos + k;
             if (slot >= ring_att) slot -= ring_att;
             vx[it] = (k < K && c < n) ? ring[(size_t)slot * n + c] : 0.0;
@@ -524,13 +526,13 @@ __global__ __launch_bounds__(256) void stoch_batch_kernel(const double* __restri
         }
     }
     // accumulator register v of lane group lk holds row lk + 4v, column li
-    double* Ps = P + (size_t)sl * np * KC;
+    double* Ps = P + (size_t)sl * np * KCP + kz;
 #pragma unroll
     for (int cb = 0; cb < KC / 16; ++cb)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = i * BT + 16 * w + lk + 4 * v;
-            Ps[(size_t)r * KC + 16 * cb + li] = acc[cb][v];
+            Ps[(size_t)r * KCP + 16 * cb + li] = acc[cb][v];
         }
 }
 
@@ -963,13 +965,11 @@ int pbx_launch_attempt_batch(const double* B, const double* Bd, int ld, int np, 
             pbx_set_error(PBX_ERR_ARG, "attempt batch: %d column slabs exceed %d", (T + S - 1) / S, SLAB_MAX);
             return PBX_ERR_ARG;
         }
-        dim3 grid(T, (T + S - 1) / S);
-        if (KC == 16)
-            PBX_LAUNCH(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
-                               K, S, P);
-        else
-            PBX_LAUNCH(stoch_batch_kernel<32>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att,
-                               K, S, P);
+        // 16 candidates per workgroup: a batch of 32 runs as two workgroup
+        // sets (twice the workgroups in flight over the same B tiles)
+        dim3 grid(T, (T + S - 1) / S, KC / 16);
+        PBX_LAUNCH(stoch_batch_kernel<16>, grid, dim3(256), 0, s, B, ld, np, n, ring, ring_pos, ring_att, K, S, KC,
+                   P);
         PBX_HIP(hipGetLastError());
     }
     double* delta = cand + (size_t)(K + 1) * n;
