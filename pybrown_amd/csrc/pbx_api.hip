@@ -2246,8 +2246,10 @@ extern "C" long long pbx_sim_device_bytes(pbx_sim* s) { return s ? (long long)s-
 // [0] mobility assembly (+ lubrication and inverses), [1] Cholesky,
 // [2] propagation (noise products, overlap checks, flux, wrap).
 extern "C" int pbx_sim_set_timing(pbx_sim* s, int on) {
+    // timing only: no system-scope fence (no L2 writeback and invalidation
+    // on the stream at each mark); the host never waits on these events
     if (on && !s->ev[0])
-        for (int k = 0; k < 8; ++k) PBX_HIP(hipEventCreate(&s->ev[k]));
+        for (int k = 0; k < 8; ++k) PBX_HIP(hipEventCreateWithFlags(&s->ev[k], hipEventDisableSystemFence));
     if (on < 0 || on > 2) return PBX_ERR_ARG;
     s->timing = on;
     s->ahead_pending = -1;
