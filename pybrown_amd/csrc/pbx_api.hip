@@ -917,7 +917,7 @@ int noise_reset(PbxSim* s) {
 }
 
 // Queue the upload of chunk q into device slot q % 2.
-int noise_take(PbxSim* s, long long q) {
+int noise_take(PbxSim* s, long long q, bool readers_done = false) {
     double* hb = s->xi_host + (q % 3) * chunk_doubles(s);
     if (s->native) {
         std::unique_lock<std::mutex> lk(s->mu);
@@ -933,10 +933,12 @@ int noise_take(PbxSim* s, long long q) {
             return PBX_ERR_STATE;
         }
     }
-    // after the stream's readers of the slot's previous chunk, before its
-    // next readers
-    PBX_HIP(hipEventRecord(s->rd_ev, s->stream));
-    PBX_HIP(hipStreamWaitEvent(s->up, s->rd_ev, 0));
+    // after the stream's readers of the slot's previous chunk (already
+    // complete when the caller says so), before its next readers
+    if (!readers_done) {
+        PBX_HIP(hipEventRecord(s->rd_ev, s->stream));
+        PBX_HIP(hipStreamWaitEvent(s->up, s->rd_ev, 0));
+    }
     PBX_HIP(hipMemcpyAsync(s->xi_dev + (q % 2) * chunk_doubles(s), hb, sizeof(double) * chunk_doubles(s),
                            hipMemcpyHostToDevice, s->up));
     PBX_HIP(hipEventRecord(s->xi_ev[q % 3], s->up));
@@ -956,6 +958,22 @@ int noise_ensure(PbxSim* s, long long K) {
     const long long q1 = (s->xi_used + K - 1) / s->xi_cap;
     while (s->xi_taken <= q1) TRY(noise_take(s, s->xi_taken));
     return PBX_OK;
+}
+
+// Upload the next chunk ahead of need, from the end of a step whose kernels
+// have all completed except a look-ahead assembly (which reads no noise):
+// when every attempt of the chunk it overwrites is consumed and the producer
+// has it ready, its copy overlaps that assembly instead of stalling a later
+// attempt batch.
+int noise_prefetch(PbxSim* s) {
+    if (!s->native) return PBX_OK;
+    const long long q = s->xi_taken;
+    if (s->xi_used < (q - 1) * s->xi_cap) return PBX_OK;  // slot q % 2 still holds unconsumed attempts
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->xi_produced <= q) return PBX_OK;
+    }
+    return noise_take(s, q, true);
 }
 
 int ring_attempts(const PbxSim* s) { return (int)(2 * s->xi_cap); }
@@ -1823,6 +1841,8 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     s->ahead_pending = (ahead && s->timing == 1) ? q : -1;
     const int rc = take_info(s);
     if (rc == PBX_OK && ahead) s->spec_ok = 1;
+    // the wait above covered every reader of the ring (the move's kernels)
+    if (rc == PBX_OK && cf.scheme == PBX_ERMAK) TRY(noise_prefetch(s));
     return rc;
 }
 
