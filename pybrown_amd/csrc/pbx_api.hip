@@ -500,6 +500,11 @@ struct PbxSim {
     double *w = nullptr, *xipad = nullptr, *u = nullptr;
     int* label = nullptr;
     long long* flux = nullptr;
+    // the other per-step counter buffer: the wrap kernel of a step clears it
+    // (flux_alt_zero), and the next step swaps it in instead of clearing
+    // `flux` with a fill of its own
+    long long* flux_alt = nullptr;
+    int flux_alt_zero = 0;
     int *flag = nullptr, *info = nullptr;
     double *D = nullptr, *B = nullptr, *Rff = nullptr, *X = nullptr, *Dmid = nullptr, *RffMid = nullptr;
     double *Wchol = nullptr, *Winv = nullptr;
@@ -1291,6 +1296,7 @@ extern "C" int pbx_sim_create(const pbx_config* cfg, int N, const double* radii,
     if ((rc = sim_alloc(s, (void**)&s->label, sizeof(int) * N))) return fail(rc);
     int nl = cfg->n_labels > 0 ? cfg->n_labels : 1;
     if ((rc = sim_alloc(s, (void**)&s->flux, sizeof(long long) * nl))) return fail(rc);
+    if ((rc = sim_alloc(s, (void**)&s->flux_alt, sizeof(long long) * nl))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->flag, sizeof(int) * 2))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->info, sizeof(int)))) return fail(rc);
     if ((rc = sim_alloc(s, (void**)&s->xi_dev, sizeof(double) * 2 * s->xi_cap * n3))) return fail(rc);
@@ -1723,12 +1729,17 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     c.pn = cf.flux_normal;
     c.pp = cf.flux_plane_point;
     c.flux_on = cf.measure_flux;
-    c.flux = s->flux;
     c.queued = 0;
     if (cf.measure_flux) {
-        int nl = cf.n_labels > 0 ? cf.n_labels : 1;
-        PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
+        if (s->flux_alt_zero) {
+            std::swap(s->flux, s->flux_alt);  // cleared by the last step's wrap kernel
+        } else {
+            int nl = cf.n_labels > 0 ? cf.n_labels : 1;
+            PBX_HIP(hipMemsetAsync(s->flux, 0, sizeof(long long) * nl, s->stream));
+        }
     }
+    s->flux_alt_zero = 0;
+    c.flux = s->flux;
     if (s->timing == 1) PBX_HIP(hipEventRecord(s->ev[0], s->stream));
     const int spec = s->spec_ok;
     s->spec_ok = 0;
@@ -1785,6 +1796,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     if (cf.measure_flux) {
         wo.flux = s->flux_hd;
         wo.flux_dev = s->flux;
+        wo.flux_clear = s->flux_alt;
         wo.nl = cf.n_labels > 0 ? cf.n_labels : 1;
     }
     if (s->forces_on && s->E_hd) {
@@ -1841,6 +1853,7 @@ extern "C" int pbx_sim_propagate(pbx_sim* s, double dt, int build_Dff_flag, int 
     s->ahead_pending = (ahead && s->timing == 1) ? q : -1;
     const int rc = take_info(s);
     if (rc == PBX_OK && ahead) s->spec_ok = 1;
+    if (rc == PBX_OK && cf.measure_flux) s->flux_alt_zero = 1;
     // the wait above covered every reader of the ring (the move's kernels)
     if (rc == PBX_OK && cf.scheme == PBX_ERMAK) TRY(noise_prefetch(s));
     return rc;

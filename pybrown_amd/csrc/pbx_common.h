@@ -234,6 +234,7 @@ struct PbxWrapOut {
     const int* info;             // device status word
     long long* flux;             // per-label crossing counters
     const long long* flux_dev;
+    long long* flux_clear;       // device counters to clear for the next step (distinct from flux_dev)
     int nl;
     double* E;                   // force-field energy
     const double* E_dev;

@@ -153,7 +153,10 @@ __global__ void wrap_kernel(double* pos, int n3, double L, const int* __restrict
         if (o.status && o.info) o.status[1] = *o.info;
         if (o.E) *o.E = *o.E_dev;
     }
-    if (o.flux && t < o.nl) o.flux[t] = o.flux_dev[t];
+    if (o.flux && t < o.nl) {
+        o.flux[t] = o.flux_dev[t];
+        if (o.flux_clear) o.flux_clear[t] = 0;
+    }
     if (t >= n3 || (sel && *sel >= K)) return;
     double x = pos[t];
     if (isfinite(x)) {
